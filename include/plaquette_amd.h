@@ -1,0 +1,202 @@
+/*
+ * plaquette_amd.h -- C ABI of the MI355X (gfx950) batched stabilizer-tableau
+ * engine.  Plain pointers and sizes only; every entry point returns an int
+ * status (PQ_OK == 0) and never throws.  pq_last_error() gives the message
+ * of the calling thread's last failure.
+ *
+ * The engine replaces the device side of the reference's
+ * Plaquette::CliffordState<int> (plaquette_simulator/src/CliffordState.hpp).
+ * Each entry point names the reference member it stands in for.
+ *
+ * Data model.  A state holds n qubits and B shots.  Its tableau has
+ * R = 2n+1 rows (destabilizers 0..n-1, stabilizers n..2n-1, scratch row 2n)
+ * and 2n+1 columns per row (x_0..x_{n-1}, z_0..z_{n-1}, r).  It is stored as
+ * shot-sliced bit-planes: one uint64 word holds one tableau entry for 64
+ * consecutive shots (bit s of word w is shot 64w+s).  With W = ceil(B/64)
+ * the device layout is word[w][col][row], i.e. each 64-shot group owns a
+ * contiguous (2n+1) x R block and every column of it is R contiguous words.
+ * Host-side I/O uses the reference's flat int32 order
+ * x/z[b][row][col], r[b][row] (CliffordState.hpp:238-259).
+ *
+ * Random measurement outcomes use coins drawn from a counter-based
+ * SplitMix64 stream keyed by (seed, measurement sequence number, shot), or
+ * coins the caller supplies (replay).  A state's results therefore do not
+ * depend on launch shape, batching or the other work on the GPU.
+ */
+#ifndef PLAQUETTE_AMD_H
+#define PLAQUETTE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define PQ_OK 0
+#define PQ_ERR_ARG (-1)         /* invalid argument (qubit out of range, bad op) */
+#define PQ_ERR_RANGE (-2)       /* record id / shot index out of range */
+#define PQ_ERR_HIP (-3)         /* HIP runtime failure */
+#define PQ_ERR_OOM (-4)         /* device allocation failed or budget exceeded */
+#define PQ_ERR_NODEV (-5)       /* no usable GPU */
+#define PQ_ERR_UNSUPPORTED (-6) /* size beyond what a kernel supports */
+#define PQ_ERR_BUSY (-7)        /* serving: no capacity right now */
+
+/* ---- op codes (one pq_op per circuit operation) ---- */
+#define PQ_OP_H 1       /* ApplyHadamardGate          CliffordState.hpp:389 */
+#define PQ_OP_S 2       /* ApplyPhaseGate             CliffordState.hpp:467 */
+#define PQ_OP_X 3       /* ApplyPauliXGate            CliffordState.hpp:488 */
+#define PQ_OP_Y 4       /* ApplyPauliYGate            CliffordState.hpp:508 */
+#define PQ_OP_Z 5       /* ApplyPauliZGate            CliffordState.hpp:529 */
+#define PQ_OP_CX 6      /* ApplyControlNotGate(q0=control, q1=target) :412 */
+#define PQ_OP_CZ 7      /* ApplyControlPhaseGate(q0, q1)              :442 */
+#define PQ_OP_H_FLAG 8  /* ApplyHadamardGateWithApplyFlag, flag = record slot :348 */
+#define PQ_OP_S_FLAG 9  /* ApplyPhaseGateWithApplyFlag, flag = record slot    :365 */
+#define PQ_OP_NOISE 10  /* per-shot Pauli channel on q0 (no reference op; see below) */
+#define PQ_OP_X_FLAG 11 /* X on shots whose record `slot` result is 1 (reset) */
+#define PQ_OP_Y_FLAG 12 /* Y on those shots */
+#define PQ_OP_Z_FLAG 13 /* Z on those shots */
+#define PQ_OP_M 14      /* MeasureQubit(q0) into record `slot`       :587 */
+
+/*
+ * One circuit operation (32 bytes).
+ *  H..CZ:      q0 (control of CX/CZ), q1 (target of CX/CZ)
+ *  *_FLAG:     q0, slot = record whose result bit is the per-shot flag
+ *  NOISE:      q0; each shot draws u = draw(seed, 2k+1, shot) >> 32 for the
+ *              state's k-th noise op and gets X if u < t0, Y if u < t1,
+ *              Z if u < t2 (t0 <= t1 <= t2 are cumulative 32-bit thresholds)
+ *  M:          q0, slot = record id to write (existing or the next new id),
+ *              t0 = coin threshold: the state's m-th measurement gives a
+ *              random shot outcome 1 iff (draw(seed, 2m, shot) >> 40) < t0
+ *              (pq_coin_threshold(bias) reproduces frand(0,1) < bias)
+ */
+typedef struct pq_op {
+  uint32_t code, q0, q1, slot, t0, t1, t2, reserved;
+} pq_op;
+
+/* Execution forms of pq_run_program. */
+#define PQ_RUN_PLAIN 0      /* one kernel launch per op (reference style) */
+#define PQ_RUN_PERSISTENT 1 /* one persistent kernel for the whole op list */
+#define PQ_RUN_GRAPH 2      /* the persistent form captured once into a hipGraph and replayed */
+
+typedef struct pq_state_s *pq_state;
+
+/* ---- library / device ---- */
+const char *pq_last_error(void);
+/* GetKokkosBackend (CliffordState.hpp:280-282): e.g. "HIP(gfx950)". */
+const char *pq_backend_name(void);
+int pq_device_count(int *count);
+/* Coin threshold for a bias: ceil(bias * 2^24) clamped to [0, 2^24]. */
+uint32_t pq_coin_threshold(float bias);
+/* SplitMix64 counter draw shared by coins and noise (for tests and tools). */
+uint64_t pq_draw(uint64_t seed, uint64_t stream, uint64_t idx);
+
+/* ---- state lifetime ---- */
+/* CliffordState(num_qubits, batch_size, seed, init_to_zero)
+ * (CliffordState.hpp:97-139).  seed < 0 selects 213434232223 (:135). */
+int pq_create(uint64_t num_qubits, uint64_t batch_size, int64_t seed,
+              int init_to_zero, pq_state *out);
+/* CliffordState(x, z, r, n, B, seed) from flat host vectors (:153-159). */
+int pq_create_from_host(const int32_t *x, const int32_t *z, const int32_t *r,
+                        uint64_t num_qubits, uint64_t batch_size, int64_t seed,
+                        pq_state *out);
+/* Copy constructor (:168-174): copies the tableau and seed. */
+int pq_clone(pq_state src, pq_state *out);
+/* ~CliffordState (:665-688). */
+int pq_destroy(pq_state s);
+/* GetNumQubits / GetBatchSize / GetSeed (:289-303) plus the record count. */
+int pq_info(pq_state s, uint64_t *num_qubits, uint64_t *batch_size,
+            int64_t *seed, uint64_t *num_records);
+/* ResetCliffordState (:309-316): re-initialise to |0...0> (the intended
+ * semantics; the reference's {B,R,R} extent writes out of bounds). */
+int pq_reset(pq_state s);
+
+/* ---- gates (plain form: one launch each) ---- */
+/* Apply{Hadamard,Phase,PauliX,PauliY,PauliZ,ControlNot,ControlPhase}Gate
+ * (:379-531; BatchXxxGateFunctor in GateFunctors.hpp). */
+int pq_gate(pq_state s, uint32_t code, uint64_t q0, uint64_t q1);
+/* Apply{Hadamard,Phase}GateWithApplyFlag (:348-372) and conditional
+ * Paulis, with the flag taken from record `rec`'s result bits. */
+int pq_gate_flag_record(pq_state s, uint32_t code, uint64_t q, uint64_t rec);
+/* Same with a host flag vector int32[B] (nonzero = apply). */
+int pq_gate_flag_host(pq_state s, uint32_t code, uint64_t q,
+                      const int32_t *flags);
+/* Per-shot Pauli channel (PQ_OP_NOISE semantics); no reference counterpart:
+ * the reference reaches per-shot Paulis only through the flagged H/S API. */
+int pq_noise(pq_state s, uint64_t q, uint32_t t0, uint32_t t1, uint32_t t2);
+
+/* ---- measurement ---- */
+/* MeasureQubit(q, bias, measurement_id) (:587-623; BatchMeasureFunctor,
+ * MeasurementFunctors.hpp:267-318).  rec < 0 appends a new record, else
+ * overwrites record rec (PQ_ERR_RANGE if rec >= count, as :609-610).
+ * coins: optional int32[B] coin per shot (replay); NULL draws them.
+ * *out_rec receives the record id. */
+int pq_measure(pq_state s, uint64_t q, float bias, int64_t rec,
+               const int32_t *coins, uint64_t *out_rec);
+/* GetMeasurement(id, b) (:635-648): PQ_ERR_RANGE when id or b is out of
+ * range (the reference's nullopt). */
+int pq_get_measurement(pq_state s, uint64_t rec, uint64_t b, int32_t *result,
+                       int32_t *determined);
+/* Whole record as int32[B] vectors (Measurement::CopyToHost :39-57). */
+int pq_get_record(pq_state s, uint64_t rec, int32_t *results,
+                  int32_t *determined);
+
+/* ---- state I/O ---- */
+/* DeviceToHost (:238-259): int32 x[B*R*n], z[B*R*n], r[B*R]. */
+int pq_device_to_host(pq_state s, int32_t *x, int32_t *z, int32_t *r);
+/* HostToDevice (:207-231).  Entries are read as (v & 1). */
+int pq_host_to_device(pq_state s, const int32_t *x, const int32_t *z,
+                      const int32_t *r);
+/* DeviceToDevice (:268-274): dst's tableau := src's (same n, B). */
+int pq_copy_from(pq_state dst, pq_state src);
+/* CheckSum(b0, b1) (:327-338) with a 64-bit total; b1 < 0 means B. */
+int pq_checksum(pq_state s, int64_t b0, int64_t b1, int64_t *out);
+/* Raw bit-plane tableau, W*(2n+1)*R words in the device layout. */
+int pq_read_planes(pq_state s, uint64_t *host_words);
+/* Device handles (GetX/GetZ/GetR :180-198 return Kokkos Views; this gives
+ * the bit-plane base, record arena base, W, R and the state's hipStream_t). */
+int pq_device_handles(pq_state s, void **tableau, void **records,
+                      uint64_t *words, uint64_t *rows, void **stream);
+
+/* ---- programs (whole op lists) ----
+ * No single reference counterpart: the reference runs op lists through
+ * CliffordCircuit::Execute (CliffordCircuit.hpp:131-135), one kernel launch
+ * per op.  A program is validated and compiled once for a state's shape and
+ * can be run many times; each run continues the state's coin and noise
+ * sequence numbers exactly as the same ops issued one by one would.
+ * M ops write record `slot` <= current record count (== count appends a new
+ * record); flagged ops read records that exist at that point. */
+typedef struct pq_program_s *pq_program;
+int pq_program_create(pq_state s, const pq_op *ops, uint64_t nops,
+                      pq_program *out);
+int pq_program_run(pq_state s, pq_program p, int form);
+int pq_program_destroy(pq_program p);
+/* Logical gates (H..CZ), measurements, noise ops, device instructions. */
+int pq_program_info(pq_program p, uint64_t *gates, uint64_t *measurements,
+                    uint64_t *noise_ops, uint64_t *instructions);
+/* Launch shape of the persistent kernel (0 keeps the current value). */
+int pq_program_set_launch(pq_program p, uint32_t threads, uint32_t blocks);
+/* Create, run once (and wait), destroy. */
+int pq_run_program(pq_state s, const pq_op *ops, uint64_t nops, int form);
+/* Wait for all work queued on the state. */
+int pq_synchronize(pq_state s);
+
+/* ---- diagnostics: the reference's test-only functors ---- */
+/* RowMultiply(i <- i*k) on every shot (BatchRowMultiplyFunctor,
+ * MeasurementFunctors.hpp:168-187). */
+int pq_row_multiply(pq_state s, uint64_t i, uint64_t k);
+/* RowProductSign(i, k) per shot into out[B], tableau unchanged
+ * (BatchRowProductSignFunctor, MeasurementFunctors.hpp:131-154). */
+int pq_row_product_sign(pq_state s, uint64_t i, uint64_t k, int32_t *out);
+/* MeasureDetermined(q) on every shot into out[B], regardless of pivots
+ * (BatchMeasureDeterminedFunctor, MeasurementFunctors.hpp:344-364). */
+int pq_measure_determined(pq_state s, uint64_t q, int32_t *out);
+
+/* ---- timing on the state's stream (HIP events) ---- */
+int pq_event_record(pq_state s, int slot); /* slot in [0, 8) */
+int pq_event_elapsed_ms(pq_state s, int slot_a, int slot_b, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,258 @@
+// pybind11 module `plaquette_simulator_bindings`: the reference's Python
+// surface (plaquette_simulator/src/Bindings.cpp:11-31, same class and method
+// names) over the gfx950 engine, plus programs, records and timing helpers.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "CliffordState.hpp"
+#include "plaquette_amd.h"
+
+namespace py = pybind11;
+using State = Plaquette::CliffordState<int>;
+
+namespace {
+
+int form_of(const py::object &f) {
+  if (py::isinstance<py::int_>(f)) return f.cast<int>();
+  const std::string s = f.cast<std::string>();
+  if (s == "plain") return PQ_RUN_PLAIN;
+  if (s == "persistent") return PQ_RUN_PERSISTENT;
+  if (s == "graph") return PQ_RUN_GRAPH;
+  throw std::invalid_argument("form must be 'plain', 'persistent' or 'graph'");
+}
+
+const pq_op *ops_ptr(const py::array_t<uint32_t, py::array::c_style | py::array::forcecast> &a,
+                     uint64_t *n) {
+  if (a.size() == 0) {
+    *n = 0;
+    return nullptr;
+  }
+  if (a.ndim() != 2 || a.shape(1) != 8)
+    throw std::invalid_argument("ops must be an integer array of shape (N, 8)");
+  *n = (uint64_t)a.shape(0);
+  return reinterpret_cast<const pq_op *>(a.data());
+}
+
+void check(int rc, const char *what) {
+  if (rc != PQ_OK) throw Plaquette::Exception(std::string(what) + ": " + pq_last_error());
+}
+
+struct Program {
+  pq_program p = nullptr;
+  pq_state s = nullptr;
+  Program(State &st, py::array_t<uint32_t, py::array::c_style | py::array::forcecast> ops) {
+    uint64_t n = 0;
+    const pq_op *o = ops_ptr(ops, &n);
+    s = st.Handle();
+    check(pq_program_create(s, o, n, &p), "Program");
+  }
+  ~Program() {
+    if (p) {
+      pq_synchronize(s);
+      pq_program_destroy(p);
+    }
+  }
+};
+
+}  // namespace
+
+PYBIND11_MODULE(plaquette_simulator_bindings, m) {
+  m.doc() = "MI355X (gfx950) batched stabilizer-tableau engine";
+  py::register_exception<Plaquette::Exception>(m, "PlaquetteError", PyExc_RuntimeError);
+
+  py::class_<State>(m, "CliffordState")
+      .def(py::init<size_t, size_t, int, bool>(), py::arg("num_qubits"),
+           py::arg("batch_size"), py::arg("seed") = -1, py::arg("init_to_zero") = true)
+      .def_static(
+          "from_host",
+          [](std::vector<int> x, std::vector<int> z, std::vector<int> r, size_t n, size_t B,
+             long seed) { return new State(x, z, r, n, B, seed); },
+          py::arg("x"), py::arg("z"), py::arg("r"), py::arg("num_qubits"),
+          py::arg("batch_size"), py::arg("seed") = -1, py::return_value_policy::take_ownership)
+      .def("copy", [](const State &s) { return new State(s); },
+           py::return_value_policy::take_ownership)
+      // --- the reference's bound methods (Bindings.cpp:15-31) ---
+      .def("get_num_qubits", &State::GetNumQubits)
+      .def("get_batch_size", &State::GetBatchSize)
+      .def("get_seed", &State::GetSeed)
+      .def("reset_clifford_state", &State::ResetCliffordState)
+      .def(
+          "measure_qubit",
+          [](State &s, size_t q, float bias, std::optional<size_t> id,
+             std::optional<std::vector<int>> coins) {
+            return s.MeasureQubit(q, bias, id, coins ? &*coins : nullptr);
+          },
+          py::arg("target_qubit"), py::arg("bias") = 0.5f,
+          py::arg("measurement_id") = py::none(), py::arg("coins") = py::none())
+      .def("get_kokkos_backend", &State::GetKokkosBackend)
+      .def("get_measurement",
+           (std::optional<std::pair<int, int>>(State::*)(size_t, size_t)) &
+               State::GetMeasurement)
+      .def("apply_hadamard_gate", &State::ApplyHadamardGate)
+      .def("apply_control_not_gate", &State::ApplyControlNotGate)
+      .def("apply_control_phase_gate", &State::ApplyControlPhaseGate)
+      .def("apply_phase_gate", &State::ApplyPhaseGate)
+      .def("apply_pauli_x_gate", &State::ApplyPauliXGate)
+      .def("apply_pauli_z_gate", &State::ApplyPauliZGate)
+      .def("device_to_host", &State::DeviceToHost)
+      // --- superset ---
+      .def("apply_pauli_y_gate", &State::ApplyPauliYGate)
+      .def(
+          "apply_hadamard_gate_with_apply_flag",
+          [](State &s, size_t q, const py::object &flag) {
+            if (py::isinstance<py::int_>(flag))
+              s.ApplyHadamardGateWithApplyFlag(q, *s.GetMeasurement(flag.cast<size_t>())
+                                                        .measurement_results_device);
+            else
+              s.ApplyHadamardGateWithApplyFlag(q, flag.cast<std::vector<int>>());
+          },
+          py::arg("target_qubit"), py::arg("flag"),
+          "flag: a measurement id (its result bits) or a per-shot int sequence")
+      .def(
+          "apply_phase_gate_with_apply_flag",
+          [](State &s, size_t q, const py::object &flag) {
+            if (py::isinstance<py::int_>(flag))
+              s.ApplyPhaseGateWithApplyFlag(q, *s.GetMeasurement(flag.cast<size_t>())
+                                                    .measurement_results_device);
+            else
+              s.ApplyPhaseGateWithApplyFlag(q, flag.cast<std::vector<int>>());
+          },
+          py::arg("target_qubit"), py::arg("flag"))
+      .def(
+          "apply_pauli_with_apply_flag",
+          [](State &s, const std::string &kind, size_t q, size_t id) {
+            uint32_t code = kind == "X" ? PQ_OP_X_FLAG
+                            : kind == "Y" ? PQ_OP_Y_FLAG
+                            : kind == "Z" ? PQ_OP_Z_FLAG
+                                          : 0;
+            if (!code) throw std::invalid_argument("kind must be X, Y or Z");
+            s.ApplyPauliWithApplyFlag(code, q, *s.GetMeasurement(id).measurement_results_device);
+          },
+          py::arg("kind"), py::arg("target_qubit"), py::arg("measurement_id"))
+      .def("apply_pauli_noise", &State::ApplyPauliNoise, py::arg("target_qubit"),
+           py::arg("px"), py::arg("py"), py::arg("pz"))
+      .def("check_sum", &State::CheckSum, py::arg("starting_batch_id") = 0,
+           py::arg("ending_batch_id") = -1)
+      .def("device_to_host_numpy",
+           [](State &s) {
+             const size_t n = s.GetNumQubits(), B = s.GetBatchSize(), R = 2 * n + 1;
+             const bool empty = n == 0 || B == 0;
+             py::array_t<int32_t> x({empty ? 0 : B, R, n}), z({empty ? 0 : B, R, n}),
+                 r({empty ? 0 : B, R});
+             check(pq_device_to_host(s.Handle(), x.mutable_data(), z.mutable_data(),
+                                     r.mutable_data()),
+                   "device_to_host");
+             return py::make_tuple(x, z, r);
+           })
+      .def("host_to_device",
+           [](State &s, std::vector<int> x, std::vector<int> z, std::vector<int> r) {
+             s.HostToDevice(x, z, r);
+           })
+      .def("device_to_device", &State::DeviceToDevice)
+      .def("get_record",
+           [](State &s, size_t id) {
+             const size_t B = s.GetBatchSize();
+             py::array_t<int32_t> res(B), det(B);
+             check(pq_get_record(s.Handle(), id, res.mutable_data(), det.mutable_data()),
+                   "get_record");
+             return py::make_tuple(res, det);
+           })
+      .def("read_planes",
+           [](State &s) {
+             uint64_t W = 0, R = 0;
+             check(pq_device_handles(s.Handle(), nullptr, nullptr, &W, &R, nullptr), "planes");
+             const size_t n = s.GetNumQubits();
+             py::array_t<uint64_t> a({(size_t)W, 2 * n + 1, (size_t)R});
+             check(pq_read_planes(s.Handle(), a.mutable_data()), "read_planes");
+             return a;
+           })
+      .def_property_readonly("num_records", &State::GetNumMeasurements)
+      // --- diagnostics mirroring the reference's test-only functors ---
+      .def("row_multiply",
+           [](State &s, size_t i, size_t k) {
+             check(pq_row_multiply(s.Handle(), i, k), "row_multiply");
+           })
+      .def("row_product_sign",
+           [](State &s, size_t i, size_t k) {
+             py::array_t<int32_t> out(s.GetBatchSize());
+             check(pq_row_product_sign(s.Handle(), i, k, out.mutable_data()), "row_product_sign");
+             return out;
+           })
+      .def("measure_determined",
+           [](State &s, size_t q) {
+             py::array_t<int32_t> out(s.GetBatchSize());
+             check(pq_measure_determined(s.Handle(), q, out.mutable_data()), "measure_determined");
+             return out;
+           })
+      .def(
+          "run_program",
+          [](State &s, py::array_t<uint32_t, py::array::c_style | py::array::forcecast> ops,
+             const py::object &form) {
+            uint64_t n = 0;
+            const pq_op *o = ops_ptr(ops, &n);
+            check(pq_run_program(s.Handle(), o, n, form_of(form)), "run_program");
+            s.SyncRecords();
+          },
+          py::arg("ops"), py::arg("form") = "persistent")
+      .def("synchronize", [](State &s) { check(pq_synchronize(s.Handle()), "synchronize"); })
+      .def("sync_records", &State::SyncRecords)
+      .def("event_record",
+           [](State &s, int slot) { check(pq_event_record(s.Handle(), slot), "event_record"); })
+      .def("event_elapsed_ms", [](State &s, int a, int b) {
+        float ms = 0;
+        check(pq_event_elapsed_ms(s.Handle(), a, b, &ms), "event_elapsed_ms");
+        return ms;
+      });
+
+  py::class_<Program>(m, "Program")
+      .def(py::init<State &, py::array_t<uint32_t, py::array::c_style | py::array::forcecast>>(),
+           py::arg("state"), py::arg("ops"), py::keep_alive<1, 2>())
+      .def(
+          "run",
+          [](Program &p, State &s, const py::object &form) {
+            if (s.Handle() != p.s) throw std::invalid_argument("program belongs to another state");
+            check(pq_program_run(p.s, p.p, form_of(form)), "Program.run");
+          },
+          py::arg("state"), py::arg("form") = "persistent")
+      .def("info",
+           [](Program &p) {
+             uint64_t g = 0, mm = 0, nz = 0, ni = 0;
+             check(pq_program_info(p.p, &g, &mm, &nz, &ni), "info");
+             py::dict d;
+             d["gates"] = g;
+             d["measurements"] = mm;
+             d["noise_ops"] = nz;
+             d["instructions"] = ni;
+             return d;
+           })
+      .def("set_launch", [](Program &p, uint32_t threads, uint32_t blocks) {
+        check(pq_program_set_launch(p.p, threads, blocks), "set_launch");
+      });
+
+  m.def("coin_threshold", &pq_coin_threshold);
+  m.def("draw", &pq_draw);
+  m.def("backend_name", &pq_backend_name);
+  m.def("device_count", []() {
+    int c = 0;
+    pq_device_count(&c);
+    return c;
+  });
+  m.attr("OP_H") = PQ_OP_H;
+  m.attr("OP_S") = PQ_OP_S;
+  m.attr("OP_X") = PQ_OP_X;
+  m.attr("OP_Y") = PQ_OP_Y;
+  m.attr("OP_Z") = PQ_OP_Z;
+  m.attr("OP_CX") = PQ_OP_CX;
+  m.attr("OP_CZ") = PQ_OP_CZ;
+  m.attr("OP_H_FLAG") = PQ_OP_H_FLAG;
+  m.attr("OP_S_FLAG") = PQ_OP_S_FLAG;
+  m.attr("OP_NOISE") = PQ_OP_NOISE;
+  m.attr("OP_X_FLAG") = PQ_OP_X_FLAG;
+  m.attr("OP_Y_FLAG") = PQ_OP_Y_FLAG;
+  m.attr("OP_Z_FLAG") = PQ_OP_Z_FLAG;
+  m.attr("OP_M") = PQ_OP_M;
+}

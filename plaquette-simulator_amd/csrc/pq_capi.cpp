@@ -1,0 +1,613 @@
+// Host implementation of the C ABI declared in include/plaquette_amd.h.
+//
+// A state owns: the bit-plane tableau (W word groups of (2n+1) x (2n+1)
+// uint64 words), a record arena ([rec][2][W] words: results, determined),
+// one HIP stream (default priority) and small device scratch buffers.
+// Every entry point validates its arguments on the host before any launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "plaquette_amd.h"
+#include "pq_capi_internal.h"
+#include "pq_common.h"
+#include "pq_kernels.h"
+
+using namespace pq;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+}  // namespace
+
+int pq_set_error(int code, const char *msg) { return fail(code, msg); }
+
+#define HIPCHK(expr)                                                         \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail(PQ_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define ARGCHK(cond, msg) \
+  do {                    \
+    if (!(cond)) return fail(PQ_ERR_ARG, msg); \
+  } while (0)
+
+static constexpr int64_t kDefaultSeed = 213434232223LL;  // CliffordState.hpp:135
+
+// ---------------------------------------------------------------------------
+// library
+// ---------------------------------------------------------------------------
+extern "C" const char *pq_last_error(void) { return g_err.c_str(); }
+
+extern "C" const char *pq_backend_name(void) {
+  static std::string name;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+      std::string arch = prop.gcnArchName;
+      arch = arch.substr(0, arch.find(':'));
+      name = "HIP(" + arch + ")";
+    } else {
+      name = "HIP(unavailable)";
+    }
+  });
+  return name.c_str();
+}
+
+extern "C" int pq_device_count(int *count) {
+  ARGCHK(count, "count is null");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  *count = c;
+  return PQ_OK;
+}
+
+extern "C" uint32_t pq_coin_threshold(float bias) {
+  const double t = (double)bias * 16777216.0;
+  if (!(t > 0.0)) return 0;
+  if (t >= 16777216.0) return 16777216u;
+  return (uint32_t)std::ceil(t);
+}
+
+extern "C" uint64_t pq_draw(uint64_t seed, uint64_t stream, uint64_t idx) {
+  return draw(seed, stream, idx);
+}
+
+// ---------------------------------------------------------------------------
+// state
+// ---------------------------------------------------------------------------
+pq_state_s::~pq_state_s() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  if (tab) (void)hipFree(tab);
+  if (recs) (void)hipFree(recs);
+  if (scratch) (void)hipFree(scratch);
+  if (params_dev) (void)hipFree(params_dev);
+  if (params_host) (void)hipHostFree(params_host);
+  if (csum) (void)hipFree(csum);
+  if (stage) (void)hipFree(stage);
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+static int check_device() {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess || c < 1)
+    return fail(PQ_ERR_NODEV, "no HIP device available");
+  return PQ_OK;
+}
+
+static int alloc_state(uint64_t n, uint64_t B, int64_t seed, pq_state *out) {
+  ARGCHK(out, "output handle is null");
+  int rc = check_device();
+  if (rc) return rc;
+  if (n > 0xFFFFFFF0ull / 2) return fail(PQ_ERR_UNSUPPORTED, "too many qubits");
+  auto s = std::make_unique<pq_state_s>();
+  s->n = n;
+  s->B = B;
+  s->R = 2 * n + 1;
+  s->C = 2 * n + 1;
+  s->W = (B + 63) / 64;
+  s->seed = seed >= 0 ? seed : kDefaultSeed;
+  HIPCHK(hipGetDevice(&s->device));
+  HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  if (n > 0 && B > 0) {
+    const uint64_t words = s->W * s->C * s->R;
+    if (hipMalloc(&s->tab, words * 8) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(PQ_ERR_OOM, "tableau allocation failed (" +
+                                  std::to_string(words * 8) + " bytes)");
+    }
+  }
+  HIPCHK(hipMalloc(&s->scratch, std::max<uint64_t>(1, 3 * s->W) * 8));
+  HIPCHK(hipMalloc(&s->params_dev, 64));
+  HIPCHK(hipHostMalloc(&s->params_host, 64, hipHostMallocDefault));
+  HIPCHK(hipMalloc(&s->csum, 8));
+  HIPCHK(prepare_kernels((uint32_t)std::min<uint64_t>(n, measure_max_qubits())));
+  *out = s.release();
+  return PQ_OK;
+}
+
+Geom geom_of(const pq_state_s *s) {
+  return Geom{(uint32_t)s->n, (uint32_t)s->R, (uint32_t)s->C, s->W, s->B};
+}
+
+extern "C" int pq_create(uint64_t n, uint64_t B, int64_t seed, int init_zero,
+                         pq_state *out) {
+  int rc = alloc_state(n, B, seed, out);
+  if (rc) return rc;
+  pq_state s = *out;
+  if (s->tab) {
+    if (init_zero) {
+      HIPCHK(launch_fill_identity(s->stream, s->tab, geom_of(s)));
+    } else {
+      HIPCHK(hipMemsetAsync(s->tab, 0, s->W * s->C * s->R * 8, s->stream));
+    }
+  }
+  return PQ_OK;
+}
+
+extern "C" int pq_create_from_host(const int32_t *x, const int32_t *z,
+                                   const int32_t *r, uint64_t n, uint64_t B,
+                                   int64_t seed, pq_state *out) {
+  int rc = pq_create(n, B, seed, 0, out);
+  if (rc) return rc;
+  rc = pq_host_to_device(*out, x, z, r);
+  if (rc) {
+    pq_destroy(*out);
+    *out = nullptr;
+  }
+  return rc;
+}
+
+extern "C" int pq_destroy(pq_state s) {
+  delete s;
+  return PQ_OK;
+}
+
+extern "C" int pq_info(pq_state s, uint64_t *n, uint64_t *B, int64_t *seed,
+                       uint64_t *nrec) {
+  ARGCHK(s, "null state");
+  if (n) *n = s->n;
+  if (B) *B = s->B;
+  if (seed) *seed = s->seed;
+  if (nrec) *nrec = s->nrec;
+  return PQ_OK;
+}
+
+extern "C" int pq_reset(pq_state s) {
+  ARGCHK(s, "null state");
+  if (s->tab) HIPCHK(launch_fill_identity(s->stream, s->tab, geom_of(s)));
+  return PQ_OK;
+}
+
+extern "C" int pq_clone(pq_state src, pq_state *out) {
+  ARGCHK(src, "null state");
+  int rc = pq_create(src->n, src->B, src->seed, 0, out);
+  if (rc) return rc;
+  rc = pq_copy_from(*out, src);
+  if (rc) {
+    pq_destroy(*out);
+    *out = nullptr;
+  }
+  return rc;
+}
+
+extern "C" int pq_copy_from(pq_state dst, pq_state src) {
+  ARGCHK(dst && src, "null state");
+  ARGCHK(dst->n == src->n && dst->B == src->B, "DeviceToDevice needs equal n and B");
+  if (!dst->tab) return PQ_OK;
+  HIPCHK(hipStreamSynchronize(src->stream));
+  HIPCHK(hipMemcpyAsync(dst->tab, src->tab, dst->W * dst->C * dst->R * 8,
+                        hipMemcpyDeviceToDevice, dst->stream));
+  return PQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// records
+// ---------------------------------------------------------------------------
+int ensure_records(pq_state s, uint64_t count) {
+  if (count <= s->rec_cap) {
+    if (count > s->nrec) {
+      s->nrec = count;
+      s->rec_host.resize(count);
+      s->rec_valid.resize(count, 0);
+    }
+    return PQ_OK;
+  }
+  uint64_t cap = std::max<uint64_t>({count, 2 * s->rec_cap, 64});
+  uint64_t *nr = nullptr;
+  const uint64_t bytes = cap * 2 * std::max<uint64_t>(s->W, 1) * 8;
+  if (hipMalloc(&nr, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(PQ_ERR_OOM, "record arena allocation failed");
+  }
+  HIPCHK(hipMemsetAsync(nr, 0, bytes, s->stream));
+  if (s->recs) {
+    HIPCHK(hipMemcpyAsync(nr, s->recs, s->rec_cap * 2 * s->W * 8,
+                          hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipFree(s->recs));
+  }
+  s->recs = nr;
+  s->rec_cap = cap;
+  s->arena_gen++;
+  s->nrec = count;
+  s->rec_host.resize(count);
+  s->rec_valid.resize(count, 0);
+  return PQ_OK;
+}
+
+uint64_t *record_ptr(pq_state s, uint64_t id) { return s->recs + id * 2 * s->W; }
+
+static int fetch_record(pq_state s, uint64_t id) {
+  if (s->rec_valid[id]) return PQ_OK;
+  auto &h = s->rec_host[id];
+  h.resize(2 * s->W);
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (s->W)
+    HIPCHK(hipMemcpy(h.data(), record_ptr(s, id), 2 * s->W * 8, hipMemcpyDeviceToHost));
+  s->rec_valid[id] = 1;
+  return PQ_OK;
+}
+
+void invalidate_records(pq_state s) {
+  std::fill(s->rec_valid.begin(), s->rec_valid.end(), 0);
+}
+
+extern "C" int pq_get_measurement(pq_state s, uint64_t rec, uint64_t b,
+                                  int32_t *result, int32_t *determined) {
+  ARGCHK(s, "null state");
+  if (rec >= s->nrec || b >= s->B)
+    return fail(PQ_ERR_RANGE, "Measurement id or batch index is out of range.");
+  int rc = fetch_record(s, rec);
+  if (rc) return rc;
+  const auto &h = s->rec_host[rec];
+  if (result) *result = (int32_t)((h[b >> 6] >> (b & 63)) & 1);
+  if (determined) *determined = (int32_t)((h[s->W + (b >> 6)] >> (b & 63)) & 1);
+  return PQ_OK;
+}
+
+extern "C" int pq_get_record(pq_state s, uint64_t rec, int32_t *results,
+                             int32_t *determined) {
+  ARGCHK(s, "null state");
+  if (rec >= s->nrec) return fail(PQ_ERR_RANGE, "Measurement id is out of range.");
+  int rc = fetch_record(s, rec);
+  if (rc) return rc;
+  const auto &h = s->rec_host[rec];
+  for (uint64_t b = 0; b < s->B; ++b) {
+    if (results) results[b] = (int32_t)((h[b >> 6] >> (b & 63)) & 1);
+    if (determined) determined[b] = (int32_t)((h[s->W + (b >> 6)] >> (b & 63)) & 1);
+  }
+  return PQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// plain form: one launch per op
+// ---------------------------------------------------------------------------
+static bool is_gate(uint32_t c) { return c >= PQ_OP_H && c <= PQ_OP_CZ; }
+static bool is_two_qubit(uint32_t c) { return c == PQ_OP_CX || c == PQ_OP_CZ; }
+static bool is_flag_op(uint32_t c) {
+  return c == PQ_OP_H_FLAG || c == PQ_OP_S_FLAG || c == PQ_OP_X_FLAG ||
+         c == PQ_OP_Y_FLAG || c == PQ_OP_Z_FLAG;
+}
+
+int plain_gate(pq_state s, uint32_t code, uint64_t q0, uint64_t q1) {
+  ARGCHK(s, "null state");
+  ARGCHK(is_gate(code), "not a gate op code");
+  if (q0 >= s->n) return fail(PQ_ERR_ARG, "qubit index out of range");
+  if (is_two_qubit(code) && (q1 >= s->n || q1 == q0))
+    return fail(PQ_ERR_ARG, "two-qubit gate needs distinct qubits in range");
+  if (!s->tab) return PQ_OK;
+  HIPCHK(launch_gate(s->stream, s->tab, geom_of(s), code, (uint32_t)q0,
+                     (uint32_t)q1, nullptr, nullptr));
+  return PQ_OK;
+}
+
+extern "C" int pq_gate(pq_state s, uint32_t code, uint64_t q0, uint64_t q1) {
+  return plain_gate(s, code, q0, q1);
+}
+
+// Flagged op with a device flag plane (W words).
+static int zero_scratch_tail(pq_state s);
+
+int plain_flag_plane(pq_state s, uint32_t code, uint64_t q, const uint64_t *plane) {
+  if (!s->tab) return PQ_OK;
+  int rc = zero_scratch_tail(s);
+  if (rc) return rc;
+  const Geom g = geom_of(s);
+  switch (code) {
+    case PQ_OP_H_FLAG:
+    case PQ_OP_S_FLAG:
+      HIPCHK(launch_gate(s->stream, s->tab, g, code, (uint32_t)q, 0, plane, nullptr));
+      break;
+    case PQ_OP_X_FLAG:
+      HIPCHK(launch_gate(s->stream, s->tab, g, PQ_OP_NOISE, (uint32_t)q, 0, plane,
+                         s->scratch + 2 * s->W));
+      break;
+    case PQ_OP_Z_FLAG:
+      HIPCHK(launch_gate(s->stream, s->tab, g, PQ_OP_NOISE, (uint32_t)q, 0,
+                         s->scratch + 2 * s->W, plane));
+      break;
+    case PQ_OP_Y_FLAG:
+      HIPCHK(launch_gate(s->stream, s->tab, g, PQ_OP_NOISE, (uint32_t)q, 0, plane, plane));
+      break;
+    default:
+      return fail(PQ_ERR_ARG, "not a flagged op code");
+  }
+  return PQ_OK;
+}
+
+static int zero_scratch_tail(pq_state s) {
+  // scratch[2W .. 3W) is kept all-zero for the "no Z part" / "no X part" mask
+  if (!s->scratch_zeroed) {
+    HIPCHK(hipMemsetAsync(s->scratch + 2 * s->W, 0, std::max<uint64_t>(s->W, 1) * 8, s->stream));
+    s->scratch_zeroed = true;
+  }
+  return PQ_OK;
+}
+
+// Copy a host int32[B] vector to the device and turn it into a shot mask
+// plane (nonzero -> bit set).  Synchronous: the staging buffer is reused.
+static int stage_host_ints(pq_state s, const int32_t *v, uint64_t *plane) {
+  if (!s->stage) HIPCHK(hipMalloc(&s->stage, std::max<uint64_t>(s->B, 1) * 4));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpy(s->stage, v, s->B * 4, hipMemcpyHostToDevice));
+  HIPCHK(launch_flags_to_plane(s->stream, s->stage, s->B, plane));
+  return PQ_OK;
+}
+
+extern "C" int pq_gate_flag_record(pq_state s, uint32_t code, uint64_t q, uint64_t rec) {
+  ARGCHK(s, "null state");
+  ARGCHK(is_flag_op(code), "not a flagged op code");
+  if (q >= s->n) return fail(PQ_ERR_ARG, "qubit index out of range");
+  if (rec >= s->nrec) return fail(PQ_ERR_RANGE, "Measurement id is out of range.");
+  int rc = zero_scratch_tail(s);
+  if (rc) return rc;
+  return plain_flag_plane(s, code, q, record_ptr(s, rec));
+}
+
+extern "C" int pq_gate_flag_host(pq_state s, uint32_t code, uint64_t q,
+                                 const int32_t *flags) {
+  ARGCHK(s, "null state");
+  ARGCHK(is_flag_op(code), "not a flagged op code");
+  ARGCHK(flags || s->B == 0, "flags is null");
+  if (q >= s->n) return fail(PQ_ERR_ARG, "qubit index out of range");
+  if (!s->tab) return PQ_OK;
+  int rc = zero_scratch_tail(s);
+  if (rc) return rc;
+  rc = stage_host_ints(s, flags, s->scratch);
+  if (rc) return rc;
+  return plain_flag_plane(s, code, q, s->scratch);
+}
+
+int plain_noise(pq_state s, uint64_t q, uint32_t t0, uint32_t t1, uint32_t t2) {
+  ARGCHK(s, "null state");
+  if (q >= s->n) return fail(PQ_ERR_ARG, "qubit index out of range");
+  ARGCHK(t0 <= t1 && t1 <= t2, "noise thresholds must be cumulative");
+  const uint64_t k = s->noise_seq++;
+  if (!s->tab) return PQ_OK;
+  NoiseDesc d{stream_key((uint64_t)s->seed, 2 * k + 1), t0, t1, t2, 0};
+  HIPCHK(launch_noise_planes(s->stream, s->scratch, d, nullptr, 1, s->W));
+  HIPCHK(launch_gate(s->stream, s->tab, geom_of(s), PQ_OP_NOISE, (uint32_t)q, 0,
+                     s->scratch, s->scratch + s->W));
+  return PQ_OK;
+}
+
+extern "C" int pq_noise(pq_state s, uint64_t q, uint32_t t0, uint32_t t1, uint32_t t2) {
+  return plain_noise(s, q, t0, t1, t2);
+}
+
+int plain_measure(pq_state s, uint64_t q, uint32_t T, uint64_t rec,
+                  const uint64_t *coin_plane_dev) {
+  const uint64_t m = s->meas_seq++;
+  s->rec_valid[rec] = 0;
+  if (!s->tab) return PQ_OK;
+  if (s->n > measure_max_qubits())
+    return fail(PQ_ERR_UNSUPPORTED, "measurement supports at most 4095 qubits");
+  MeasParams mp{(uint32_t)q, T, stream_key((uint64_t)s->seed, 2 * m), coin_plane_dev,
+                record_ptr(s, rec), false};
+  HIPCHK(launch_measure(s->stream, s->tab, geom_of(s), mp));
+  return PQ_OK;
+}
+
+extern "C" int pq_measure(pq_state s, uint64_t q, float bias, int64_t rec,
+                          const int32_t *coins, uint64_t *out_rec) {
+  ARGCHK(s, "null state");
+  if (q >= s->n) return fail(PQ_ERR_ARG, "qubit index out of range");
+  uint64_t id;
+  if (rec < 0) {
+    id = s->nrec;
+    int rc = ensure_records(s, s->nrec + 1);
+    if (rc) return rc;
+  } else {
+    if ((uint64_t)rec >= s->nrec) return fail(PQ_ERR_RANGE, "Measurement id is out of range.");
+    id = (uint64_t)rec;
+  }
+  const uint64_t *cp = nullptr;
+  if (coins && s->B) {
+    int rc = stage_host_ints(s, coins, s->scratch + s->W);
+    if (rc) return rc;
+    cp = s->scratch + s->W;
+  }
+  int rc = plain_measure(s, q, pq_coin_threshold(bias), id, cp);
+  if (rc) return rc;
+  if (out_rec) *out_rec = id;
+  return PQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// state I/O
+// ---------------------------------------------------------------------------
+static uint64_t io_chunk_shots(const pq_state_s *s) {
+  const uint64_t per_shot = s->R * s->n * 8 + s->R * 4;  // x+z+r int32
+  uint64_t c = (512ull << 20) / std::max<uint64_t>(per_shot, 1);
+  c = std::max<uint64_t>(64, c - c % 64);
+  return c;
+}
+
+extern "C" int pq_device_to_host(pq_state s, int32_t *x, int32_t *z, int32_t *r) {
+  ARGCHK(s, "null state");
+  if (!s->tab) return PQ_OK;
+  ARGCHK(x && z && r, "output buffer is null");
+  const uint64_t chunk = std::min<uint64_t>(io_chunk_shots(s), s->B);
+  const uint64_t per = s->R * s->n;
+  int32_t *dx = nullptr, *dz = nullptr, *dr = nullptr;
+  HIPCHK(hipMalloc(&dx, chunk * per * 4));
+  HIPCHK(hipMalloc(&dz, chunk * per * 4));
+  HIPCHK(hipMalloc(&dr, chunk * s->R * 4));
+  int rc = PQ_OK;
+  for (uint64_t b0 = 0; b0 < s->B && rc == PQ_OK; b0 += chunk) {
+    const uint64_t nb = std::min(chunk, s->B - b0);
+    hipError_t e = launch_unpack(s->stream, s->tab, geom_of(s), b0, nb, dx, dz, dr);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) e = hipMemcpy(x + b0 * per, dx, nb * per * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(z + b0 * per, dz, nb * per * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(r + b0 * s->R, dr, nb * s->R * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(PQ_ERR_HIP, std::string("device_to_host: ") + hipGetErrorString(e));
+  }
+  (void)hipFree(dx);
+  (void)hipFree(dz);
+  (void)hipFree(dr);
+  return rc;
+}
+
+extern "C" int pq_host_to_device(pq_state s, const int32_t *x, const int32_t *z,
+                                 const int32_t *r) {
+  ARGCHK(s, "null state");
+  if (!s->tab) return PQ_OK;
+  ARGCHK(x && z && r, "input buffer is null");
+  const uint64_t chunk = std::min<uint64_t>(io_chunk_shots(s), 64 * s->W);
+  const uint64_t per = s->R * s->n;
+  int32_t *dx = nullptr, *dz = nullptr, *dr = nullptr;
+  HIPCHK(hipMalloc(&dx, chunk * per * 4));
+  HIPCHK(hipMalloc(&dz, chunk * per * 4));
+  HIPCHK(hipMalloc(&dr, chunk * s->R * 4));
+  int rc = PQ_OK;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  for (uint64_t b0 = 0; b0 < s->B && rc == PQ_OK; b0 += chunk) {
+    const uint64_t nb = std::min(chunk, s->B - b0);
+    hipError_t e = hipMemcpy(dx, x + b0 * per, nb * per * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dz, z + b0 * per, nb * per * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dr, r + b0 * s->R, nb * s->R * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = launch_pack(s->stream, s->tab, geom_of(s), b0 / 64, (nb + 63) / 64, dx, dz, dr);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) rc = fail(PQ_ERR_HIP, std::string("host_to_device: ") + hipGetErrorString(e));
+  }
+  (void)hipFree(dx);
+  (void)hipFree(dz);
+  (void)hipFree(dr);
+  return rc;
+}
+
+extern "C" int pq_checksum(pq_state s, int64_t b0, int64_t b1, int64_t *out) {
+  ARGCHK(s && out, "null argument");
+  if (b1 < 0) b1 = (int64_t)s->B;
+  ARGCHK(b0 >= 0 && b0 <= b1 && (uint64_t)b1 <= s->B, "shot range out of bounds");
+  *out = 0;
+  if (!s->tab || b0 == b1) return PQ_OK;
+  HIPCHK(hipMemsetAsync(s->csum, 0, 8, s->stream));
+  HIPCHK(launch_checksum(s->stream, s->tab, geom_of(s), (uint64_t)b0, (uint64_t)b1,
+                         (unsigned long long *)s->csum));
+  unsigned long long v = 0;
+  HIPCHK(hipMemcpyAsync(&v, s->csum, 8, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  *out = (int64_t)v;
+  return PQ_OK;
+}
+
+extern "C" int pq_read_planes(pq_state s, uint64_t *host_words) {
+  ARGCHK(s && host_words, "null argument");
+  if (!s->tab) return PQ_OK;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpy(host_words, s->tab, s->W * s->C * s->R * 8, hipMemcpyDeviceToHost));
+  return PQ_OK;
+}
+
+extern "C" int pq_device_handles(pq_state s, void **tableau, void **records,
+                                 uint64_t *words, uint64_t *rows, void **stream) {
+  ARGCHK(s, "null state");
+  if (tableau) *tableau = s->tab;
+  if (records) *records = s->recs;
+  if (words) *words = s->W;
+  if (rows) *rows = s->R;
+  if (stream) *stream = (void *)s->stream;
+  return PQ_OK;
+}
+
+extern "C" int pq_synchronize(pq_state s) {
+  ARGCHK(s, "null state");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return PQ_OK;
+}
+
+extern "C" int pq_event_record(pq_state s, int slot) {
+  ARGCHK(s && slot >= 0 && slot < 8, "bad event slot");
+  if (!s->ev[slot]) HIPCHK(hipEventCreate(&s->ev[slot]));
+  HIPCHK(hipEventRecord(s->ev[slot], s->stream));
+  return PQ_OK;
+}
+
+extern "C" int pq_event_elapsed_ms(pq_state s, int a, int b, float *ms) {
+  ARGCHK(s && ms && a >= 0 && a < 8 && b >= 0 && b < 8, "bad event slot");
+  ARGCHK(s->ev[a] && s->ev[b], "event slot not recorded");
+  HIPCHK(hipEventSynchronize(s->ev[b]));
+  HIPCHK(hipEventElapsedTime(ms, s->ev[a], s->ev[b]));
+  return PQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics mirroring the reference's test-only functors
+// ---------------------------------------------------------------------------
+extern "C" int pq_row_multiply(pq_state s, uint64_t i, uint64_t k) {
+  ARGCHK(s, "null state");
+  ARGCHK(i < s->R && k < s->R, "row out of range");
+  if (!s->tab) return PQ_OK;
+  HIPCHK(launch_rowsum(s->stream, s->tab, geom_of(s), (uint32_t)i, (uint32_t)k, nullptr));
+  return PQ_OK;
+}
+
+extern "C" int pq_row_product_sign(pq_state s, uint64_t i, uint64_t k, int32_t *out) {
+  ARGCHK(s && out, "null argument");
+  ARGCHK(i < s->R && k < s->R, "row out of range");
+  if (!s->tab) return PQ_OK;
+  HIPCHK(launch_rowsum(s->stream, s->tab, geom_of(s), (uint32_t)i, (uint32_t)k, s->scratch));
+  std::vector<uint64_t> h(s->W);
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpy(h.data(), s->scratch, s->W * 8, hipMemcpyDeviceToHost));
+  for (uint64_t b = 0; b < s->B; ++b) out[b] = (int32_t)((h[b >> 6] >> (b & 63)) & 1);
+  return PQ_OK;
+}
+
+extern "C" int pq_measure_determined(pq_state s, uint64_t q, int32_t *out) {
+  ARGCHK(s && out, "null argument");
+  if (q >= s->n) return fail(PQ_ERR_ARG, "qubit index out of range");
+  if (!s->tab) return PQ_OK;
+  uint64_t *rec = s->scratch;  // 2W words: results, determined (scratch is 3W)
+  MeasParams mp{(uint32_t)q, 0, 0, nullptr, rec, true};
+  HIPCHK(launch_measure(s->stream, s->tab, geom_of(s), mp));
+  std::vector<uint64_t> h(s->W);
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpy(h.data(), rec, s->W * 8, hipMemcpyDeviceToHost));
+  s->scratch_zeroed = false;
+  for (uint64_t b = 0; b < s->B; ++b) out[b] = (int32_t)((h[b >> 6] >> (b & 63)) & 1);
+  return PQ_OK;
+}

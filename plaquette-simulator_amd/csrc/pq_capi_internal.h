@@ -1,0 +1,43 @@
+// Internal state of the C ABI (shared by pq_capi.cpp and pq_program.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "plaquette_amd.h"
+#include "pq_kernels.h"
+
+struct pq_state_s {
+  uint64_t n = 0, B = 0, R = 0, C = 0, W = 0;
+  int64_t seed = 0;
+  int device = 0;
+  uint64_t *tab = nullptr;     // W * C * R words
+  uint64_t *recs = nullptr;    // rec_cap * 2 * W words
+  uint64_t nrec = 0, rec_cap = 0;
+  uint64_t arena_gen = 0;      // bumped whenever `recs` moves
+  uint64_t meas_seq = 0;       // measurements issued so far (coin streams)
+  uint64_t noise_seq = 0;      // noise ops issued so far (noise streams)
+  hipStream_t stream = nullptr;
+  uint64_t *scratch = nullptr;  // 3W words: flag/noise planes for plain ops
+  bool scratch_zeroed = false;
+  int32_t *stage = nullptr;     // int32[B] host-vector staging
+  void *params_dev = nullptr;   // ProgParams of the running program
+  void *params_host = nullptr;  // pinned mirror
+  void *csum = nullptr;
+  hipEvent_t ev[8] = {};
+  std::vector<std::vector<uint64_t>> rec_host;  // host copies of records
+  std::vector<char> rec_valid;
+  ~pq_state_s();
+};
+
+int pq_set_error(int code, const char *msg);
+pq::Geom geom_of(const pq_state_s *s);
+int ensure_records(pq_state s, uint64_t count);
+uint64_t *record_ptr(pq_state s, uint64_t id);
+int plain_gate(pq_state s, uint32_t code, uint64_t q0, uint64_t q1);
+int plain_flag_plane(pq_state s, uint32_t code, uint64_t q, const uint64_t *plane);
+int plain_noise(pq_state s, uint64_t q, uint32_t t0, uint32_t t1, uint32_t t2);
+int plain_measure(pq_state s, uint64_t q, uint32_t T, uint64_t rec,
+                  const uint64_t *coin_plane_dev);

@@ -1,0 +1,85 @@
+// Shared host/device primitives of the bit-plane tableau engine.
+//
+// Layout: a state with n qubits and B shots keeps W = ceil(B/64) word groups;
+// group w owns a contiguous block of C = 2n+1 columns x R = 2n+1 rows of
+// uint64 words (column c < n: x_c, n <= c < 2n: z_{c-n}, c = 2n: r), and bit
+// s of every word of group w belongs to shot 64w+s.  Word (w, c, row) lives at
+// tab[(w*C + c)*R + row].
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PQ_HD __host__ __device__ __forceinline__
+
+namespace pq {
+
+constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
+
+// SplitMix64 output function (Steele, Lea, Flood, "Fast splittable
+// pseudorandom number generators", OOPSLA 2014).
+PQ_HD uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Output idx+1 of a SplitMix64 generator whose state is output stream+1 of
+// a SplitMix64 generator seeded with `seed`.  Coins of the m-th measurement
+// use stream 2m, the k-th noise op uses stream 2k+1; idx is the shot.
+PQ_HD uint64_t stream_key(uint64_t seed, uint64_t stream) {
+  return mix64(seed + (stream + 1) * kGolden);
+}
+PQ_HD uint64_t draw_keyed(uint64_t key, uint64_t idx) {
+  return mix64(key + (idx + 1) * kGolden);
+}
+PQ_HD uint64_t draw(uint64_t seed, uint64_t stream, uint64_t idx) {
+  return draw_keyed(stream_key(seed, stream), idx);
+}
+
+// Bit-sliced mod-4 accumulation of the Pauli-product phase exponent
+// g(dest, src) (MeasurementFunctors.hpp:10-72; destination first, as
+// RowProductSign calls it at :112-113).  Per shot bit: the Paulis
+// anticommute iff anti = x1 z2 ^ z1 x2; then g = +1 for (X,Y),(Y,Z),(Z,X)
+// and -1 otherwise, and -1 == ~(z1 ^ x2) & (x1 | z2) on those inputs.
+// (c0, c1) is the running sum mod 4; adding +1 carries c0 into c1, adding
+// -1 (== +3) carries ~c0.
+PQ_HD void phase_acc(uint64_t x1, uint64_t z1, uint64_t x2, uint64_t z2,
+                     uint64_t &c0, uint64_t &c1) {
+  const uint64_t anti = (x1 & z2) ^ (z1 & x2);
+  const uint64_t minus = ~(z1 ^ x2) & (x1 | z2);
+  c1 ^= anti & (c0 ^ minus);
+  c0 ^= anti;
+}
+
+// (a0, a1) += (b0, b1) mod 4, bitwise per shot.
+PQ_HD void add_mod4(uint64_t &a0, uint64_t &a1, uint64_t b0, uint64_t b1) {
+  a1 ^= b1 ^ (a0 & b0);
+  a0 ^= b0;
+}
+
+// Device-side instruction of a compiled program (32 bytes).
+//  gates: code, qa, qb; flagged ops: slot = record id;
+//  na / nb: noise plane fused after the op on qa / qb (kNone if none);
+//  PQ_OP_NOISE alone: qa, na;  M: qa, slot = record id, t0 = coin
+//  threshold, (na, nb) = coin stream (low, high 32 bits).
+struct DInstr {
+  uint32_t code, qa, qb, slot, na, nb, t0, pad;
+};
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// Segment of a compiled program: a run of gate batches or a run of
+// measurements.  Gate segments hold `count` instructions in batches; batch
+// boundaries are flagged in DInstr::pad (bit 0 = first op of a batch).
+struct DSegment {
+  uint32_t kind;  // 0 = gates, 1 = measurements
+  uint32_t begin, count, pad;
+};
+
+// Noise plane descriptor: stream and cumulative X/Y/Z thresholds.
+struct NoiseDesc {
+  uint64_t key;  // stream_key(seed, 2k+1)
+  uint32_t t0, t1, t2, pad;
+};
+
+}  // namespace pq

@@ -1,0 +1,871 @@
+// gfx950 kernels of the bit-plane stabilizer-tableau engine.
+//
+// Word (w, c, row) of the tableau is tab[(w*C + c)*R + row]; every column of
+// a 64-shot word group is R contiguous words, so a gate sweep over column q
+// is a fully coalesced stream and a rowsum over rows i of column j is too.
+#include "pq_kernels.h"
+
+#include <algorithm>
+
+#include "plaquette_amd.h"
+
+namespace pq {
+
+namespace {
+
+constexpr uint32_t kMeasThreads = 256;
+constexpr uint32_t kMaxWaves = 16;  // up to 1024 threads per block
+constexpr uint32_t kMeasMaxQubits = 4095;
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+// ---------------------------------------------------------------------------
+// Init: word (c, row) of every group is all-ones iff row == c, which is the
+// identity tableau with r = 1 on the scratch row (MiscFunctors.hpp:22-27).
+// ---------------------------------------------------------------------------
+__global__ void k_fill_identity(uint64_t *__restrict__ tab, uint32_t R,
+                                uint32_t C, uint64_t total) {
+  const uint64_t blk = (uint64_t)C * R;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t rem = i % blk;
+    const uint32_t c = (uint32_t)(rem / R);
+    const uint32_t row = (uint32_t)(rem - (uint64_t)c * R);
+    tab[i] = (row == c) ? ~0ull : 0ull;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Gate sweep: one thread per (word group, row).  The rules are the
+// reference's per-(shot,row) functors (GateFunctors.hpp) applied to 64 shots
+// at once; fa/fb are optional per-word shot masks (flag, or noise X/Z parts).
+// ---------------------------------------------------------------------------
+struct GateArgs {
+  uint32_t code, n, R, C, qa, qb;
+  uint64_t W;
+  const uint64_t *fa;
+  const uint64_t *fb;
+};
+
+__global__ __launch_bounds__(256) void k_gate(uint64_t *__restrict__ tab,
+                                              GateArgs a) {
+  const uint64_t total = a.W * a.R;
+  const uint64_t R = a.R;
+  for (uint64_t idx = blockIdx.x * 256ull + threadIdx.x; idx < total;
+       idx += (uint64_t)gridDim.x * 256ull) {
+    const uint64_t w = idx / R;
+    const uint64_t row = idx - w * R;
+    uint64_t *T = tab + w * (uint64_t)a.C * R + row;
+    uint64_t *pxa = T + (uint64_t)a.qa * R;
+    uint64_t *pza = T + (uint64_t)(a.n + a.qa) * R;
+    uint64_t *pr = T + (uint64_t)(2 * a.n) * R;
+    switch (a.code) {
+      case PQ_OP_H: {  // GateFunctors.hpp:90-93
+        const uint64_t x = *pxa, z = *pza;
+        *pr ^= x & z;
+        *pxa = z;
+        *pza = x;
+        break;
+      }
+      case PQ_OP_S: {  // GateFunctors.hpp:151-170
+        const uint64_t x = *pxa, z = *pza;
+        *pr ^= x & z;
+        *pza = z ^ x;
+        break;
+      }
+      case PQ_OP_CX: {  // GateFunctors.hpp:248-252, qa control, qb target
+        uint64_t *pxb = T + (uint64_t)a.qb * R;
+        uint64_t *pzb = T + (uint64_t)(a.n + a.qb) * R;
+        const uint64_t xc = *pxa, zc = *pza, xt = *pxb, zt = *pzb;
+        *pr ^= xc & zt & ~(xt ^ zc);
+        *pxb = xt ^ xc;
+        *pza = zc ^ zt;
+        break;
+      }
+      case PQ_OP_CZ: {  // GateFunctors.hpp:30-35
+        uint64_t *pxb = T + (uint64_t)a.qb * R;
+        uint64_t *pzb = T + (uint64_t)(a.n + a.qb) * R;
+        const uint64_t xc = *pxa, zc = *pza, xt = *pxb, zt = *pzb;
+        *pr ^= xc & xt & (zt ^ zc);
+        *pzb = zt ^ xc;
+        *pza = zc ^ xt;
+        break;
+      }
+      case PQ_OP_X: *pr ^= *pza; break;            // :304-307
+      case PQ_OP_Y: *pr ^= *pza ^ *pxa; break;     // :324-327
+      case PQ_OP_Z: *pr ^= *pxa; break;            // :373-376
+      case PQ_OP_H_FLAG: {                          // :142-148
+        const uint64_t f = a.fa[w];
+        const uint64_t x = *pxa, z = *pza;
+        const uint64_t d = f & (x ^ z);
+        *pr ^= f & x & z;
+        *pxa = x ^ d;
+        *pza = z ^ d;
+        break;
+      }
+      case PQ_OP_S_FLAG: {                          // :220-228
+        const uint64_t f = a.fa[w];
+        const uint64_t x = *pxa, z = *pza;
+        *pr ^= f & x & z;
+        *pza = z ^ (f & x);
+        break;
+      }
+      case PQ_OP_NOISE: {  // X part where fa, Z part where fb
+        const uint64_t fx = a.fa[w], fz = a.fb[w];
+        *pr ^= (fx & *pza) ^ (fz & *pxa);
+        break;
+      }
+      default: break;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Noise planes: plane k holds, per word group, the X-part and Z-part shot
+// masks of the k-th noise op.  One wave per (k, w): each lane draws its shot.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_noise_planes(
+    uint64_t *__restrict__ planes, NoiseDesc one,
+    const NoiseDesc *__restrict__ desc, uint32_t nk, uint64_t W) {
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
+  const uint64_t wave = blockIdx.x * (uint64_t)(blockDim.x / 64) + (threadIdx.x >> 6);
+  const uint32_t lane = lane_id();
+  for (uint64_t p = wave; p < (uint64_t)nk * W; p += waves) {
+    const uint64_t k = p / W;
+    const uint64_t w = p - k * W;
+    const NoiseDesc d = desc ? desc[k] : one;
+    const uint32_t u = (uint32_t)(draw_keyed(d.key, 64 * w + lane) >> 32);
+    const uint64_t mx = __ballot(u < d.t1);                 // X or Y
+    const uint64_t mz = __ballot((u >= d.t0) & (u < d.t2)); // Y or Z
+    if (lane == 0) {
+      planes[(2 * k) * W + w] = mx;
+      planes[(2 * k + 1) * W + w] = mz;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Block-wide scans (blockDim a multiple of 64, at most 1024 threads).
+// ---------------------------------------------------------------------------
+struct ScanShared {
+  uint64_t w64[kMaxWaves];
+  uint32_t w32[kMaxWaves];
+  uint64_t tot64;
+  uint32_t tot32;
+};
+
+// Exclusive OR-scan of v over threads in thread order; *total = OR of all.
+__device__ uint64_t block_excl_or(uint64_t v, ScanShared &sh, uint64_t *total) {
+  const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
+  const uint32_t nw = blockDim.x >> 6;
+  uint64_t inc = v;
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint64_t t = __shfl_up(inc, d, 64);
+    if (lane >= d) inc |= t;
+  }
+  uint64_t exc = __shfl_up(inc, 1, 64);
+  if (lane == 0) exc = 0;
+  if (lane == 63) sh.w64[wid] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t run = 0;
+    for (uint32_t i = 0; i < nw; ++i) {
+      const uint64_t t = sh.w64[i];
+      sh.w64[i] = run;
+      run |= t;
+    }
+    sh.tot64 = run;
+  }
+  __syncthreads();
+  exc |= sh.w64[wid];
+  *total = sh.tot64;
+  __syncthreads();
+  return exc;
+}
+
+// Exclusive sum-scan of v over threads in thread order; *total = sum.
+__device__ uint32_t block_excl_sum(uint32_t v, ScanShared &sh, uint32_t *total) {
+  const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
+  const uint32_t nw = blockDim.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += t;
+  }
+  if (lane == 63) sh.w32[wid] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (uint32_t i = 0; i < nw; ++i) {
+      const uint32_t t = sh.w32[i];
+      sh.w32[i] = run;
+      run += t;
+    }
+    sh.tot32 = run;
+  }
+  __syncthreads();
+  const uint32_t exc = inc - v + sh.w32[wid];
+  *total = sh.tot32;
+  __syncthreads();
+  return exc;
+}
+
+// ---------------------------------------------------------------------------
+// MeasureQubit for one 64-shot word group (BatchMeasureFunctor,
+// MeasurementFunctors.hpp:291-317), executed by a whole workgroup.
+//
+//  1. Pivot per shot = first stabilizer row with x_q set (:297-301): an
+//     exclusive OR-scan over the stabilizer rows turns column q into
+//     disjoint "first pivot" masks, one group (p, M) per distinct pivot.
+//  2. Random shots, per group (MeasureRandom :226-265): copy row n+p to row
+//     p, clear row n+p, set its z_q and its r to the coin, all masked by M;
+//     then every row i (not p, n+p) with x_q set in M gets rowsum(i <- p),
+//     masked per shot, with the bit-sliced mod-4 phase (only columns where
+//     the source row is non-identity can contribute).
+//  3. Determined shots D (MeasureDetermined :190-201): one wave clears the
+//     scratch row in D and folds in stabilizer n+i for ascending i with
+//     destabilizer x_q set, reducing each step's phase across lanes.
+//  4. Record: result = coin (random) | scratch r (determined), determined = D.
+// ---------------------------------------------------------------------------
+struct MeasShared {
+  ScanShared scan;
+  uint64_t gm[64];
+  uint32_t gp[64];
+  uint32_t ngroups;
+  uint32_t nlist;
+  uint64_t rm;    // random shots
+  uint64_t coin;  // coin word
+};
+
+__device__ void measure_word(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
+                             uint32_t R, uint32_t q, uint32_t coin_T,
+                             uint64_t coin_key, const uint64_t *coin_plane,
+                             uint64_t *rec_res, uint64_t *rec_det,
+                             uint64_t *s_src, uint32_t *s_list,
+                             MeasShared &sh, bool force_determined = false) {
+  const uint32_t tid = threadIdx.x, NT = blockDim.x;
+  const uint64_t Ru = R;
+  const uint64_t *xq = T + (uint64_t)q * Ru;
+
+  // ---- 1. pivot groups ----
+  const uint32_t cp = (n + NT - 1) / NT;
+  const uint32_t p0 = min(n, tid * cp), p1 = min(n, p0 + cp);
+  uint64_t acc = 0;
+  for (uint32_t p = p0; p < p1; ++p) acc |= xq[n + p];
+  if (tid == 0) sh.ngroups = 0;
+  uint64_t rm;
+  uint64_t before = block_excl_or(acc, sh.scan, &rm);
+  if (force_determined) rm = 0;
+  for (uint32_t p = p0; p < p1 && !force_determined; ++p) {
+    const uint64_t v = xq[n + p];
+    const uint64_t first = v & ~before;
+    before |= v;
+    if (first) {
+      const uint32_t g = atomicAdd(&sh.ngroups, 1u);
+      sh.gp[g] = p;
+      sh.gm[g] = first;
+    }
+  }
+  if (tid < 64 && rm) {
+    uint64_t coin;
+    if (coin_plane) {
+      coin = coin_plane[w];
+    } else {
+      const uint32_t u = (uint32_t)(draw_keyed(coin_key, 64 * w + tid) >> 40);
+      coin = __ballot(u < coin_T);
+    }
+    if (tid == 0) sh.coin = coin;
+  }
+  __syncthreads();
+  const uint64_t coin = rm ? sh.coin : 0;
+  const uint32_t ngroups = sh.ngroups;
+
+  // ---- 2. random shots, one pivot group at a time ----
+  const uint32_t C = 2 * n + 1;
+  for (uint32_t g = 0; g < ngroups; ++g) {
+    const uint32_t p = sh.gp[g];
+    const uint64_t M = sh.gm[g];
+    for (uint32_t c = tid; c < C; c += NT) {
+      uint64_t *col = T + (uint64_t)c * Ru;
+      const uint64_t a = col[n + p];
+      const uint64_t b = col[p];
+      col[p] = (b & ~M) | (a & M);
+      uint64_t na = a & ~M;
+      if (c == n + q) na |= M;
+      if (c == 2 * n) na |= coin & M;
+      col[n + p] = na;
+      s_src[c] = a;
+    }
+    if (tid == 0) sh.nlist = 0;
+    __syncthreads();
+    for (uint32_t j = tid; j < n; j += NT)
+      if ((s_src[j] | s_src[n + j]) & M) s_list[atomicAdd(&sh.nlist, 1u)] = j;
+    __syncthreads();
+    const uint32_t nl = sh.nlist;
+    const uint64_t rsrc = s_src[2 * n];
+    for (uint32_t i = tid; i < 2 * n; i += NT) {
+      if (i == p || i == n + p) continue;
+      const uint64_t mi = M & xq[i];
+      if (!mi) continue;
+      uint64_t c0 = 0, c1 = 0;
+      for (uint32_t k = 0; k < nl; ++k) {
+        const uint32_t j = s_list[k];
+        uint64_t *px = T + (uint64_t)j * Ru + i;
+        uint64_t *pz = T + (uint64_t)(n + j) * Ru + i;
+        const uint64_t x1 = *px, z1 = *pz;
+        const uint64_t x2 = s_src[j], z2 = s_src[n + j];
+        phase_acc(x1, z1, x2, z2, c0, c1);
+        *px = x1 ^ (mi & x2);
+        *pz = z1 ^ (mi & z2);
+      }
+      T[(uint64_t)(2 * n) * Ru + i] ^= mi & (rsrc ^ c1);
+    }
+    __syncthreads();
+  }
+
+  // ---- 3. determined shots ----
+  const uint64_t D = ~rm;
+  uint64_t rdet = 0;
+  if (D) {
+    // ordered list of destabilizers i with x_q set in some determined shot
+    uint32_t cnt = 0;
+    for (uint32_t i = p0; i < p1; ++i) cnt += (xq[i] & D) ? 1u : 0u;
+    uint32_t nchain;
+    uint32_t pos = block_excl_sum(cnt, sh.scan, &nchain);
+    for (uint32_t i = p0; i < p1; ++i)
+      if (xq[i] & D) s_list[pos++] = i;
+    __syncthreads();
+    if (tid < 64) {
+      const uint32_t lane = tid;
+      const uint64_t scratch = 2 * n;
+      for (uint32_t j = lane; j < 2 * n; j += 64)
+        s_src[j] = T[(uint64_t)j * Ru + scratch] & ~D;
+      uint64_t racc = T[scratch * Ru + scratch] & ~D;
+      for (uint32_t k = 0; k < nchain; ++k) {
+        const uint32_t s = n + s_list[k];
+        const uint64_t m = xq[s_list[k]] & D;
+        uint64_t c0 = 0, c1 = 0;
+        for (uint32_t j = lane; j < n; j += 64) {
+          const uint64_t x2 = T[(uint64_t)j * Ru + s];
+          const uint64_t z2 = T[(uint64_t)(n + j) * Ru + s];
+          const uint64_t x1 = s_src[j], z1 = s_src[n + j];
+          phase_acc(x1, z1, x2, z2, c0, c1);
+          s_src[j] = x1 ^ (m & x2);
+          s_src[n + j] = z1 ^ (m & z2);
+        }
+#pragma unroll
+        for (uint32_t off = 32; off > 0; off >>= 1) {
+          const uint64_t t0 = __shfl_xor(c0, off, 64);
+          const uint64_t t1 = __shfl_xor(c1, off, 64);
+          add_mod4(c0, c1, t0, t1);
+        }
+        racc ^= m & (T[scratch * Ru + s] ^ c1);
+      }
+      for (uint32_t j = lane; j < 2 * n; j += 64)
+        T[(uint64_t)j * Ru + scratch] = s_src[j];
+      if (lane == 0) T[scratch * Ru + scratch] = racc;
+      rdet = racc;
+    }
+  }
+  if (tid == 0) {
+    rec_res[w] = (coin & rm) | (rdet & D);
+    rec_det[w] = D;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kMeasThreads) void k_measure(
+    uint64_t *__restrict__ tab, uint32_t n, uint32_t R, uint64_t W, uint32_t q,
+    uint32_t coin_T, uint64_t coin_key, const uint64_t *coin_plane,
+    uint64_t *rec, uint32_t force_determined) {
+  extern __shared__ uint64_t dyn[];
+  __shared__ MeasShared sh;
+  uint64_t *s_src = dyn;
+  uint32_t *s_list = reinterpret_cast<uint32_t *>(dyn + (2 * n + 1));
+  const uint64_t C = 2ull * n + 1;
+  for (uint64_t w = blockIdx.x; w < W; w += gridDim.x)
+    measure_word(tab + w * C * R, w, n, R, q, coin_T, coin_key, coin_plane,
+                 rec, rec + W, s_src, s_list, sh, force_determined != 0);
+}
+
+// ---------------------------------------------------------------------------
+// Host I/O: flat int32 [b][row][col] <-> bit-planes.
+// ---------------------------------------------------------------------------
+__global__ void k_unpack(const uint64_t *__restrict__ tab, uint32_t n,
+                         uint32_t R, uint32_t C, uint64_t b0, uint64_t nb,
+                         int32_t *__restrict__ x, int32_t *__restrict__ z,
+                         int32_t *__restrict__ r) {
+  const uint64_t per = (uint64_t)R * n;
+  const uint64_t total = nb * per;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t b = i / per;
+    const uint64_t rem = i - b * per;
+    const uint32_t row = (uint32_t)(rem / n);
+    const uint32_t col = (uint32_t)(rem - (uint64_t)row * n);
+    const uint64_t sb = b0 + b;
+    const uint64_t *G = tab + (sb >> 6) * (uint64_t)C * R;
+    const uint32_t bit = (uint32_t)(sb & 63);
+    x[i] = (int32_t)((G[(uint64_t)col * R + row] >> bit) & 1);
+    z[i] = (int32_t)((G[(uint64_t)(n + col) * R + row] >> bit) & 1);
+    if (col == 0)
+      r[b * R + row] = (int32_t)((G[(uint64_t)(2 * n) * R + row] >> bit) & 1);
+  }
+}
+
+__global__ void k_pack(uint64_t *__restrict__ tab, uint32_t n, uint32_t R,
+                       uint32_t C, uint64_t B, uint64_t w0, uint64_t nw,
+                       const int32_t *__restrict__ x,
+                       const int32_t *__restrict__ z,
+                       const int32_t *__restrict__ r) {
+  // x/z/r hold shots [64*w0, 64*w0 + 64*nw) (clipped to B)
+  const uint64_t blk = (uint64_t)C * R;
+  const uint64_t total = nw * blk;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t wl = i / blk;
+    const uint64_t rem = i - wl * blk;
+    const uint32_t c = (uint32_t)(rem / R);
+    const uint32_t row = (uint32_t)(rem - (uint64_t)c * R);
+    uint64_t word = 0;
+    for (uint32_t s = 0; s < 64; ++s) {
+      const uint64_t gb = 64 * (w0 + wl) + s;
+      if (gb >= B) break;
+      const uint64_t lb = gb - 64 * w0;
+      int32_t v;
+      if (c < n) v = x[(lb * R + row) * n + c];
+      else if (c < 2 * n) v = z[(lb * R + row) * n + (c - n)];
+      else v = r[lb * R + row];
+      word |= (uint64_t)(v & 1) << s;
+    }
+    tab[(w0 + wl) * blk + rem] = word;
+  }
+}
+
+// CheckSum (MiscFunctors.hpp:43-49): popcount of x and z over all rows and
+// of r once per row (the reference adds r at column 0, so only when n > 0).
+__global__ __launch_bounds__(256) void k_checksum(
+    const uint64_t *__restrict__ tab, uint32_t n, uint32_t R, uint32_t C,
+    uint64_t W, uint64_t b0, uint64_t b1, unsigned long long *out) {
+  const uint64_t blk = (uint64_t)C * R;
+  const uint64_t total = W * blk;
+  unsigned long long acc = 0;
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * 256ull) {
+    const uint64_t w = i / blk;
+    const uint64_t rem = i - w * blk;
+    const uint32_t c = (uint32_t)(rem / R);
+    if (c == 2 * n && n == 0) continue;
+    const uint64_t lo = 64 * w, hi = lo + 64;
+    const uint64_t a = b0 > lo ? b0 : lo, e = b1 < hi ? b1 : hi;
+    if (a >= e) continue;
+    const uint32_t sh = (uint32_t)(a - lo), len = (uint32_t)(e - a);
+    const uint64_t mask = (len == 64 ? ~0ull : ((1ull << len) - 1)) << sh;
+    acc += __popcll(tab[i] & mask);
+  }
+  __shared__ unsigned long long red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && red[0]) atomicAdd(out, red[0]);
+}
+
+__global__ void k_flags_to_plane(const int32_t *__restrict__ flags, uint64_t B,
+                                 uint64_t *__restrict__ plane) {
+  const uint64_t W = (B + 63) / 64;
+  const uint32_t lane = lane_id();
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
+  for (uint64_t w = blockIdx.x * (uint64_t)(blockDim.x / 64) + (threadIdx.x >> 6); w < W;
+       w += waves) {
+    const uint64_t b = 64 * w + lane;
+    const uint64_t m = __ballot(b < B && flags[b] != 0);
+    if (lane == 0) plane[w] = m;
+  }
+}
+
+__global__ void k_record_to_int(const uint64_t *__restrict__ rec, uint64_t B,
+                                uint64_t W, int32_t *__restrict__ res,
+                                int32_t *__restrict__ det) {
+  for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; b < B;
+       b += (uint64_t)gridDim.x * blockDim.x) {
+    res[b] = (int32_t)((rec[b >> 6] >> (b & 63)) & 1);
+    det[b] = (int32_t)((rec[W + (b >> 6)] >> (b & 63)) & 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent program kernel: one workgroup runs a whole compiled op list on
+// one 64-shot word group at a time.  Shots never interact, so no grid-wide
+// synchronisation is needed, measurements included.
+//
+// Gate segments: each thread owns rows; r stays in a register for the whole
+// segment; instructions come in batches of pairwise qubit-disjoint ops, so a
+// batch's operand loads are all issued before any of its stores (memory-level
+// parallelism), and noise planes fused into an op are applied to the columns
+// that op just produced.  Measurement segments call measure_word.
+// ---------------------------------------------------------------------------
+__global__ void k_set_params(uint64_t *p, uint64_t seed, uint64_t mseq0,
+                             uint64_t nseq0) {
+  if (threadIdx.x == 0) {
+    p[0] = seed;
+    p[1] = mseq0;
+    p[2] = nseq0;
+  }
+}
+
+struct ProgParams {
+  uint64_t seed;
+  uint64_t mseq0;  // measurement sequence number of the program's first M
+  uint64_t nseq0;  // noise sequence number of the program's first NOISE
+};
+
+constexpr int kBatch = 8;
+
+__device__ __forceinline__ void apply_fused_noise(const uint64_t *__restrict__ planes,
+                                                  uint64_t W, uint64_t w, uint32_t k,
+                                                  uint64_t x, uint64_t z, uint64_t &r) {
+  if (k != kNone) {
+    const uint64_t fx = planes[(2ull * k) * W + w];
+    const uint64_t fz = planes[(2ull * k + 1) * W + w];
+    r ^= (fx & z) ^ (fz & x);
+  }
+}
+
+__device__ void run_gate_segment(uint64_t *__restrict__ T, uint64_t w,
+                                 const pq::ProgramArgs &a, const DSegment &sg) {
+  const uint64_t Ru = a.R;
+  const uint32_t n = a.n;
+  const uint64_t W = a.W;
+  const DInstr *__restrict__ ins = a.instrs;
+  for (uint32_t row = threadIdx.x; row < a.R; row += blockDim.x) {
+    uint64_t *Tr = T + row;
+    uint64_t r = Tr[(uint64_t)(2 * n) * Ru];
+    uint32_t k = sg.begin;
+    const uint32_t end = sg.begin + sg.count;
+    while (k < end) {
+      const uint32_t len = ins[k].pad;
+      uint64_t xa[kBatch], za[kBatch], xb[kBatch], zb[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        if (u < (int)len) {
+          const DInstr I = ins[k + u];
+          xa[u] = Tr[(uint64_t)I.qa * Ru];
+          za[u] = Tr[(uint64_t)(n + I.qa) * Ru];
+          if (I.code == PQ_OP_CX || I.code == PQ_OP_CZ) {
+            xb[u] = Tr[(uint64_t)I.qb * Ru];
+            zb[u] = Tr[(uint64_t)(n + I.qb) * Ru];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        if (u < (int)len) {
+          const DInstr I = ins[k + u];
+          uint64_t *pxa = Tr + (uint64_t)I.qa * Ru;
+          uint64_t *pza = Tr + (uint64_t)(n + I.qa) * Ru;
+          uint64_t x = xa[u], z = za[u];
+          switch (I.code) {
+            case PQ_OP_H:
+              r ^= x & z;
+              *pxa = z; *pza = x;
+              { const uint64_t t = x; x = z; z = t; }
+              break;
+            case PQ_OP_S:
+              r ^= x & z;
+              z ^= x;
+              *pza = z;
+              break;
+            case PQ_OP_X: r ^= z; break;
+            case PQ_OP_Y: r ^= z ^ x; break;
+            case PQ_OP_Z: r ^= x; break;
+            case PQ_OP_CX: {
+              uint64_t xt = xb[u], zt = zb[u];
+              r ^= x & zt & ~(xt ^ z);
+              xt ^= x;
+              z ^= zt;
+              Tr[(uint64_t)I.qb * Ru] = xt;
+              *pza = z;
+              apply_fused_noise(a.planes, W, w, I.nb, xt, zt, r);
+              break;
+            }
+            case PQ_OP_CZ: {
+              uint64_t xt = xb[u], zt = zb[u];
+              r ^= x & xt & (zt ^ z);
+              zt ^= x;
+              z ^= xt;
+              Tr[(uint64_t)(n + I.qb) * Ru] = zt;
+              *pza = z;
+              apply_fused_noise(a.planes, W, w, I.nb, xt, zt, r);
+              break;
+            }
+            case PQ_OP_H_FLAG: {
+              const uint64_t f = a.recs[(2ull * I.slot) * W + w];
+              const uint64_t d = f & (x ^ z);
+              r ^= f & x & z;
+              x ^= d; z ^= d;
+              *pxa = x; *pza = z;
+              break;
+            }
+            case PQ_OP_S_FLAG: {
+              const uint64_t f = a.recs[(2ull * I.slot) * W + w];
+              r ^= f & x & z;
+              z ^= f & x;
+              *pza = z;
+              break;
+            }
+            case PQ_OP_X_FLAG: r ^= a.recs[(2ull * I.slot) * W + w] & z; break;
+            case PQ_OP_Y_FLAG: r ^= a.recs[(2ull * I.slot) * W + w] & (z ^ x); break;
+            case PQ_OP_Z_FLAG: r ^= a.recs[(2ull * I.slot) * W + w] & x; break;
+            default: break;  // PQ_OP_NOISE: only the fused plane below
+          }
+          apply_fused_noise(a.planes, W, w, I.na, x, z, r);
+        }
+      }
+      k += len;
+    }
+    Tr[(uint64_t)(2 * n) * Ru] = r;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_program(pq::ProgramArgs a,
+                                                  const ProgParams *__restrict__ pp) {
+  extern __shared__ uint64_t dyn[];
+  __shared__ MeasShared sh;
+  const uint32_t n = a.n;
+  uint64_t *s_src = dyn;
+  uint32_t *s_list = reinterpret_cast<uint32_t *>(dyn + (2 * n + 1));
+  const uint64_t C = 2ull * n + 1;
+  const uint64_t seed = pp->seed, mseq0 = pp->mseq0;
+  for (uint64_t wi = blockIdx.x; wi < a.nwords; wi += gridDim.x) {
+    const uint64_t w = a.word0 + wi;
+    uint64_t *T = a.tab + w * C * a.R;
+    for (uint32_t si = 0; si < a.nseg; ++si) {
+      const DSegment sg = a.segs[si];
+      if (sg.kind == 0) {
+        run_gate_segment(T, w, a, sg);
+        __syncthreads();
+      } else {
+        for (uint32_t k = sg.begin; k < sg.begin + sg.count; ++k) {
+          const DInstr I = a.instrs[k];
+          const uint64_t m = (uint64_t)I.na | ((uint64_t)I.nb << 32);
+          const uint64_t key = stream_key(seed, 2 * (mseq0 + m));
+          const uint64_t *cp = a.coin_planes ? a.coin_planes + m * a.W : nullptr;
+          uint64_t *rec = a.recs + (2ull * I.slot) * a.W;
+          measure_word(T, w, n, a.R, I.qa, I.t0, key, cp, rec, rec + a.W, s_src,
+                       s_list, sh);
+        }
+      }
+    }
+  }
+}
+
+// Noise planes of a program: plane k uses stream 2*(nseq0 + k) + 1.
+__global__ __launch_bounds__(256) void k_program_noise(
+    uint64_t *__restrict__ planes, const NoiseDesc *__restrict__ desc,
+    uint32_t nk, uint64_t W, const ProgParams *__restrict__ pp) {
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
+  const uint64_t wave = blockIdx.x * (uint64_t)(blockDim.x / 64) + (threadIdx.x >> 6);
+  const uint32_t lane = lane_id();
+  const uint64_t seed = pp->seed, nseq0 = pp->nseq0;
+  for (uint64_t p = wave; p < (uint64_t)nk * W; p += waves) {
+    const uint64_t k = p / W;
+    const uint64_t w = p - k * W;
+    const NoiseDesc d = desc[k];
+    const uint64_t key = stream_key(seed, 2 * (nseq0 + k) + 1);
+    const uint32_t u = (uint32_t)(draw_keyed(key, 64 * w + lane) >> 32);
+    const uint64_t mx = __ballot(u < d.t1);
+    const uint64_t mz = __ballot((u >= d.t0) & (u < d.t2));
+    if (lane == 0) {
+      planes[(2 * k) * W + w] = mx;
+      planes[(2 * k + 1) * W + w] = mz;
+    }
+  }
+}
+
+// RowMultiply / RowProductSign (MeasurementFunctors.hpp:98-166) on every
+// shot: one thread per word group; sign_out (if set) receives the sign plane
+// without modifying the tableau.
+__global__ void k_rowsum(uint64_t *__restrict__ tab, uint32_t n, uint32_t R,
+                         uint64_t W, uint32_t i, uint32_t k, uint64_t *sign_out) {
+  const uint64_t C = 2ull * n + 1;
+  for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < W;
+       w += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t *T = tab + w * C * R;
+    uint64_t c0 = 0, c1 = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+      uint64_t *px = T + (uint64_t)j * R, *pz = T + (uint64_t)(n + j) * R;
+      phase_acc(px[i], pz[i], px[k], pz[k], c0, c1);
+    }
+    uint64_t *pr = T + (uint64_t)(2 * n) * R;
+    const uint64_t sign = pr[i] ^ pr[k] ^ c1;
+    if (sign_out) {
+      sign_out[w] = sign;
+    } else {
+      pr[i] = sign;
+      for (uint32_t j = 0; j < n; ++j) {
+        uint64_t *px = T + (uint64_t)j * R, *pz = T + (uint64_t)(n + j) * R;
+        px[i] ^= px[k];
+        pz[i] ^= pz[k];
+      }
+    }
+  }
+}
+
+inline uint32_t grid_for(uint64_t items, uint32_t threads, uint32_t cap = 1u << 20) {
+  uint64_t g = (items + threads - 1) / threads;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (uint32_t)g;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Persistent program kernel (defined in pq_program.hip, shares measure_word).
+// ---------------------------------------------------------------------------
+
+hipError_t launch_fill_identity(hipStream_t s, uint64_t *tab, const Geom &g) {
+  const uint64_t total = g.W * (uint64_t)g.C * g.R;
+  if (!total) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_identity, dim3(grid_for(total, 256, 65536)), dim3(256),
+                     0, s, tab, g.R, g.C, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_gate(hipStream_t s, uint64_t *tab, const Geom &g, uint32_t code,
+                       uint32_t qa, uint32_t qb, const uint64_t *fa,
+                       const uint64_t *fb) {
+  const uint64_t total = g.W * g.R;
+  if (!total) return hipSuccess;
+  GateArgs a{code, g.n, g.R, g.C, qa, qb, g.W, fa, fb};
+  hipLaunchKernelGGL(k_gate, dim3(grid_for(total, 256)), dim3(256), 0, s, tab, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_noise_planes(hipStream_t s, uint64_t *planes, NoiseDesc one,
+                               const NoiseDesc *desc, uint32_t nk, uint64_t W) {
+  const uint64_t items = (uint64_t)nk * W;
+  if (!items) return hipSuccess;
+  hipLaunchKernelGGL(k_noise_planes, dim3(grid_for(items * 64, 256, 16384)),
+                     dim3(256), 0, s, planes, one, desc, nk, W);
+  return hipGetLastError();
+}
+
+size_t measure_lds_bytes(uint32_t n) {
+  return sizeof(uint64_t) * (2ull * n + 1) + sizeof(uint32_t) * (uint64_t)(n + 1);
+}
+
+uint32_t measure_max_qubits() { return kMeasMaxQubits; }
+
+static hipError_t ensure_lds_attr(const void *fn, size_t bytes) {
+  if (bytes <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes);
+}
+
+hipError_t launch_measure(hipStream_t s, uint64_t *tab, const Geom &g,
+                          const MeasParams &m) {
+  if (!g.W) return hipSuccess;
+  const size_t lds = measure_lds_bytes(g.n);
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>(g.W, 1u << 20);
+  hipLaunchKernelGGL(k_measure, dim3(blocks), dim3(kMeasThreads), lds, s, tab,
+                     g.n, g.R, g.W, m.q, m.coin_T, m.coin_key, m.coin_plane,
+                     m.rec, m.force_determined ? 1u : 0u);
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack(hipStream_t s, const uint64_t *tab, const Geom &g,
+                         uint64_t b0, uint64_t nb, int32_t *x, int32_t *z,
+                         int32_t *r) {
+  const uint64_t total = nb * (uint64_t)g.R * g.n;
+  if (!total) return hipSuccess;
+  hipLaunchKernelGGL(k_unpack, dim3(grid_for(total, 256, 65536)), dim3(256), 0, s,
+                     tab, g.n, g.R, g.C, b0, nb, x, z, r);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack(hipStream_t s, uint64_t *tab, const Geom &g, uint64_t w0,
+                       uint64_t nw, const int32_t *x, const int32_t *z,
+                       const int32_t *r) {
+  const uint64_t total = nw * (uint64_t)g.C * g.R;
+  if (!total) return hipSuccess;
+  hipLaunchKernelGGL(k_pack, dim3(grid_for(total, 256, 65536)), dim3(256), 0, s,
+                     tab, g.n, g.R, g.C, g.B, w0, nw, x, z, r);
+  return hipGetLastError();
+}
+
+hipError_t launch_checksum(hipStream_t s, const uint64_t *tab, const Geom &g,
+                           uint64_t b0, uint64_t b1,
+                           unsigned long long *out_dev) {
+  const uint64_t total = g.W * (uint64_t)g.C * g.R;
+  if (!total) return hipSuccess;
+  hipLaunchKernelGGL(k_checksum, dim3(grid_for(total, 256, 8192)), dim3(256), 0, s,
+                     tab, g.n, g.R, g.C, g.W, b0, b1, out_dev);
+  return hipGetLastError();
+}
+
+hipError_t launch_flags_to_plane(hipStream_t s, const int32_t *flags,
+                                 uint64_t B, uint64_t *plane) {
+  const uint64_t W = (B + 63) / 64;
+  if (!W) return hipSuccess;
+  hipLaunchKernelGGL(k_flags_to_plane, dim3(grid_for(W * 64, 256, 4096)), dim3(256),
+                     0, s, flags, B, plane);
+  return hipGetLastError();
+}
+
+hipError_t launch_record_to_int(hipStream_t s, const uint64_t *rec, uint64_t B,
+                                uint64_t W, int32_t *res, int32_t *det) {
+  if (!B) return hipSuccess;
+  hipLaunchKernelGGL(k_record_to_int, dim3(grid_for(B, 256, 4096)), dim3(256), 0,
+                     s, rec, B, W, res, det);
+  return hipGetLastError();
+}
+
+hipError_t launch_program(hipStream_t s, const ProgramArgs &a, uint32_t threads,
+                          uint32_t blocks, const void *params) {
+  if (!a.nwords) return hipSuccess;
+  const size_t lds = measure_lds_bytes(a.n);
+  hipLaunchKernelGGL(k_program, dim3(blocks), dim3(threads), lds, s, a,
+                     (const ProgParams *)params);
+  return hipGetLastError();
+}
+
+hipError_t launch_program_noise(hipStream_t s, uint64_t *planes,
+                                const NoiseDesc *desc, uint32_t nk, uint64_t W,
+                                const void *params) {
+  const uint64_t items = (uint64_t)nk * W;
+  if (!items) return hipSuccess;
+  hipLaunchKernelGGL(k_program_noise, dim3(grid_for(items * 64, 256, 16384)),
+                     dim3(256), 0, s, planes, desc, nk, W,
+                     (const ProgParams *)params);
+  return hipGetLastError();
+}
+
+hipError_t launch_rowsum(hipStream_t s, uint64_t *tab, const Geom &g, uint32_t i,
+                         uint32_t k, uint64_t *sign_out) {
+  if (!g.W) return hipSuccess;
+  hipLaunchKernelGGL(k_rowsum, dim3(grid_for(g.W, 64, 4096)), dim3(64), 0, s, tab,
+                     g.n, g.R, g.W, i, k, sign_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_params(hipStream_t s, void *params, uint64_t seed,
+                             uint64_t mseq0, uint64_t nseq0) {
+  hipLaunchKernelGGL(k_set_params, dim3(1), dim3(64), 0, s, (uint64_t *)params,
+                     seed, mseq0, nseq0);
+  return hipGetLastError();
+}
+
+hipError_t prepare_kernels(uint32_t n) {
+  const size_t lds = measure_lds_bytes(n);
+  hipError_t e = ensure_lds_attr((const void *)k_measure, lds);
+  if (e == hipSuccess) e = ensure_lds_attr((const void *)k_program, lds);
+  return e;
+}
+
+}  // namespace pq

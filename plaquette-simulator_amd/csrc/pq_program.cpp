@@ -1,0 +1,360 @@
+// Op-list programs: validation, compilation for the persistent kernel, and the
+// three execution forms (plain per-op launches, one persistent kernel, a
+// captured hipGraph of the persistent form).
+//
+// Compilation of a gate segment (a maximal run of non-measurement ops):
+//  * ASAP layering on qubit dependencies.  Two ops on disjoint qubits commute
+//    bit-exactly on the tableau: each rewrites only its own columns and XORs
+//    into r a function of those columns (GateFunctors.hpp rules), so any
+//    order that keeps per-qubit order gives identical words.
+//  * A NOISE op on qubit q is fused into the op that last touched q in the
+//    segment (applied to the columns that op produced), else kept alone.
+//  * Each layer is cut into batches of at most 8 qubit-disjoint ops; the
+//    kernel issues a whole batch's loads before its stores.
+// Measurement segments keep program order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "plaquette_amd.h"
+#include "pq_capi_internal.h"
+#include "pq_common.h"
+#include "pq_kernels.h"
+#include "pq_program_internal.h"
+
+using namespace pq;
+
+#define HIPCHK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return pq_set_error(PQ_ERR_HIP,                                             \
+                          (std::string(#expr) + ": " + hipGetErrorString(e_)).c_str()); \
+  } while (0)
+
+namespace {
+
+bool is_two(uint32_t c) { return c == PQ_OP_CX || c == PQ_OP_CZ; }
+bool is_flag(uint32_t c) {
+  return c == PQ_OP_H_FLAG || c == PQ_OP_S_FLAG || c == PQ_OP_X_FLAG ||
+         c == PQ_OP_Y_FLAG || c == PQ_OP_Z_FLAG;
+}
+
+int validate(uint64_t n, const pq_op *ops, uint64_t nops) {
+  for (uint64_t i = 0; i < nops; ++i) {
+    const pq_op &o = ops[i];
+    if (o.code < PQ_OP_H || o.code > PQ_OP_M)
+      return pq_set_error(PQ_ERR_ARG, ("op " + std::to_string(i) + ": unknown code").c_str());
+    if (o.q0 >= n)
+      return pq_set_error(PQ_ERR_ARG, ("op " + std::to_string(i) + ": qubit out of range").c_str());
+    if (is_two(o.code) && (o.q1 >= n || o.q1 == o.q0))
+      return pq_set_error(PQ_ERR_ARG,
+                          ("op " + std::to_string(i) + ": two-qubit op needs distinct qubits in range").c_str());
+    if (o.code == PQ_OP_NOISE && !(o.t0 <= o.t1 && o.t1 <= o.t2))
+      return pq_set_error(PQ_ERR_ARG, ("op " + std::to_string(i) + ": noise thresholds not cumulative").c_str());
+  }
+  return PQ_OK;
+}
+
+}  // namespace
+
+// Compile `ops` into segments/instructions (host side only).
+int compile_program(pq_program_s *p) {
+  const pq_op *ops = p->ops.data();
+  const uint64_t nops = p->ops.size();
+  uint32_t nnoise = 0, nmeas = 0;
+  uint64_t i = 0;
+  while (i < nops) {
+    const bool meas = ops[i].code == PQ_OP_M;
+    uint64_t j = i;
+    while (j < nops && (ops[j].code == PQ_OP_M) == meas) ++j;
+    DSegment sg{meas ? 1u : 0u, (uint32_t)p->instrs.size(), 0, 0};
+    if (meas) {
+      for (uint64_t k = i; k < j; ++k) {
+        DInstr I{PQ_OP_M, ops[k].q0, 0, ops[k].slot, nmeas, 0, ops[k].t0, 0};
+        ++nmeas;
+        p->instrs.push_back(I);
+      }
+    } else {
+      // ASAP layering with noise fusion
+      std::vector<DInstr> seg;
+      std::vector<uint32_t> layer;
+      std::vector<int64_t> last(p->n, -1);
+      std::vector<int32_t> qlayer(p->n, -1);
+      std::vector<char> noise_only;
+      for (uint64_t k = i; k < j; ++k) {
+        const pq_op &o = ops[k];
+        if (o.code == PQ_OP_NOISE) {
+          const uint32_t kn = nnoise++;
+          p->noise.push_back(NoiseDesc{0, o.t0, o.t1, o.t2, 0});
+          const int64_t l = last[o.q0];
+          if (l >= 0 && !noise_only[l]) {
+            DInstr &I = seg[l];
+            if (I.qa == o.q0 && I.na == kNone) { I.na = kn; continue; }
+            if (is_two(I.code) && I.qb == o.q0 && I.nb == kNone) { I.nb = kn; continue; }
+          }
+          const int32_t L = qlayer[o.q0] + 1;
+          qlayer[o.q0] = L;
+          last[o.q0] = (int64_t)seg.size();
+          seg.push_back(DInstr{PQ_OP_NOISE, o.q0, 0, 0, kn, kNone, 0, 0});
+          layer.push_back((uint32_t)L);
+          noise_only.push_back(1);
+          continue;
+        }
+        int32_t L = qlayer[o.q0] + 1;
+        if (is_two(o.code)) L = std::max(L, qlayer[o.q1] + 1);
+        qlayer[o.q0] = L;
+        last[o.q0] = (int64_t)seg.size();
+        if (is_two(o.code)) {
+          qlayer[o.q1] = L;
+          last[o.q1] = (int64_t)seg.size();
+        }
+        seg.push_back(DInstr{o.code, o.q0, is_two(o.code) ? o.q1 : 0, o.slot, kNone, kNone, 0, 0});
+        layer.push_back((uint32_t)L);
+        noise_only.push_back(0);
+        if (o.code >= PQ_OP_H && o.code <= PQ_OP_CZ) ++p->ngates;
+      }
+      std::vector<uint32_t> order(seg.size());
+      for (uint32_t k = 0; k < order.size(); ++k) order[k] = k;
+      std::stable_sort(order.begin(), order.end(),
+                       [&](uint32_t a, uint32_t b) { return layer[a] < layer[b]; });
+      uint32_t k = 0;
+      while (k < order.size()) {
+        uint32_t e = k;
+        while (e < order.size() && layer[order[e]] == layer[order[k]]) ++e;
+        for (uint32_t b = k; b < e; b += kProgramBatch) {
+          const uint32_t len = std::min<uint32_t>(kProgramBatch, e - b);
+          for (uint32_t u = 0; u < len; ++u) {
+            DInstr I = seg[order[b + u]];
+            I.pad = (u == 0) ? len : 0;
+            p->instrs.push_back(I);
+          }
+        }
+        k = e;
+      }
+    }
+    sg.count = (uint32_t)(p->instrs.size() - sg.begin);
+    if (sg.count) p->segs.push_back(sg);
+    i = j;
+  }
+  p->nmeas = nmeas;
+  p->nnoise = nnoise;
+  return PQ_OK;
+}
+
+pq_program_s::~pq_program_s() {
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (d_segs) (void)hipFree(d_segs);
+  if (d_instrs) (void)hipFree(d_instrs);
+  if (d_noise) (void)hipFree(d_noise);
+  if (d_planes) (void)hipFree(d_planes);
+}
+
+extern "C" int pq_program_create(pq_state s, const pq_op *ops, uint64_t nops,
+                                 pq_program *out) {
+  if (!s || !out || (!ops && nops)) return pq_set_error(PQ_ERR_ARG, "null argument");
+  int rc = validate(s->n, ops, nops);
+  if (rc) return rc;
+  auto p = std::make_unique<pq_program_s>();
+  p->n = s->n;
+  p->W = s->W;
+  p->R = s->R;
+  p->ops.assign(ops, ops + nops);
+  rc = compile_program(p.get());
+  if (rc) return rc;
+  // upload
+  if (!p->segs.empty()) {
+    HIPCHK(hipMalloc(&p->d_segs, p->segs.size() * sizeof(DSegment)));
+    HIPCHK(hipMemcpy(p->d_segs, p->segs.data(), p->segs.size() * sizeof(DSegment),
+                     hipMemcpyHostToDevice));
+  }
+  if (!p->instrs.empty()) {
+    HIPCHK(hipMalloc(&p->d_instrs, p->instrs.size() * sizeof(DInstr)));
+    HIPCHK(hipMemcpy(p->d_instrs, p->instrs.data(), p->instrs.size() * sizeof(DInstr),
+                     hipMemcpyHostToDevice));
+  }
+  if (p->nnoise) {
+    HIPCHK(hipMalloc(&p->d_noise, p->noise.size() * sizeof(NoiseDesc)));
+    HIPCHK(hipMemcpy(p->d_noise, p->noise.data(), p->noise.size() * sizeof(NoiseDesc),
+                     hipMemcpyHostToDevice));
+    const uint64_t bytes = (uint64_t)p->nnoise * 2 * std::max<uint64_t>(p->W, 1) * 8;
+    if (hipMalloc(&p->d_planes, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return pq_set_error(PQ_ERR_OOM, "noise plane allocation failed");
+    }
+  }
+  uint64_t th = ((s->R + 63) / 64) * 64;
+  p->threads = (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(64, th));
+  p->blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(s->W, 1u << 20));
+  HIPCHK(prepare_kernels((uint32_t)s->n));
+  *out = p.release();
+  return PQ_OK;
+}
+
+extern "C" int pq_program_destroy(pq_program p) {
+  delete p;
+  return PQ_OK;
+}
+
+extern "C" int pq_program_info(pq_program p, uint64_t *gates, uint64_t *measurements,
+                               uint64_t *noise_ops, uint64_t *instructions) {
+  if (!p) return pq_set_error(PQ_ERR_ARG, "null program");
+  if (gates) *gates = p->ngates;
+  if (measurements) *measurements = p->nmeas;
+  if (noise_ops) *noise_ops = p->nnoise;
+  if (instructions) *instructions = p->instrs.size();
+  return PQ_OK;
+}
+
+extern "C" int pq_program_set_launch(pq_program p, uint32_t threads, uint32_t blocks) {
+  if (!p) return pq_set_error(PQ_ERR_ARG, "null program");
+  if (threads) {
+    if (threads % 64 || threads > 1024)
+      return pq_set_error(PQ_ERR_ARG, "threads must be a multiple of 64, at most 1024");
+    p->threads = threads;
+  }
+  if (blocks) p->blocks = blocks;
+  if (p->gexec) {
+    (void)hipGraphExecDestroy(p->gexec);
+    p->gexec = nullptr;
+  }
+  if (p->graph) {
+    (void)hipGraphDestroy(p->graph);
+    p->graph = nullptr;
+  }
+  return PQ_OK;
+}
+
+// Record slots: M ops write slot <= current count (== count appends), flagged
+// ops read slot < current count, checked in program order.
+static int prepare_records(pq_state s, pq_program p) {
+  uint64_t cur = s->nrec;
+  for (uint64_t i = 0; i < p->ops.size(); ++i) {
+    const pq_op &o = p->ops[i];
+    if (o.code == PQ_OP_M) {
+      if (o.slot > cur)
+        return pq_set_error(PQ_ERR_RANGE, ("op " + std::to_string(i) + ": record id out of range").c_str());
+      if (o.slot == cur) ++cur;
+    } else if (is_flag(o.code) && o.slot >= cur) {
+      return pq_set_error(PQ_ERR_RANGE, ("op " + std::to_string(i) + ": flag record out of range").c_str());
+    }
+  }
+  int rc = ensure_records(s, cur);
+  if (rc) return rc;
+  for (const pq_op &o : p->ops)
+    if (o.code == PQ_OP_M) s->rec_valid[o.slot] = 0;
+  return PQ_OK;
+}
+
+static int run_plain(pq_state s, pq_program p) {
+  for (const pq_op &o : p->ops) {
+    int rc = PQ_OK;
+    switch (o.code) {
+      case PQ_OP_H: case PQ_OP_S: case PQ_OP_X: case PQ_OP_Y: case PQ_OP_Z:
+      case PQ_OP_CX: case PQ_OP_CZ:
+        rc = plain_gate(s, o.code, o.q0, o.q1);
+        break;
+      case PQ_OP_NOISE:
+        rc = plain_noise(s, o.q0, o.t0, o.t1, o.t2);
+        break;
+      case PQ_OP_M:
+        rc = plain_measure(s, o.q0, o.t0, o.slot, nullptr);
+        break;
+      default:
+        rc = plain_flag_plane(s, o.code, o.q0, record_ptr(s, o.slot));
+        break;
+    }
+    if (rc) return rc;
+  }
+  return PQ_OK;
+}
+
+static ProgramArgs program_args(pq_state s, pq_program p) {
+  ProgramArgs a{};
+  a.tab = s->tab;
+  a.recs = s->recs;
+  a.planes = p->d_planes;
+  a.segs = p->d_segs;
+  a.instrs = p->d_instrs;
+  a.coin_planes = nullptr;
+  a.nseg = (uint32_t)p->segs.size();
+  a.n = (uint32_t)s->n;
+  a.R = (uint32_t)s->R;
+  a.C = (uint32_t)s->C;
+  a.W = s->W;
+  a.word0 = 0;
+  a.nwords = s->W;
+  return a;
+}
+
+static int enqueue_persistent(pq_state s, pq_program p) {
+  if (p->nnoise)
+    HIPCHK(launch_program_noise(s->stream, p->d_planes, p->d_noise,
+                                (uint32_t)p->nnoise, s->W, s->params_dev));
+  HIPCHK(launch_program(s->stream, program_args(s, p), p->threads, p->blocks,
+                        s->params_dev));
+  return PQ_OK;
+}
+
+extern "C" int pq_program_run(pq_state s, pq_program p, int form) {
+  if (!s || !p) return pq_set_error(PQ_ERR_ARG, "null argument");
+  if (p->n != s->n || p->W != s->W)
+    return pq_set_error(PQ_ERR_ARG, "program was compiled for a state of another shape");
+  int rc = prepare_records(s, p);
+  if (rc) return rc;
+  if (!s->tab) {
+    s->meas_seq += p->nmeas;
+    s->noise_seq += p->nnoise;
+    return PQ_OK;
+  }
+  if (form == PQ_RUN_PLAIN) return run_plain(s, p);
+  if (s->n > measure_max_qubits())
+    return pq_set_error(PQ_ERR_UNSUPPORTED, "persistent form supports at most 4095 qubits");
+  HIPCHK(launch_set_params(s->stream, s->params_dev, (uint64_t)s->seed, s->meas_seq,
+                           s->noise_seq));
+  s->meas_seq += p->nmeas;
+  s->noise_seq += p->nnoise;
+  if (form == PQ_RUN_PERSISTENT) return enqueue_persistent(s, p);
+  if (form != PQ_RUN_GRAPH) return pq_set_error(PQ_ERR_ARG, "unknown run form");
+  if (p->gexec && (p->graph_state != s || p->graph_arena_gen != s->arena_gen)) {
+    (void)hipGraphExecDestroy(p->gexec);
+    (void)hipGraphDestroy(p->graph);
+    p->gexec = nullptr;
+    p->graph = nullptr;
+  }
+  if (!p->gexec) {
+    HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_persistent(s, p);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    HIPCHK(e);
+    p->graph = g;
+    HIPCHK(hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0));
+    p->graph_state = s;
+    p->graph_arena_gen = s->arena_gen;
+  }
+  HIPCHK(hipGraphLaunch(p->gexec, s->stream));
+  return PQ_OK;
+}
+
+extern "C" int pq_run_program(pq_state s, const pq_op *ops, uint64_t nops, int form) {
+  pq_program p = nullptr;
+  int rc = pq_program_create(s, ops, nops, &p);
+  if (rc) return rc;
+  rc = pq_program_run(s, p, form);
+  if (rc == PQ_OK && form != PQ_RUN_PLAIN) {
+    // the program's device buffers are freed below: wait for the kernels
+    if (hipStreamSynchronize(s->stream) != hipSuccess)
+      rc = pq_set_error(PQ_ERR_HIP, "program execution failed");
+  }
+  pq_program_destroy(p);
+  return rc;
+}

@@ -1,0 +1,33 @@
+// Internal representation of a compiled program.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "plaquette_amd.h"
+#include "pq_common.h"
+
+constexpr uint32_t kProgramBatch = 8;  // must equal kBatch in pq_kernels.hip
+
+struct pq_program_s {
+  uint64_t n = 0, W = 0, R = 0;
+  std::vector<pq_op> ops;
+  uint64_t ngates = 0, nmeas = 0, nnoise = 0;
+  std::vector<pq::DSegment> segs;
+  std::vector<pq::DInstr> instrs;
+  std::vector<pq::NoiseDesc> noise;
+  pq::DSegment *d_segs = nullptr;
+  pq::DInstr *d_instrs = nullptr;
+  pq::NoiseDesc *d_noise = nullptr;
+  uint64_t *d_planes = nullptr;  // noise planes [nnoise][2][W]
+  uint32_t threads = 0, blocks = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  pq_state graph_state = nullptr;
+  uint64_t graph_arena_gen = 0;
+  ~pq_program_s();
+};
+
+int compile_program(pq_program_s *p);

@@ -1,0 +1,5 @@
+"""Re-export of the compiled engine, as the reference's simulator.py:1 does.
+
+Importing fails loudly when the HIP extension is not built: there is no CPU
+fallback."""
+from plaquette_simulator_bindings import CliffordState, Program  # noqa: F401
