@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched stabilizer-tableau engine on one MI355X.
+
+Workload: rotated surface-code memory-Z experiment, distance 9 (161 qubits),
+9 rounds, 16384 shots, Pauli X/Y/Z noise p=1e-3 each after every gate layer,
+seed 7 (3312 gates, 801 measurements, 5904 noise ops).  One step = reset the
+tableau to |0...0> and run the whole circuit (synthetic, generated in
+process).  The timed form is the batched persistent kernel; the plain form
+(one kernel launch per op, as the reference issues them) and the hipGraph
+replay of the persistent form are timed beside it.
+
+Output: one JSON line (rank 0) with gate-shots/s as `value`, a `roofline`
+object for the persistent kernel (algorithmic bytes per launch from the
+engine's own work census / HIP-event kernel time) and a `cpu_baseline`
+object (the CPU oracle, OpenMP over shots/rows, on a bounded shot sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "plaquette-simulator_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "gate-shots/s on d=9 surface code (%HBM peak); batched vs plain speedup"
+
+# bytes per op per (row x word group), unfused, at bit-plane density
+GATE_PLANES = {1: 6, 2: 5, 3: 3, 4: 4, 5: 3, 6: 8, 7: 8}
+
+
+def algorithmic_bytes(ops, n, W, census):
+    """Algorithmic HBM bytes of one run of `ops` on n qubits, W word groups.
+
+    Gates: planes touched x R rows x W words x 8 B (H 6, S 5, CX/CZ 8,
+    X/Z 3, Y 4).  Flagged H/S: as H/S plus the flag plane (8W).  Conditional
+    Paulis and noise: read x_q, z_q, r and write r (4 planes) plus 8W per
+    mask (1 for conditional, 2 for noise).  Measurements, per word group:
+    random 8[(4+2K)R + 2], determined 8[2n + (K'+1)R + 2], K = rows rowsummed,
+    K' = rowsum steps, summed over word groups by the engine's census."""
+    import numpy as np
+    R = 2 * n + 1
+    codes = ops[:, 0]
+    out = {"gates": 0, "flagged": 0, "noise": 0, "measurements": 0}
+    for c, planes in GATE_PLANES.items():
+        out["gates"] += int((codes == c).sum()) * planes * 8 * R * W
+    out["flagged"] += int((codes == 8).sum()) * (48 * R * W + 8 * W)
+    out["flagged"] += int((codes == 9).sum()) * (40 * R * W + 8 * W)
+    out["flagged"] += int(np.isin(codes, (11, 12, 13)).sum()) * (32 * R * W + 8 * W)
+    out["noise"] += int((codes == 10).sum()) * (32 * R * W + 16 * W)
+    nr, nd = census["random"], census["determined"]
+    kr, kd = census["random_rows"], census["determined_steps"]
+    out["measurements"] = 8 * ((4 * R + 2) * nr + 2 * R * kr) + 8 * ((2 * n + R + 2) * nd + R * kd)
+    out["total"] = sum(out.values())
+    return out
+
+
+def cpu_baseline(w, target_s):
+    """The CPU oracle (reference int32 layout, OpenMP) on the first S shots of
+    the same circuit; S is scaled from a 64-shot calibration run so the timed
+    run takes about `target_s` seconds.  Shots are independent (coins and
+    noise keyed by shot index), so S shots are an exact subset of the job."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    O.set_threads(cores)
+    ops32 = w.ops.astype(np.int32)
+    cal = O.OracleState(w.num_qubits, 64, w.seed)
+    t = time.perf_counter()
+    cal.run_program(ops32)
+    t64 = time.perf_counter() - t
+    S = int(min(w.batch_size, max(64, 64 * round(target_s / max(t64, 1e-3)))))
+    st = O.OracleState(w.num_qubits, S, w.seed)
+    t = time.perf_counter()
+    st.run_program(ops32)
+    dt = time.perf_counter() - t
+    return {
+        "value": w.gates * S / dt,
+        "unit": "gate-shots/s",
+        "cores": O.get_threads(),
+        "kind": "port",
+        "sample": f"{w.name}: full circuit on shots [0,{S}) of {w.batch_size}, "
+                  f"{dt:.2f} s, int32 [B][2n+1][n] layout",
+        "meas_shots_per_s": w.measurements * S / dt,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--shots", type=int, default=16384)
+    ap.add_argument("--threads", type=int, default=0, help="persistent kernel block size")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs on a single MI355X "
+              f"(--gpus {args.gpus} is not supported; shots shard with no exchange, "
+              f"see the sharding notes in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import plaquette_simulator as ps
+    from plaquette_simulator import workloads as W
+
+    w = W.config_surface_d9(shots=args.shots)
+    n, B = w.num_qubits, w.batch_size
+    words = (B + 63) // 64
+    st = ps.CliffordState(n, B, w.seed, True)
+    prog = ps.Program(st, w.ops)
+    if args.threads:
+        prog.set_launch(args.threads, 0)
+
+    # work census (untimed run)
+    prog.set_census(True)
+    st.reset_clifford_state()
+    prog.run(st, "persistent")
+    census = prog.read_census()
+    prog.set_census(False)
+    abytes = algorithmic_bytes(w.ops, n, words, census)
+
+    # persistent form: warmup, then K timed steps with HIP events per kernel
+    for _ in range(args.warmup):
+        st.reset_clifford_state()
+        prog.run(st, "persistent")
+    st.synchronize()
+    prog.kernel_ms()
+    prog.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st.reset_clifford_state()
+        prog.run(st, "persistent")
+    st.synchronize()
+    wall = time.perf_counter() - t0
+    kms, launches = prog.kernel_ms()
+    prog.set_timing(False)
+    ms_step = 1e3 * wall / args.steps
+    kernel_ms = kms / max(1, launches)
+
+    # graph form (same kernels, captured once, replayed)
+    prog.run(st, "graph")
+    st.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st.reset_clifford_state()
+        prog.run(st, "graph")
+    st.synchronize()
+    graph_ms = 1e3 * (time.perf_counter() - t0) / args.steps
+
+    # plain form: one kernel launch per op
+    st.reset_clifford_state()
+    prog.run(st, "plain")
+    st.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.plain_steps):
+        st.reset_clifford_state()
+        prog.run(st, "plain")
+    st.synchronize()
+    plain_ms = 1e3 * (time.perf_counter() - t0) / args.plain_steps
+
+    gate_shots = w.gates * B
+    value = gate_shots / (ms_step / 1e3)
+    achieved = abytes["total"] / (kernel_ms / 1e3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "gate-shots/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "surface_code_memory_z_d9", "qubits": n, "rounds": 9,
+                   "shots": B, "p": 1e-3, "seed": w.seed, "gates": w.gates,
+                   "measurements": w.measurements, "noise_ops": w.noise_ops,
+                   "form": "persistent"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_program", "kernel_ms": kernel_ms,
+                     "algorithmic_bytes": abytes["total"]},
+        "meas_shots_per_s": w.measurements * B / (ms_step / 1e3),
+        "hbm_frac_step": abytes["total"] / (ms_step / 1e3) / (HBM_PEAK_GBS * 1e9),
+        "graph_ms_per_step": graph_ms,
+        "graph_gate_shots_per_s": gate_shots / (graph_ms / 1e3),
+        "plain_ms_per_step": plain_ms,
+        "plain_gate_shots_per_s": gate_shots / (plain_ms / 1e3),
+        "speedup_vs_plain": plain_ms / ms_step,
+        "algorithmic_bytes": abytes,
+        "census": census,
+    }
+    traffic_file = os.path.join(ROOT, "profiles", "k_program_surface_d9_traffic.json")
+    if os.path.exists(traffic_file):
+        tr = json.load(open(traffic_file))
+        out["roofline"]["traffic"] = tr.get("bytes_per_launch")
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

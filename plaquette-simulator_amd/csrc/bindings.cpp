@@ -231,6 +231,27 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
            })
       .def("set_launch", [](Program &p, uint32_t threads, uint32_t blocks) {
         check(pq_program_set_launch(p.p, threads, blocks), "set_launch");
+      })
+      .def("set_census",
+           [](Program &p, bool on) { check(pq_program_set_census(p.p, on ? 1 : 0), "set_census"); })
+      .def("read_census",
+           [](Program &p) {
+             uint64_t c[4];
+             check(pq_program_read_census(p.s, p.p, c), "read_census");
+             py::dict d;
+             d["random"] = c[0];
+             d["determined"] = c[1];
+             d["random_rows"] = c[2];
+             d["determined_steps"] = c[3];
+             return d;
+           })
+      .def("set_timing",
+           [](Program &p, bool on) { check(pq_program_set_timing(p.p, on ? 1 : 0), "set_timing"); })
+      .def("kernel_ms", [](Program &p) {
+        float ms = 0;
+        uint64_t k = 0;
+        check(pq_program_kernel_ms(p.p, &ms, &k), "kernel_ms");
+        return py::make_tuple(ms, k);
       });
 
   m.def("coin_threshold", &pq_coin_threshold);

@@ -63,7 +63,14 @@ struct ProgramArgs {
   uint64_t W;
   uint64_t word0;  // first word group handled by this launch
   uint64_t nwords; // word groups handled by this launch
+  // optional work census (4 counters, accumulated over word groups):
+  // random measurements, determined measurements, random rowsum rows,
+  // determined rowsum steps
+  unsigned long long *census;
+  uint32_t det_waves;  // waves that run fully-determined measurements in parallel
 };
+// Dynamic LDS bytes k_program needs for n qubits and det_waves slabs.
+size_t program_lds_bytes(uint32_t n, uint32_t det_waves);
 // RowMultiply(i <- i*k) on every shot, or only the sign into sign_out.
 hipError_t launch_rowsum(hipStream_t s, uint64_t *tab, const Geom &g, uint32_t i,
                          uint32_t k, uint64_t *sign_out);

@@ -235,6 +235,7 @@ struct MeasShared {
   uint32_t gp[64];
   uint32_t ngroups;
   uint32_t nlist;
+  uint32_t krows;  // census: rows rowsummed by random groups
   uint64_t rm;    // random shots
   uint64_t coin;  // coin word
 };
@@ -244,7 +245,8 @@ __device__ void measure_word(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
                              uint64_t coin_key, const uint64_t *coin_plane,
                              uint64_t *rec_res, uint64_t *rec_det,
                              uint64_t *s_src, uint32_t *s_list,
-                             MeasShared &sh, bool force_determined = false) {
+                             MeasShared &sh, bool force_determined = false,
+                             unsigned long long *census = nullptr) {
   const uint32_t tid = threadIdx.x, NT = blockDim.x;
   const uint64_t Ru = R;
   const uint64_t *xq = T + (uint64_t)q * Ru;
@@ -254,7 +256,10 @@ __device__ void measure_word(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
   const uint32_t p0 = min(n, tid * cp), p1 = min(n, p0 + cp);
   uint64_t acc = 0;
   for (uint32_t p = p0; p < p1; ++p) acc |= xq[n + p];
-  if (tid == 0) sh.ngroups = 0;
+  if (tid == 0) {
+    sh.ngroups = 0;
+    sh.krows = 0;
+  }
   uint64_t rm;
   uint64_t before = block_excl_or(acc, sh.scan, &rm);
   if (force_determined) rm = 0;
@@ -305,10 +310,12 @@ __device__ void measure_word(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
     __syncthreads();
     const uint32_t nl = sh.nlist;
     const uint64_t rsrc = s_src[2 * n];
+    uint32_t my_rows = 0;
     for (uint32_t i = tid; i < 2 * n; i += NT) {
       if (i == p || i == n + p) continue;
       const uint64_t mi = M & xq[i];
       if (!mi) continue;
+      ++my_rows;
       uint64_t c0 = 0, c1 = 0;
       for (uint32_t k = 0; k < nl; ++k) {
         const uint32_t j = s_list[k];
@@ -322,17 +329,18 @@ __device__ void measure_word(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
       }
       T[(uint64_t)(2 * n) * Ru + i] ^= mi & (rsrc ^ c1);
     }
+    if (census && my_rows) atomicAdd(&sh.krows, my_rows);
     __syncthreads();
   }
 
   // ---- 3. determined shots ----
   const uint64_t D = ~rm;
   uint64_t rdet = 0;
+  uint32_t nchain = 0;
   if (D) {
     // ordered list of destabilizers i with x_q set in some determined shot
     uint32_t cnt = 0;
     for (uint32_t i = p0; i < p1; ++i) cnt += (xq[i] & D) ? 1u : 0u;
-    uint32_t nchain;
     uint32_t pos = block_excl_sum(cnt, sh.scan, &nchain);
     for (uint32_t i = p0; i < p1; ++i)
       if (xq[i] & D) s_list[pos++] = i;
@@ -372,6 +380,12 @@ __device__ void measure_word(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
   if (tid == 0) {
     rec_res[w] = (coin & rm) | (rdet & D);
     rec_det[w] = D;
+    if (census) {
+      atomicAdd(census + 0, rm ? 1ull : 0ull);
+      atomicAdd(census + 1, D ? 1ull : 0ull);
+      atomicAdd(census + 2, (unsigned long long)sh.krows);
+      atomicAdd(census + 3, (unsigned long long)nchain);
+    }
   }
   __syncthreads();
 }
@@ -525,6 +539,8 @@ struct ProgParams {
 };
 
 constexpr int kBatch = 8;
+constexpr uint32_t kProgramMaxThreads = 512;  // block-size bound of k_program
+constexpr uint32_t kMeasBatch = 64;  // measurements pivot-checked per pass
 
 __device__ __forceinline__ void apply_fused_noise(const uint64_t *__restrict__ planes,
                                                   uint64_t W, uint64_t w, uint32_t k,
@@ -542,6 +558,8 @@ __device__ void run_gate_segment(uint64_t *__restrict__ T, uint64_t w,
   const uint32_t n = a.n;
   const uint64_t W = a.W;
   const DInstr *__restrict__ ins = a.instrs;
+  const uint64_t *__restrict__ planes = a.planes;
+  const uint64_t *__restrict__ recs = a.recs;
   for (uint32_t row = threadIdx.x; row < a.R; row += blockDim.x) {
     uint64_t *Tr = T + row;
     uint64_t r = Tr[(uint64_t)(2 * n) * Ru];
@@ -549,19 +567,34 @@ __device__ void run_gate_segment(uint64_t *__restrict__ T, uint64_t w,
     const uint32_t end = sg.begin + sg.count;
     while (k < end) {
       const uint32_t len = ins[k].pad;
+      // load phase: every column word and shot mask the batch needs
       uint64_t xa[kBatch], za[kBatch], xb[kBatch], zb[kBatch];
+      uint64_t fxa[kBatch], fza[kBatch], fxb[kBatch], fzb[kBatch];
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         if (u < (int)len) {
           const DInstr I = ins[k + u];
           xa[u] = Tr[(uint64_t)I.qa * Ru];
           za[u] = Tr[(uint64_t)(n + I.qa) * Ru];
+          xb[u] = zb[u] = 0;
           if (I.code == PQ_OP_CX || I.code == PQ_OP_CZ) {
             xb[u] = Tr[(uint64_t)I.qb * Ru];
             zb[u] = Tr[(uint64_t)(n + I.qb) * Ru];
           }
+          fxa[u] = fza[u] = fxb[u] = fzb[u] = 0;
+          if (I.na != kNone) {
+            fxa[u] = planes[(2ull * I.na) * W + w];
+            fza[u] = planes[(2ull * I.na + 1) * W + w];
+          }
+          if (I.nb != kNone) {
+            fxb[u] = planes[(2ull * I.nb) * W + w];
+            fzb[u] = planes[(2ull * I.nb + 1) * W + w];
+          } else if (I.code >= PQ_OP_H_FLAG && I.code != PQ_OP_NOISE) {
+            fxb[u] = recs[(2ull * I.slot) * W + w];  // flag of a flagged op
+          }
         }
       }
+      // compute phase: registers only, stores are fire-and-forget
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         if (u < (int)len) {
@@ -569,6 +602,7 @@ __device__ void run_gate_segment(uint64_t *__restrict__ T, uint64_t w,
           uint64_t *pxa = Tr + (uint64_t)I.qa * Ru;
           uint64_t *pza = Tr + (uint64_t)(n + I.qa) * Ru;
           uint64_t x = xa[u], z = za[u];
+          const uint64_t f = fxb[u];
           switch (I.code) {
             case PQ_OP_H:
               r ^= x & z;
@@ -590,7 +624,7 @@ __device__ void run_gate_segment(uint64_t *__restrict__ T, uint64_t w,
               z ^= zt;
               Tr[(uint64_t)I.qb * Ru] = xt;
               *pza = z;
-              apply_fused_noise(a.planes, W, w, I.nb, xt, zt, r);
+              r ^= (fxb[u] & zt) ^ (fzb[u] & xt);
               break;
             }
             case PQ_OP_CZ: {
@@ -600,30 +634,27 @@ __device__ void run_gate_segment(uint64_t *__restrict__ T, uint64_t w,
               z ^= xt;
               Tr[(uint64_t)(n + I.qb) * Ru] = zt;
               *pza = z;
-              apply_fused_noise(a.planes, W, w, I.nb, xt, zt, r);
+              r ^= (fxb[u] & zt) ^ (fzb[u] & xt);
               break;
             }
             case PQ_OP_H_FLAG: {
-              const uint64_t f = a.recs[(2ull * I.slot) * W + w];
               const uint64_t d = f & (x ^ z);
               r ^= f & x & z;
               x ^= d; z ^= d;
               *pxa = x; *pza = z;
               break;
             }
-            case PQ_OP_S_FLAG: {
-              const uint64_t f = a.recs[(2ull * I.slot) * W + w];
+            case PQ_OP_S_FLAG:
               r ^= f & x & z;
               z ^= f & x;
               *pza = z;
               break;
-            }
-            case PQ_OP_X_FLAG: r ^= a.recs[(2ull * I.slot) * W + w] & z; break;
-            case PQ_OP_Y_FLAG: r ^= a.recs[(2ull * I.slot) * W + w] & (z ^ x); break;
-            case PQ_OP_Z_FLAG: r ^= a.recs[(2ull * I.slot) * W + w] & x; break;
+            case PQ_OP_X_FLAG: r ^= f & z; break;
+            case PQ_OP_Y_FLAG: r ^= f & (z ^ x); break;
+            case PQ_OP_Z_FLAG: r ^= f & x; break;
             default: break;  // PQ_OP_NOISE: only the fused plane below
           }
-          apply_fused_noise(a.planes, W, w, I.na, x, z, r);
+          r ^= (fxa[u] & z) ^ (fza[u] & x);
         }
       }
       k += len;
@@ -632,15 +663,97 @@ __device__ void run_gate_segment(uint64_t *__restrict__ T, uint64_t w,
   }
 }
 
-__global__ __launch_bounds__(1024) void k_program(pq::ProgramArgs a,
+__device__ __forceinline__ uint64_t wave_or(uint64_t v) {
+#pragma unroll
+  for (uint32_t off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off, 64);
+  return v;
+}
+
+// MeasureDetermined (MeasurementFunctors.hpp:190-201) of a measurement that
+// is determined in all 64 shots of the group, run by one wave: the scratch
+// row lives in the wave's private LDS slab (lane-owned columns), chain
+// members come from a ballot over destabilizer column q in ascending order,
+// and each step's mod-4 phase is reduced across the wave.  Only the last
+// measurement of a batch writes its scratch row back to the tableau.
+__device__ void determined_wave(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
+                                uint32_t R, uint32_t q, uint64_t *slab,
+                                uint64_t *rec, uint64_t W, bool write_scratch,
+                                unsigned long long *census) {
+  const uint32_t lane = lane_id();
+  const uint64_t Ru = R;
+  const uint64_t *xq = T + (uint64_t)q * Ru;
+  const uint64_t scratch = 2ull * n;
+  for (uint32_t j = lane; j < n; j += 64) {
+    slab[j] = 0;
+    slab[n + j] = 0;
+  }
+  uint64_t racc = 0;
+  uint32_t steps = 0;
+  for (uint32_t base = 0; base < n; base += 64) {
+    const uint32_t i = base + lane;
+    const uint64_t xi = i < n ? xq[i] : 0ull;
+    uint64_t bal = __ballot(xi != 0);
+    while (bal) {
+      const uint32_t t = (uint32_t)__ffsll((long long)bal) - 1;
+      bal &= bal - 1;
+      const uint64_t m = __shfl(xi, t, 64);
+      const uint64_t s = n + base + t;
+      uint64_t c0 = 0, c1 = 0;
+      for (uint32_t j = lane; j < n; j += 64) {
+        const uint64_t x2 = T[(uint64_t)j * Ru + s];
+        const uint64_t z2 = T[(uint64_t)(n + j) * Ru + s];
+        const uint64_t x1 = slab[j], z1 = slab[n + j];
+        phase_acc(x1, z1, x2, z2, c0, c1);
+        slab[j] = x1 ^ (m & x2);
+        slab[n + j] = z1 ^ (m & z2);
+      }
+#pragma unroll
+      for (uint32_t off = 32; off > 0; off >>= 1) {
+        const uint64_t t0 = __shfl_xor(c0, off, 64);
+        const uint64_t t1 = __shfl_xor(c1, off, 64);
+        add_mod4(c0, c1, t0, t1);
+      }
+      racc ^= m & (T[scratch * Ru + s] ^ c1);
+      ++steps;
+    }
+  }
+  if (lane == 0) {
+    rec[w] = racc;
+    rec[W + w] = ~0ull;
+    if (census) {
+      atomicAdd(census + 1, 1ull);
+      atomicAdd(census + 3, (unsigned long long)steps);
+    }
+  }
+  if (write_scratch) {
+    for (uint32_t j = lane; j < n; j += 64) {
+      T[(uint64_t)j * Ru + scratch] = slab[j];
+      T[(uint64_t)(n + j) * Ru + scratch] = slab[n + j];
+    }
+    if (lane == 0) T[scratch * Ru + scratch] = racc;
+  }
+}
+
+// Dynamic LDS of k_program: s_src[2n+1] u64 | s_list[n+1] u32 (padded) |
+// s_rm[kMeasBatch] u64 | ndw slabs of 2n u64.
+__host__ __device__ inline size_t program_lds_words(uint32_t n, uint32_t ndw) {
+  return (2ull * n + 1) + ((uint64_t)n + 2) / 2 + kMeasBatch + (uint64_t)ndw * 2 * n;
+}
+
+__global__ __launch_bounds__(kProgramMaxThreads) void k_program(pq::ProgramArgs a,
                                                   const ProgParams *__restrict__ pp) {
   extern __shared__ uint64_t dyn[];
   __shared__ MeasShared sh;
   const uint32_t n = a.n;
   uint64_t *s_src = dyn;
   uint32_t *s_list = reinterpret_cast<uint32_t *>(dyn + (2 * n + 1));
+  uint64_t *s_rm = dyn + (2 * n + 1) + (n + 2) / 2;
+  uint64_t *s_slab = s_rm + kMeasBatch;
   const uint64_t C = 2ull * n + 1;
+  const uint64_t Ru = a.R;
   const uint64_t seed = pp->seed, mseq0 = pp->mseq0;
+  const uint32_t lane = lane_id(), wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const uint32_t ndw = a.det_waves;
   for (uint64_t wi = blockIdx.x; wi < a.nwords; wi += gridDim.x) {
     const uint64_t w = a.word0 + wi;
     uint64_t *T = a.tab + w * C * a.R;
@@ -649,15 +762,51 @@ __global__ __launch_bounds__(1024) void k_program(pq::ProgramArgs a,
       if (sg.kind == 0) {
         run_gate_segment(T, w, a, sg);
         __syncthreads();
-      } else {
-        for (uint32_t k = sg.begin; k < sg.begin + sg.count; ++k) {
-          const DInstr I = a.instrs[k];
+        continue;
+      }
+      uint32_t k = sg.begin;
+      const uint32_t end = sg.begin + sg.count;
+      while (k < end) {
+        // which of the next measurements have a pivot in some shot?
+        const uint32_t nm = min(end - k, kMeasBatch);
+        for (uint32_t idx = wv; idx < nm; idx += nw) {
+          const uint64_t *xq = T + (uint64_t)a.instrs[k + idx].qa * Ru;
+          uint64_t acc = 0;
+          for (uint32_t p = lane; p < n; p += 64) acc |= xq[n + p];
+          acc = wave_or(acc);
+          if (lane == 0) s_rm[idx] = acc;
+        }
+        __syncthreads();
+        uint32_t jr = nm;
+        for (uint32_t idx = 0; idx < nm; ++idx)
+          if (s_rm[idx]) {
+            jr = idx;
+            break;
+          }
+        // measurements [k, k+jr) are determined in every shot: one wave each
+        if (jr > 0) {
+          if (wv < ndw) {
+            uint64_t *slab = s_slab + (uint64_t)wv * 2 * n;
+            for (uint32_t idx = wv; idx < jr; idx += ndw) {
+              const DInstr I = a.instrs[k + idx];
+              uint64_t *rec = a.recs + (2ull * I.slot) * a.W;
+              determined_wave(T, w, n, a.R, I.qa, slab, rec, a.W, idx == jr - 1,
+                              a.census);
+            }
+          }
+          __syncthreads();
+        }
+        if (jr < nm) {
+          const DInstr I = a.instrs[k + jr];
           const uint64_t m = (uint64_t)I.na | ((uint64_t)I.nb << 32);
           const uint64_t key = stream_key(seed, 2 * (mseq0 + m));
           const uint64_t *cp = a.coin_planes ? a.coin_planes + m * a.W : nullptr;
           uint64_t *rec = a.recs + (2ull * I.slot) * a.W;
           measure_word(T, w, n, a.R, I.qa, I.t0, key, cp, rec, rec + a.W, s_src,
-                       s_list, sh);
+                       s_list, sh, false, a.census);
+          k += jr + 1;
+        } else {
+          k += nm;
         }
       }
     }
@@ -762,6 +911,8 @@ size_t measure_lds_bytes(uint32_t n) {
 
 uint32_t measure_max_qubits() { return kMeasMaxQubits; }
 
+constexpr size_t kProgramLdsBudget = 150 * 1024;  // dynamic LDS cap of k_program
+
 static hipError_t ensure_lds_attr(const void *fn, size_t bytes) {
   if (bytes <= 65536) return hipSuccess;
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -829,10 +980,14 @@ hipError_t launch_record_to_int(hipStream_t s, const uint64_t *rec, uint64_t B,
 hipError_t launch_program(hipStream_t s, const ProgramArgs &a, uint32_t threads,
                           uint32_t blocks, const void *params) {
   if (!a.nwords) return hipSuccess;
-  const size_t lds = measure_lds_bytes(a.n);
+  const size_t lds = 8 * program_lds_words(a.n, a.det_waves);
   hipLaunchKernelGGL(k_program, dim3(blocks), dim3(threads), lds, s, a,
                      (const ProgParams *)params);
   return hipGetLastError();
+}
+
+size_t program_lds_bytes(uint32_t n, uint32_t det_waves) {
+  return 8 * program_lds_words(n, det_waves);
 }
 
 hipError_t launch_program_noise(hipStream_t s, uint64_t *planes,
@@ -862,9 +1017,9 @@ hipError_t launch_set_params(hipStream_t s, void *params, uint64_t seed,
 }
 
 hipError_t prepare_kernels(uint32_t n) {
-  const size_t lds = measure_lds_bytes(n);
-  hipError_t e = ensure_lds_attr((const void *)k_measure, lds);
-  if (e == hipSuccess) e = ensure_lds_attr((const void *)k_program, lds);
+  hipError_t e = ensure_lds_attr((const void *)k_measure, measure_lds_bytes(n));
+  if (e == hipSuccess)
+    e = ensure_lds_attr((const void *)k_program, kProgramLdsBudget);
   return e;
 }
 

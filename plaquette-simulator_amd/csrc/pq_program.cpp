@@ -145,9 +145,17 @@ int compile_program(pq_program_s *p) {
   return PQ_OK;
 }
 
-pq_program_s::~pq_program_s() {
+void pq_program_s::drop_graph() {
   if (gexec) (void)hipGraphExecDestroy(gexec);
   if (graph) (void)hipGraphDestroy(graph);
+  gexec = nullptr;
+  graph = nullptr;
+}
+
+pq_program_s::~pq_program_s() {
+  drop_graph();
+  if (d_census) (void)hipFree(d_census);
+  for (hipEvent_t e : ev) (void)hipEventDestroy(e);
   if (d_segs) (void)hipFree(d_segs);
   if (d_instrs) (void)hipFree(d_instrs);
   if (d_noise) (void)hipFree(d_noise);
@@ -188,7 +196,7 @@ extern "C" int pq_program_create(pq_state s, const pq_op *ops, uint64_t nops,
     }
   }
   uint64_t th = ((s->R + 63) / 64) * 64;
-  p->threads = (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(64, th));
+  p->threads = (uint32_t)std::min<uint64_t>(kProgramThreadsMax, std::max<uint64_t>(64, th));
   p->blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(s->W, 1u << 20));
   HIPCHK(prepare_kernels((uint32_t)s->n));
   *out = p.release();
@@ -213,19 +221,12 @@ extern "C" int pq_program_info(pq_program p, uint64_t *gates, uint64_t *measurem
 extern "C" int pq_program_set_launch(pq_program p, uint32_t threads, uint32_t blocks) {
   if (!p) return pq_set_error(PQ_ERR_ARG, "null program");
   if (threads) {
-    if (threads % 64 || threads > 1024)
-      return pq_set_error(PQ_ERR_ARG, "threads must be a multiple of 64, at most 1024");
+    if (threads % 64 || threads > kProgramThreadsMax)
+      return pq_set_error(PQ_ERR_ARG, "threads must be a multiple of 64, at most 512");
     p->threads = threads;
   }
   if (blocks) p->blocks = blocks;
-  if (p->gexec) {
-    (void)hipGraphExecDestroy(p->gexec);
-    p->gexec = nullptr;
-  }
-  if (p->graph) {
-    (void)hipGraphDestroy(p->graph);
-    p->graph = nullptr;
-  }
+  p->drop_graph();
   return PQ_OK;
 }
 
@@ -288,15 +289,38 @@ static ProgramArgs program_args(pq_state s, pq_program p) {
   a.W = s->W;
   a.word0 = 0;
   a.nwords = s->W;
+  a.census = p->census_on ? p->d_census : nullptr;
+  // waves that run fully-determined measurements in parallel, each with a
+  // 2n-word LDS slab, within the kernel's dynamic-LDS budget
+  const uint64_t waves = p->threads / 64;
+  const uint64_t budget = 150 * 1024;
+  const uint64_t base = program_lds_bytes(a.n, 0);
+  const uint64_t slab = 16ull * a.n;
+  uint64_t ndw = (budget > base && slab) ? (budget - base) / slab : 0;
+  a.det_waves = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(waves, ndw));
   return a;
 }
 
-static int enqueue_persistent(pq_state s, pq_program p) {
+static int enqueue_persistent(pq_state s, pq_program p, bool timed) {
   if (p->nnoise)
     HIPCHK(launch_program_noise(s->stream, p->d_planes, p->d_noise,
                                 (uint32_t)p->nnoise, s->W, s->params_dev));
+  if (timed) {
+    if (p->ev_used + 2 > p->ev.size()) {
+      hipEvent_t a, b;
+      HIPCHK(hipEventCreate(&a));
+      HIPCHK(hipEventCreate(&b));
+      p->ev.push_back(a);
+      p->ev.push_back(b);
+    }
+    HIPCHK(hipEventRecord(p->ev[p->ev_used], s->stream));
+  }
   HIPCHK(launch_program(s->stream, program_args(s, p), p->threads, p->blocks,
                         s->params_dev));
+  if (timed) {
+    HIPCHK(hipEventRecord(p->ev[p->ev_used + 1], s->stream));
+    p->ev_used += 2;
+  }
   return PQ_OK;
 }
 
@@ -318,7 +342,7 @@ extern "C" int pq_program_run(pq_state s, pq_program p, int form) {
                            s->noise_seq));
   s->meas_seq += p->nmeas;
   s->noise_seq += p->nnoise;
-  if (form == PQ_RUN_PERSISTENT) return enqueue_persistent(s, p);
+  if (form == PQ_RUN_PERSISTENT) return enqueue_persistent(s, p, p->timing_on);
   if (form != PQ_RUN_GRAPH) return pq_set_error(PQ_ERR_ARG, "unknown run form");
   if (p->gexec && (p->graph_state != s || p->graph_arena_gen != s->arena_gen)) {
     (void)hipGraphExecDestroy(p->gexec);
@@ -328,7 +352,7 @@ extern "C" int pq_program_run(pq_state s, pq_program p, int form) {
   }
   if (!p->gexec) {
     HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    rc = enqueue_persistent(s, p);
+    rc = enqueue_persistent(s, p, false);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s->stream, &g);
     if (rc) {
@@ -342,6 +366,48 @@ extern "C" int pq_program_run(pq_state s, pq_program p, int form) {
     p->graph_arena_gen = s->arena_gen;
   }
   HIPCHK(hipGraphLaunch(p->gexec, s->stream));
+  return PQ_OK;
+}
+
+extern "C" int pq_program_set_census(pq_program p, int enable) {
+  if (!p) return pq_set_error(PQ_ERR_ARG, "null program");
+  if (enable && !p->d_census) HIPCHK(hipMalloc(&p->d_census, 4 * sizeof(unsigned long long)));
+  if (enable) HIPCHK(hipMemset(p->d_census, 0, 4 * sizeof(unsigned long long)));
+  if ((bool)enable != p->census_on) p->drop_graph();
+  p->census_on = enable != 0;
+  return PQ_OK;
+}
+
+extern "C" int pq_program_read_census(pq_state s, pq_program p, uint64_t out[4]) {
+  if (!s || !p || !out) return pq_set_error(PQ_ERR_ARG, "null argument");
+  for (int i = 0; i < 4; ++i) out[i] = 0;
+  if (!p->d_census) return PQ_OK;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  unsigned long long h[4];
+  HIPCHK(hipMemcpy(h, p->d_census, sizeof(h), hipMemcpyDeviceToHost));
+  for (int i = 0; i < 4; ++i) out[i] = h[i];
+  return PQ_OK;
+}
+
+extern "C" int pq_program_set_timing(pq_program p, int enable) {
+  if (!p) return pq_set_error(PQ_ERR_ARG, "null program");
+  p->timing_on = enable != 0;
+  p->ev_used = 0;
+  return PQ_OK;
+}
+
+extern "C" int pq_program_kernel_ms(pq_program p, float *total_ms, uint64_t *launches) {
+  if (!p || !total_ms) return pq_set_error(PQ_ERR_ARG, "null argument");
+  float tot = 0;
+  for (size_t i = 0; i + 1 < p->ev_used; i += 2) {
+    float ms = 0;
+    HIPCHK(hipEventSynchronize(p->ev[i + 1]));
+    HIPCHK(hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]));
+    tot += ms;
+  }
+  *total_ms = tot;
+  if (launches) *launches = p->ev_used / 2;
+  p->ev_used = 0;
   return PQ_OK;
 }
 

@@ -10,6 +10,7 @@
 #include "pq_common.h"
 
 constexpr uint32_t kProgramBatch = 8;  // must equal kBatch in pq_kernels.hip
+constexpr uint32_t kProgramThreadsMax = 512;  // must equal kProgramMaxThreads there
 
 struct pq_program_s {
   uint64_t n = 0, W = 0, R = 0;
@@ -27,6 +28,12 @@ struct pq_program_s {
   hipGraphExec_t gexec = nullptr;
   pq_state graph_state = nullptr;
   uint64_t graph_arena_gen = 0;
+  unsigned long long *d_census = nullptr;  // 4 counters when census is on
+  bool census_on = false;
+  bool timing_on = false;
+  std::vector<hipEvent_t> ev;  // start/end pairs of timed kernel launches
+  size_t ev_used = 0;
+  void drop_graph();
   ~pq_program_s();
 };
 
