@@ -177,11 +177,14 @@ int pq_program_info(pq_program p, uint64_t *gates, uint64_t *measurements,
 /* Launch shape of the persistent kernel (0 keeps the current value). */
 int pq_program_set_launch(pq_program p, uint32_t threads, uint32_t blocks);
 /* Work census of persistent/graph runs (enable zeroes the counters):
- * out = {random measurements, determined measurements, rows rowsummed by
- * random measurements, rowsum steps of determined measurements}, each summed
- * over 64-shot word groups.  Used for algorithmic-byte accounting. */
+ * out[0..3] = {random measurements, determined measurements, rows rowsummed
+ * by random measurements, rowsum steps of determined measurements}, each
+ * summed over 64-shot word groups (the algorithmic-byte accounting);
+ * out[4..7] = wall-clock ticks (100 MHz) spent per workgroup, summed over
+ * workgroups, in {gate segments, pivot checks, determined batches, other
+ * measurements}. */
 int pq_program_set_census(pq_program p, int enable);
-int pq_program_read_census(pq_state s, pq_program p, uint64_t out[4]);
+int pq_program_read_census(pq_state s, pq_program p, uint64_t out[8]);
 /* HIP-event timing of the persistent kernel launches of PQ_RUN_PERSISTENT
  * runs on the state's stream; pq_program_kernel_ms sums and clears them. */
 int pq_program_set_timing(pq_program p, int enable);

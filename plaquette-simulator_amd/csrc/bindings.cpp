@@ -236,13 +236,17 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
            [](Program &p, bool on) { check(pq_program_set_census(p.p, on ? 1 : 0), "set_census"); })
       .def("read_census",
            [](Program &p) {
-             uint64_t c[4];
+             uint64_t c[8];
              check(pq_program_read_census(p.s, p.p, c), "read_census");
              py::dict d;
              d["random"] = c[0];
              d["determined"] = c[1];
              d["random_rows"] = c[2];
              d["determined_steps"] = c[3];
+             d["ticks_gates"] = c[4];
+             d["ticks_pivot_check"] = c[5];
+             d["ticks_determined"] = c[6];
+             d["ticks_general"] = c[7];
              return d;
            })
       .def("set_timing",

@@ -317,15 +317,27 @@ __device__ void measure_word(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
       if (!mi) continue;
       ++my_rows;
       uint64_t c0 = 0, c1 = 0;
-      for (uint32_t k = 0; k < nl; ++k) {
-        const uint32_t j = s_list[k];
-        uint64_t *px = T + (uint64_t)j * Ru + i;
-        uint64_t *pz = T + (uint64_t)(n + j) * Ru + i;
-        const uint64_t x1 = *px, z1 = *pz;
-        const uint64_t x2 = s_src[j], z2 = s_src[n + j];
-        phase_acc(x1, z1, x2, z2, c0, c1);
-        *px = x1 ^ (mi & x2);
-        *pz = z1 ^ (mi & z2);
+      // eight source columns at a time: all loads issued before any store
+      for (uint32_t k0 = 0; k0 < nl; k0 += 8) {
+        uint64_t xs[8], zs[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) {
+          if (k0 + u < nl) {
+            const uint32_t j = s_list[k0 + u];
+            xs[u] = T[(uint64_t)j * Ru + i];
+            zs[u] = T[(uint64_t)(n + j) * Ru + i];
+          }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) {
+          if (k0 + u < nl) {
+            const uint32_t j = s_list[k0 + u];
+            const uint64_t x2 = s_src[j], z2 = s_src[n + j];
+            phase_acc(xs[u], zs[u], x2, z2, c0, c1);
+            T[(uint64_t)j * Ru + i] = xs[u] ^ (mi & x2);
+            T[(uint64_t)(n + j) * Ru + i] = zs[u] ^ (mi & z2);
+          }
+        }
       }
       T[(uint64_t)(2 * n) * Ru + i] ^= mi & (rsrc ^ c1);
     }
@@ -539,8 +551,40 @@ struct ProgParams {
 };
 
 constexpr int kBatch = 8;
+
+// The compiled program and the noise planes are read-only while k_program
+// runs; reading them through the constant address space lets uniform loads
+// become scalar (SMEM) loads, so a batch's column loads are not serialised
+// behind per-op vector loads of its instructions.
+#define PQ_CONST __attribute__((address_space(4)))
+typedef const PQ_CONST DInstr *cinstr_ptr;
+typedef const PQ_CONST uint64_t *cword_ptr;
+
+__device__ __forceinline__ DInstr ld_instr(cinstr_ptr p, uint32_t i) {
+  DInstr I;
+  I.code = p[i].code;
+  I.qa = p[i].qa;
+  I.qb = p[i].qb;
+  I.slot = p[i].slot;
+  I.na = p[i].na;
+  I.nb = p[i].nb;
+  I.t0 = p[i].t0;
+  I.pad = p[i].pad;
+  return I;
+}
+
+__device__ __forceinline__ DSegment ld_segment(const DSegment *g, uint32_t i) {
+  const PQ_CONST DSegment *p = (const PQ_CONST DSegment *)g;
+  DSegment sg;
+  sg.kind = p[i].kind;
+  sg.begin = p[i].begin;
+  sg.count = p[i].count;
+  sg.pad = p[i].pad;
+  return sg;
+}
 constexpr uint32_t kProgramMaxThreads = 512;  // block-size bound of k_program
-constexpr uint32_t kMeasBatch = 64;  // measurements pivot-checked per pass
+constexpr uint32_t kMeasBatch = 64;   // most measurements pivot-checked per pass
+constexpr uint32_t kMeasWindow0 = 8;  // first window of a run
 
 __device__ __forceinline__ void apply_fused_noise(const uint64_t *__restrict__ planes,
                                                   uint64_t W, uint64_t w, uint32_t k,
@@ -552,114 +596,133 @@ __device__ __forceinline__ void apply_fused_noise(const uint64_t *__restrict__ p
   }
 }
 
+// Gate segment, processed in chunks of up to kStage instructions.  Each
+// chunk's instructions and the per-word shot masks they need (fused noise
+// planes, flags) are staged in LDS by the whole workgroup in one round trip;
+// after that every batch costs exactly one global round trip (its column
+// loads), and the stores are fire-and-forget.
+constexpr uint32_t kStage = 512;
+constexpr uint32_t kStageWords = kStage * (sizeof(DInstr) / 8 + 4);
+
 __device__ void run_gate_segment(uint64_t *__restrict__ T, uint64_t w,
-                                 const pq::ProgramArgs &a, const DSegment &sg) {
+                                 const pq::ProgramArgs &a, const DSegment &sg,
+                                 DInstr *s_ins, uint64_t *s_msk) {
   const uint64_t Ru = a.R;
   const uint32_t n = a.n;
   const uint64_t W = a.W;
-  const DInstr *__restrict__ ins = a.instrs;
-  const uint64_t *__restrict__ planes = a.planes;
+  cinstr_ptr ins = (cinstr_ptr)a.instrs;
+  cword_ptr planes = (cword_ptr)a.planes;
   const uint64_t *__restrict__ recs = a.recs;
-  for (uint32_t row = threadIdx.x; row < a.R; row += blockDim.x) {
-    uint64_t *Tr = T + row;
-    uint64_t r = Tr[(uint64_t)(2 * n) * Ru];
-    uint32_t k = sg.begin;
-    const uint32_t end = sg.begin + sg.count;
-    while (k < end) {
-      const uint32_t len = ins[k].pad;
-      // load phase: every column word and shot mask the batch needs
-      uint64_t xa[kBatch], za[kBatch], xb[kBatch], zb[kBatch];
-      uint64_t fxa[kBatch], fza[kBatch], fxb[kBatch], fzb[kBatch];
-#pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
-        if (u < (int)len) {
-          const DInstr I = ins[k + u];
-          xa[u] = Tr[(uint64_t)I.qa * Ru];
-          za[u] = Tr[(uint64_t)(n + I.qa) * Ru];
-          xb[u] = zb[u] = 0;
-          if (I.code == PQ_OP_CX || I.code == PQ_OP_CZ) {
-            xb[u] = Tr[(uint64_t)I.qb * Ru];
-            zb[u] = Tr[(uint64_t)(n + I.qb) * Ru];
-          }
-          fxa[u] = fza[u] = fxb[u] = fzb[u] = 0;
-          if (I.na != kNone) {
-            fxa[u] = planes[(2ull * I.na) * W + w];
-            fza[u] = planes[(2ull * I.na + 1) * W + w];
-          }
-          if (I.nb != kNone) {
-            fxb[u] = planes[(2ull * I.nb) * W + w];
-            fzb[u] = planes[(2ull * I.nb + 1) * W + w];
-          } else if (I.code >= PQ_OP_H_FLAG && I.code != PQ_OP_NOISE) {
-            fxb[u] = recs[(2ull * I.slot) * W + w];  // flag of a flagged op
-          }
-        }
+  const uint32_t seg_end = sg.begin + sg.count;
+  for (uint32_t cb = sg.begin; cb < seg_end; cb += kStage) {
+    const uint32_t cn = min(kStage, seg_end - cb);
+    for (uint32_t t = threadIdx.x; t < cn; t += blockDim.x) {
+      const DInstr I = ld_instr(ins, cb + t);
+      uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+      if (I.na != kNone) {
+        m0 = planes[(2ull * I.na) * W + w];
+        m1 = planes[(2ull * I.na + 1) * W + w];
       }
-      // compute phase: registers only, stores are fire-and-forget
-#pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
-        if (u < (int)len) {
-          const DInstr I = ins[k + u];
-          uint64_t *pxa = Tr + (uint64_t)I.qa * Ru;
-          uint64_t *pza = Tr + (uint64_t)(n + I.qa) * Ru;
-          uint64_t x = xa[u], z = za[u];
-          const uint64_t f = fxb[u];
-          switch (I.code) {
-            case PQ_OP_H:
-              r ^= x & z;
-              *pxa = z; *pza = x;
-              { const uint64_t t = x; x = z; z = t; }
-              break;
-            case PQ_OP_S:
-              r ^= x & z;
-              z ^= x;
-              *pza = z;
-              break;
-            case PQ_OP_X: r ^= z; break;
-            case PQ_OP_Y: r ^= z ^ x; break;
-            case PQ_OP_Z: r ^= x; break;
-            case PQ_OP_CX: {
-              uint64_t xt = xb[u], zt = zb[u];
-              r ^= x & zt & ~(xt ^ z);
-              xt ^= x;
-              z ^= zt;
-              Tr[(uint64_t)I.qb * Ru] = xt;
-              *pza = z;
-              r ^= (fxb[u] & zt) ^ (fzb[u] & xt);
-              break;
-            }
-            case PQ_OP_CZ: {
-              uint64_t xt = xb[u], zt = zb[u];
-              r ^= x & xt & (zt ^ z);
-              zt ^= x;
-              z ^= xt;
-              Tr[(uint64_t)(n + I.qb) * Ru] = zt;
-              *pza = z;
-              r ^= (fxb[u] & zt) ^ (fzb[u] & xt);
-              break;
-            }
-            case PQ_OP_H_FLAG: {
-              const uint64_t d = f & (x ^ z);
-              r ^= f & x & z;
-              x ^= d; z ^= d;
-              *pxa = x; *pza = z;
-              break;
-            }
-            case PQ_OP_S_FLAG:
-              r ^= f & x & z;
-              z ^= f & x;
-              *pza = z;
-              break;
-            case PQ_OP_X_FLAG: r ^= f & z; break;
-            case PQ_OP_Y_FLAG: r ^= f & (z ^ x); break;
-            case PQ_OP_Z_FLAG: r ^= f & x; break;
-            default: break;  // PQ_OP_NOISE: only the fused plane below
-          }
-          r ^= (fxa[u] & z) ^ (fza[u] & x);
-        }
+      if (I.nb != kNone) {
+        m2 = planes[(2ull * I.nb) * W + w];
+        m3 = planes[(2ull * I.nb + 1) * W + w];
+      } else if (I.code >= PQ_OP_H_FLAG && I.code != PQ_OP_NOISE) {
+        m2 = recs[(2ull * I.slot) * W + w];  // flag of a flagged op
       }
-      k += len;
+      s_ins[t] = I;
+      s_msk[4 * t + 0] = m0;
+      s_msk[4 * t + 1] = m1;
+      s_msk[4 * t + 2] = m2;
+      s_msk[4 * t + 3] = m3;
     }
-    Tr[(uint64_t)(2 * n) * Ru] = r;
+    __syncthreads();
+    for (uint32_t row = threadIdx.x; row < a.R; row += blockDim.x) {
+      uint64_t *Tr = T + row;
+      uint64_t r = Tr[(uint64_t)(2 * n) * Ru];
+      uint32_t k = 0;
+      while (k < cn) {
+        const uint32_t len = s_ins[k].pad;
+        uint64_t xa[kBatch], za[kBatch], xb[kBatch], zb[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+          if (u < (int)len) {
+            const DInstr &I = s_ins[k + u];
+            xa[u] = Tr[(uint64_t)I.qa * Ru];
+            za[u] = Tr[(uint64_t)(n + I.qa) * Ru];
+            if (I.code == PQ_OP_CX || I.code == PQ_OP_CZ) {
+              xb[u] = Tr[(uint64_t)I.qb * Ru];
+              zb[u] = Tr[(uint64_t)(n + I.qb) * Ru];
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+          if (u < (int)len) {
+            const DInstr &I = s_ins[k + u];
+            const uint64_t *M = s_msk + 4 * (k + u);
+            uint64_t *pxa = Tr + (uint64_t)I.qa * Ru;
+            uint64_t *pza = Tr + (uint64_t)(n + I.qa) * Ru;
+            uint64_t x = xa[u], z = za[u];
+            const uint64_t f = M[2];
+            switch (I.code) {
+              case PQ_OP_H:
+                r ^= x & z;
+                *pxa = z; *pza = x;
+                { const uint64_t t = x; x = z; z = t; }
+                break;
+              case PQ_OP_S:
+                r ^= x & z;
+                z ^= x;
+                *pza = z;
+                break;
+              case PQ_OP_X: r ^= z; break;
+              case PQ_OP_Y: r ^= z ^ x; break;
+              case PQ_OP_Z: r ^= x; break;
+              case PQ_OP_CX: {
+                uint64_t xt = xb[u], zt = zb[u];
+                r ^= x & zt & ~(xt ^ z);
+                xt ^= x;
+                z ^= zt;
+                Tr[(uint64_t)I.qb * Ru] = xt;
+                *pza = z;
+                r ^= (M[2] & zt) ^ (M[3] & xt);
+                break;
+              }
+              case PQ_OP_CZ: {
+                uint64_t xt = xb[u], zt = zb[u];
+                r ^= x & xt & (zt ^ z);
+                zt ^= x;
+                z ^= xt;
+                Tr[(uint64_t)(n + I.qb) * Ru] = zt;
+                *pza = z;
+                r ^= (M[2] & zt) ^ (M[3] & xt);
+                break;
+              }
+              case PQ_OP_H_FLAG: {
+                const uint64_t d = f & (x ^ z);
+                r ^= f & x & z;
+                x ^= d; z ^= d;
+                *pxa = x; *pza = z;
+                break;
+              }
+              case PQ_OP_S_FLAG:
+                r ^= f & x & z;
+                z ^= f & x;
+                *pza = z;
+                break;
+              case PQ_OP_X_FLAG: r ^= f & z; break;
+              case PQ_OP_Y_FLAG: r ^= f & (z ^ x); break;
+              case PQ_OP_Z_FLAG: r ^= f & x; break;
+              default: break;  // PQ_OP_NOISE: only the fused plane below
+            }
+            r ^= (M[0] & z) ^ (M[1] & x);
+          }
+        }
+        k += len;
+      }
+      Tr[(uint64_t)(2 * n) * Ru] = r;
+    }
+    __syncthreads();
   }
 }
 
@@ -670,11 +733,17 @@ __device__ __forceinline__ uint64_t wave_or(uint64_t v) {
 }
 
 // MeasureDetermined (MeasurementFunctors.hpp:190-201) of a measurement that
-// is determined in all 64 shots of the group, run by one wave: the scratch
-// row lives in the wave's private LDS slab (lane-owned columns), chain
-// members come from a ballot over destabilizer column q in ascending order,
-// and each step's mod-4 phase is reduced across the wave.  Only the last
-// measurement of a batch writes its scratch row back to the tableau.
+// is determined in all 64 shots of the group, run by one wave.  Chain
+// members (destabilizers i with x_q set, ascending) are compacted with
+// ballots; each step folds stabilizer n+i into the scratch row and reduces
+// that step's mod-4 phase across the wave.  Only the last measurement of a
+// batch writes its scratch row back to the tableau.
+//
+// n <= 64*kDetRegs: each lane owns at most kDetRegs columns, the scratch row
+// lives in registers and the next step's source words are loaded while the
+// current step is folded in.  Larger n: scratch row in the wave's LDS slab.
+constexpr int kDetRegs = 8;
+
 __device__ void determined_wave(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
                                 uint32_t R, uint32_t q, uint64_t *slab,
                                 uint64_t *rec, uint64_t W, bool write_scratch,
@@ -683,29 +752,75 @@ __device__ void determined_wave(uint64_t *__restrict__ T, uint64_t w, uint32_t n
   const uint64_t Ru = R;
   const uint64_t *xq = T + (uint64_t)q * Ru;
   const uint64_t scratch = 2ull * n;
-  for (uint32_t j = lane; j < n; j += 64) {
-    slab[j] = 0;
-    slab[n + j] = 0;
-  }
+  const uint64_t *rq = T + scratch * Ru;
   uint64_t racc = 0;
   uint32_t steps = 0;
-  for (uint32_t base = 0; base < n; base += 64) {
-    const uint32_t i = base + lane;
-    const uint64_t xi = i < n ? xq[i] : 0ull;
-    uint64_t bal = __ballot(xi != 0);
-    while (bal) {
-      const uint32_t t = (uint32_t)__ffsll((long long)bal) - 1;
-      bal &= bal - 1;
-      const uint64_t m = __shfl(xi, t, 64);
-      const uint64_t s = n + base + t;
+  if (n <= 64u * kDetRegs) {
+    // chain list into the wave's slab: rows in slab[0..], masks in slab[n..]
+    uint32_t count = 0;
+    for (uint32_t base = 0; base < n; base += 4 * 64) {
+      uint64_t xi[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t i = base + 64 * u + lane;
+        xi[u] = i < n ? xq[i] : 0ull;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint64_t bal = __ballot(xi[u] != 0);
+        const uint32_t pos = count + (uint32_t)__popcll(bal & ((1ull << lane) - 1));
+        if (xi[u]) {
+          slab[pos] = base + 64 * u + lane;
+          slab[n + pos] = xi[u];
+        }
+        count += (uint32_t)__popcll(bal);
+      }
+    }
+    steps = count;
+    uint64_t sx[kDetRegs], sz[kDetRegs], nx[kDetRegs], nz[kDetRegs];
+#pragma unroll
+    for (int u = 0; u < kDetRegs; ++u) sx[u] = sz[u] = nx[u] = nz[u] = 0;
+    uint64_t nm = 0, nr = 0;
+    if (count) {
+      const uint64_t s = n + slab[0];
+      nm = slab[n];
+#pragma unroll
+      for (int u = 0; u < kDetRegs; ++u) {
+        const uint32_t j = lane + 64 * u;
+        if (j < n) {
+          nx[u] = T[(uint64_t)j * Ru + s];
+          nz[u] = T[(uint64_t)(n + j) * Ru + s];
+        }
+      }
+      nr = rq[s];
+    }
+    for (uint32_t t = 0; t < count; ++t) {
+      uint64_t cx[kDetRegs], cz[kDetRegs];
+#pragma unroll
+      for (int u = 0; u < kDetRegs; ++u) {
+        cx[u] = nx[u];
+        cz[u] = nz[u];
+      }
+      const uint64_t m = nm, rs = nr;
+      if (t + 1 < count) {
+        const uint64_t s = n + slab[t + 1];
+        nm = slab[n + t + 1];
+#pragma unroll
+        for (int u = 0; u < kDetRegs; ++u) {
+          const uint32_t j = lane + 64 * u;
+          if (j < n) {
+            nx[u] = T[(uint64_t)j * Ru + s];
+            nz[u] = T[(uint64_t)(n + j) * Ru + s];
+          }
+        }
+        nr = rq[s];
+      }
       uint64_t c0 = 0, c1 = 0;
-      for (uint32_t j = lane; j < n; j += 64) {
-        const uint64_t x2 = T[(uint64_t)j * Ru + s];
-        const uint64_t z2 = T[(uint64_t)(n + j) * Ru + s];
-        const uint64_t x1 = slab[j], z1 = slab[n + j];
-        phase_acc(x1, z1, x2, z2, c0, c1);
-        slab[j] = x1 ^ (m & x2);
-        slab[n + j] = z1 ^ (m & z2);
+#pragma unroll
+      for (int u = 0; u < kDetRegs; ++u) {
+        phase_acc(sx[u], sz[u], cx[u], cz[u], c0, c1);
+        sx[u] ^= m & cx[u];
+        sz[u] ^= m & cz[u];
       }
 #pragma unroll
       for (uint32_t off = 32; off > 0; off >>= 1) {
@@ -713,8 +828,56 @@ __device__ void determined_wave(uint64_t *__restrict__ T, uint64_t w, uint32_t n
         const uint64_t t1 = __shfl_xor(c1, off, 64);
         add_mod4(c0, c1, t0, t1);
       }
-      racc ^= m & (T[scratch * Ru + s] ^ c1);
-      ++steps;
+      racc ^= m & (rs ^ c1);
+    }
+    if (write_scratch) {
+#pragma unroll
+      for (int u = 0; u < kDetRegs; ++u) {
+        const uint32_t j = lane + 64 * u;
+        if (j < n) {
+          T[(uint64_t)j * Ru + scratch] = sx[u];
+          T[(uint64_t)(n + j) * Ru + scratch] = sz[u];
+        }
+      }
+    }
+  } else {
+    for (uint32_t j = lane; j < n; j += 64) {
+      slab[j] = 0;
+      slab[n + j] = 0;
+    }
+    for (uint32_t base = 0; base < n; base += 64) {
+      const uint32_t i = base + lane;
+      const uint64_t xi = i < n ? xq[i] : 0ull;
+      uint64_t bal = __ballot(xi != 0);
+      while (bal) {
+        const uint32_t t = (uint32_t)__ffsll((long long)bal) - 1;
+        bal &= bal - 1;
+        const uint64_t m = __shfl(xi, t, 64);
+        const uint64_t s = n + base + t;
+        uint64_t c0 = 0, c1 = 0;
+        for (uint32_t j = lane; j < n; j += 64) {
+          const uint64_t x2 = T[(uint64_t)j * Ru + s];
+          const uint64_t z2 = T[(uint64_t)(n + j) * Ru + s];
+          const uint64_t x1 = slab[j], z1 = slab[n + j];
+          phase_acc(x1, z1, x2, z2, c0, c1);
+          slab[j] = x1 ^ (m & x2);
+          slab[n + j] = z1 ^ (m & z2);
+        }
+#pragma unroll
+        for (uint32_t off = 32; off > 0; off >>= 1) {
+          const uint64_t t0 = __shfl_xor(c0, off, 64);
+          const uint64_t t1 = __shfl_xor(c1, off, 64);
+          add_mod4(c0, c1, t0, t1);
+        }
+        racc ^= m & (rq[s] ^ c1);
+        ++steps;
+      }
+    }
+    if (write_scratch) {
+      for (uint32_t j = lane; j < n; j += 64) {
+        T[(uint64_t)j * Ru + scratch] = slab[j];
+        T[(uint64_t)(n + j) * Ru + scratch] = slab[n + j];
+      }
     }
   }
   if (lane == 0) {
@@ -724,13 +887,7 @@ __device__ void determined_wave(uint64_t *__restrict__ T, uint64_t w, uint32_t n
       atomicAdd(census + 1, 1ull);
       atomicAdd(census + 3, (unsigned long long)steps);
     }
-  }
-  if (write_scratch) {
-    for (uint32_t j = lane; j < n; j += 64) {
-      T[(uint64_t)j * Ru + scratch] = slab[j];
-      T[(uint64_t)(n + j) * Ru + scratch] = slab[n + j];
-    }
-    if (lane == 0) T[scratch * Ru + scratch] = racc;
+    if (write_scratch) T[scratch * Ru + scratch] = racc;
   }
 }
 
@@ -754,25 +911,42 @@ __global__ __launch_bounds__(kProgramMaxThreads) void k_program(pq::ProgramArgs 
   const uint64_t seed = pp->seed, mseq0 = pp->mseq0;
   const uint32_t lane = lane_id(), wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const uint32_t ndw = a.det_waves;
+  // census mode also times the phases (thread 0, wall-clock ticks)
+  const bool timing = a.census != nullptr && threadIdx.x == 0;
+  uint64_t t_gate = 0, t_check = 0, t_det = 0, t_gen = 0, t0 = 0;
   for (uint64_t wi = blockIdx.x; wi < a.nwords; wi += gridDim.x) {
     const uint64_t w = a.word0 + wi;
     uint64_t *T = a.tab + w * C * a.R;
     for (uint32_t si = 0; si < a.nseg; ++si) {
-      const DSegment sg = a.segs[si];
+      const DSegment sg = ld_segment(a.segs, si);
       if (sg.kind == 0) {
-        run_gate_segment(T, w, a, sg);
-        __syncthreads();
+        if (timing) t0 = wall_clock64();
+        run_gate_segment(T, w, a, sg, reinterpret_cast<DInstr *>(dyn),
+                         dyn + kStage * (sizeof(DInstr) / 8));
+        if (timing) t_gate += wall_clock64() - t0;
         continue;
       }
       uint32_t k = sg.begin;
       const uint32_t end = sg.begin + sg.count;
+      uint32_t win = kMeasWindow0;
       while (k < end) {
+        if (timing) t0 = wall_clock64();
         // which of the next measurements have a pivot in some shot?
-        const uint32_t nm = min(end - k, kMeasBatch);
+        // the window grows while every checked measurement is determined and
+        // restarts small after a random one (the state then changes)
+        const uint32_t nm = min(end - k, win);
         for (uint32_t idx = wv; idx < nm; idx += nw) {
-          const uint64_t *xq = T + (uint64_t)a.instrs[k + idx].qa * Ru;
+          const uint64_t *xq = T + (uint64_t)((cinstr_ptr)a.instrs)[k + idx].qa * Ru;
           uint64_t acc = 0;
-          for (uint32_t p = lane; p < n; p += 64) acc |= xq[n + p];
+          for (uint32_t p0 = 0; p0 < n; p0 += 4 * 64) {
+            uint64_t v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const uint32_t p = p0 + 64 * u + lane;
+              v[u] = p < n ? xq[n + p] : 0ull;
+            }
+            acc |= v[0] | v[1] | v[2] | v[3];
+          }
           acc = wave_or(acc);
           if (lane == 0) s_rm[idx] = acc;
         }
@@ -783,12 +957,18 @@ __global__ __launch_bounds__(kProgramMaxThreads) void k_program(pq::ProgramArgs 
             jr = idx;
             break;
           }
+        win = (jr == nm) ? min(2 * win, kMeasBatch) : kMeasWindow0;
+        if (timing) {
+          const uint64_t t1 = wall_clock64();
+          t_check += t1 - t0;
+          t0 = t1;
+        }
         // measurements [k, k+jr) are determined in every shot: one wave each
         if (jr > 0) {
           if (wv < ndw) {
             uint64_t *slab = s_slab + (uint64_t)wv * 2 * n;
             for (uint32_t idx = wv; idx < jr; idx += ndw) {
-              const DInstr I = a.instrs[k + idx];
+              const DInstr I = ld_instr((cinstr_ptr)a.instrs, k + idx);
               uint64_t *rec = a.recs + (2ull * I.slot) * a.W;
               determined_wave(T, w, n, a.R, I.qa, slab, rec, a.W, idx == jr - 1,
                               a.census);
@@ -796,8 +976,13 @@ __global__ __launch_bounds__(kProgramMaxThreads) void k_program(pq::ProgramArgs 
           }
           __syncthreads();
         }
+        if (timing) {
+          const uint64_t t1 = wall_clock64();
+          t_det += t1 - t0;
+          t0 = t1;
+        }
         if (jr < nm) {
-          const DInstr I = a.instrs[k + jr];
+          const DInstr I = ld_instr((cinstr_ptr)a.instrs, k + jr);
           const uint64_t m = (uint64_t)I.na | ((uint64_t)I.nb << 32);
           const uint64_t key = stream_key(seed, 2 * (mseq0 + m));
           const uint64_t *cp = a.coin_planes ? a.coin_planes + m * a.W : nullptr;
@@ -805,11 +990,18 @@ __global__ __launch_bounds__(kProgramMaxThreads) void k_program(pq::ProgramArgs 
           measure_word(T, w, n, a.R, I.qa, I.t0, key, cp, rec, rec + a.W, s_src,
                        s_list, sh, false, a.census);
           k += jr + 1;
+          if (timing) t_gen += wall_clock64() - t0;
         } else {
           k += nm;
         }
       }
     }
+  }
+  if (timing) {
+    atomicAdd(a.census + 4, (unsigned long long)t_gate);
+    atomicAdd(a.census + 5, (unsigned long long)t_check);
+    atomicAdd(a.census + 6, (unsigned long long)t_det);
+    atomicAdd(a.census + 7, (unsigned long long)t_gen);
   }
 }
 
@@ -980,14 +1172,15 @@ hipError_t launch_record_to_int(hipStream_t s, const uint64_t *rec, uint64_t B,
 hipError_t launch_program(hipStream_t s, const ProgramArgs &a, uint32_t threads,
                           uint32_t blocks, const void *params) {
   if (!a.nwords) return hipSuccess;
-  const size_t lds = 8 * program_lds_words(a.n, a.det_waves);
+  const size_t lds = program_lds_bytes(a.n, a.det_waves);
   hipLaunchKernelGGL(k_program, dim3(blocks), dim3(threads), lds, s, a,
                      (const ProgParams *)params);
   return hipGetLastError();
 }
 
 size_t program_lds_bytes(uint32_t n, uint32_t det_waves) {
-  return 8 * program_lds_words(n, det_waves);
+  // the gate-segment staging area and the measurement area share the LDS
+  return 8 * std::max<size_t>(program_lds_words(n, det_waves), kStageWords);
 }
 
 hipError_t launch_program_noise(hipStream_t s, uint64_t *planes,

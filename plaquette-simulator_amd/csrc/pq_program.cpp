@@ -125,13 +125,17 @@ int compile_program(pq_program_s *p) {
       while (k < order.size()) {
         uint32_t e = k;
         while (e < order.size() && layer[order[e]] == layer[order[k]]) ++e;
-        for (uint32_t b = k; b < e; b += kProgramBatch) {
-          const uint32_t len = std::min<uint32_t>(kProgramBatch, e - b);
+        for (uint32_t b = k; b < e;) {
+          // batches never straddle a staging chunk of the kernel
+          const uint32_t pos = (uint32_t)(p->instrs.size() - sg.begin) % kProgramStage;
+          const uint32_t len =
+              std::min<uint32_t>({kProgramBatch, e - b, kProgramStage - pos});
           for (uint32_t u = 0; u < len; ++u) {
             DInstr I = seg[order[b + u]];
             I.pad = (u == 0) ? len : 0;
             p->instrs.push_back(I);
           }
+          b += len;
         }
         k = e;
       }
@@ -195,7 +199,8 @@ extern "C" int pq_program_create(pq_state s, const pq_op *ops, uint64_t nops,
       return pq_set_error(PQ_ERR_OOM, "noise plane allocation failed");
     }
   }
-  uint64_t th = ((s->R + 63) / 64) * 64;
+  // the measurement phases use every wave, so the default is the largest block
+  const uint64_t th = kProgramThreadsMax;
   p->threads = (uint32_t)std::min<uint64_t>(kProgramThreadsMax, std::max<uint64_t>(64, th));
   p->blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(s->W, 1u << 20));
   HIPCHK(prepare_kernels((uint32_t)s->n));
@@ -371,21 +376,21 @@ extern "C" int pq_program_run(pq_state s, pq_program p, int form) {
 
 extern "C" int pq_program_set_census(pq_program p, int enable) {
   if (!p) return pq_set_error(PQ_ERR_ARG, "null program");
-  if (enable && !p->d_census) HIPCHK(hipMalloc(&p->d_census, 4 * sizeof(unsigned long long)));
-  if (enable) HIPCHK(hipMemset(p->d_census, 0, 4 * sizeof(unsigned long long)));
+  if (enable && !p->d_census) HIPCHK(hipMalloc(&p->d_census, 8 * sizeof(unsigned long long)));
+  if (enable) HIPCHK(hipMemset(p->d_census, 0, 8 * sizeof(unsigned long long)));
   if ((bool)enable != p->census_on) p->drop_graph();
   p->census_on = enable != 0;
   return PQ_OK;
 }
 
-extern "C" int pq_program_read_census(pq_state s, pq_program p, uint64_t out[4]) {
+extern "C" int pq_program_read_census(pq_state s, pq_program p, uint64_t out[8]) {
   if (!s || !p || !out) return pq_set_error(PQ_ERR_ARG, "null argument");
-  for (int i = 0; i < 4; ++i) out[i] = 0;
+  for (int i = 0; i < 8; ++i) out[i] = 0;
   if (!p->d_census) return PQ_OK;
   HIPCHK(hipStreamSynchronize(s->stream));
-  unsigned long long h[4];
+  unsigned long long h[8];
   HIPCHK(hipMemcpy(h, p->d_census, sizeof(h), hipMemcpyDeviceToHost));
-  for (int i = 0; i < 4; ++i) out[i] = h[i];
+  for (int i = 0; i < 8; ++i) out[i] = h[i];
   return PQ_OK;
 }
 

@@ -11,6 +11,7 @@
 
 constexpr uint32_t kProgramBatch = 8;  // must equal kBatch in pq_kernels.hip
 constexpr uint32_t kProgramThreadsMax = 512;  // must equal kProgramMaxThreads there
+constexpr uint32_t kProgramStage = 512;       // must equal kStage there
 
 struct pq_program_s {
   uint64_t n = 0, W = 0, R = 0;
