@@ -194,6 +194,37 @@ int pq_run_program(pq_state s, const pq_op *ops, uint64_t nops, int form);
 /* Wait for all work queued on the state. */
 int pq_synchronize(pq_state s);
 
+/* ---- mixed batches ----
+ * No reference counterpart (the reference runs one state at a time).  A batch
+ * holds (state, program) members of any sizes and runs all of them in one
+ * persistent kernel over a packed work list of (member, 64-shot word group)
+ * items, heaviest first, pulled from a global counter; PQ_RUN_GRAPH captures
+ * the whole launch once and replays it.  Each run takes and advances every
+ * member's coin/noise sequence numbers as a standalone pq_program_run would,
+ * so every member's results are bit-identical alone and in the batch.
+ * Members must not be used until pq_batch_wait returns. */
+typedef struct pq_batch_s *pq_batch;
+int pq_batch_create(pq_batch *out);
+int pq_batch_destroy(pq_batch b);
+int pq_batch_add(pq_batch b, pq_state s, pq_program p);
+int pq_batch_run(pq_batch b, int form);
+int pq_batch_wait(pq_batch b);
+/* Launch shape: threads per workgroup (multiple of 64, <= 512), workgroups
+ * (0 = CUs x workgroups per CU) and the dynamic-LDS budget per workgroup
+ * (40 KB..150 KB; it sets workgroups per CU and parallel determined waves). */
+int pq_batch_set_launch(pq_batch b, uint32_t threads, uint32_t blocks,
+                        uint64_t lds_budget);
+/* Times candidate launch shapes on the batch and keeps the fastest (cached
+ * per batch shape).  Members are reset to |0...0> afterwards with their
+ * sequence numbers restored. */
+int pq_batch_autotune(pq_batch b, int reps, uint32_t *threads, uint64_t *lds_budget,
+                      float *best_ms);
+int pq_batch_info(pq_batch b, uint64_t *members, uint64_t *items, uint32_t *threads,
+                  uint32_t *blocks, uint64_t *lds_budget);
+/* HIP-event time of the last run (enable first). */
+int pq_batch_set_timing(pq_batch b, int enable);
+int pq_batch_last_ms(pq_batch b, float *ms);
+
 /* ---- diagnostics: the reference's test-only functors ---- */
 /* RowMultiply(i <- i*k) on every shot (BatchRowMultiplyFunctor,
  * MeasurementFunctors.hpp:168-187). */

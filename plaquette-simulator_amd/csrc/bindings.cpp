@@ -58,6 +58,18 @@ struct Program {
   }
 };
 
+struct Batch {
+  pq_batch b = nullptr;
+  std::vector<py::object> keep;  // member states and programs stay alive
+  Batch() { check(pq_batch_create(&b), "Batch"); }
+  ~Batch() {
+    if (b) {
+      pq_batch_wait(b);
+      pq_batch_destroy(b);
+    }
+  }
+};
+
 }  // namespace
 
 PYBIND11_MODULE(plaquette_simulator_bindings, m) {
@@ -256,6 +268,66 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
         uint64_t k = 0;
         check(pq_program_kernel_ms(p.p, &ms, &k), "kernel_ms");
         return py::make_tuple(ms, k);
+      });
+
+  py::class_<Batch>(m, "Batch")
+      .def(py::init<>())
+      .def(
+          "add",
+          [](Batch &b, State &st, Program &p) {
+            if (st.Handle() != p.s) throw std::invalid_argument("program belongs to another state");
+            check(pq_batch_add(b.b, st.Handle(), p.p), "Batch.add");
+            b.keep.push_back(py::cast(&st));
+            b.keep.push_back(py::cast(&p));
+          },
+          py::arg("state"), py::arg("program"))
+      .def(
+          "run",
+          [](Batch &b, const py::object &form, bool wait) {
+            check(pq_batch_run(b.b, form_of(form)), "Batch.run");
+            if (wait) check(pq_batch_wait(b.b), "Batch.wait");
+          },
+          py::arg("form") = "graph", py::arg("wait") = true)
+      .def("wait", [](Batch &b) { check(pq_batch_wait(b.b), "Batch.wait"); })
+      .def(
+          "set_launch",
+          [](Batch &b, uint32_t threads, uint32_t blocks, uint64_t lds) {
+            check(pq_batch_set_launch(b.b, threads, blocks, lds), "Batch.set_launch");
+          },
+          py::arg("threads") = 0, py::arg("blocks") = 0, py::arg("lds_budget") = 0)
+      .def(
+          "autotune",
+          [](Batch &b, int reps) {
+            uint32_t t = 0;
+            uint64_t l = 0;
+            float ms = 0;
+            check(pq_batch_autotune(b.b, reps, &t, &l, &ms), "Batch.autotune");
+            py::dict d;
+            d["threads"] = t;
+            d["lds_budget"] = l;
+            d["best_ms"] = ms;
+            return d;
+          },
+          py::arg("reps") = 2)
+      .def("info",
+           [](Batch &b) {
+             uint64_t mem = 0, items = 0, lds = 0;
+             uint32_t t = 0, bl = 0;
+             check(pq_batch_info(b.b, &mem, &items, &t, &bl, &lds), "Batch.info");
+             py::dict d;
+             d["members"] = mem;
+             d["items"] = items;
+             d["threads"] = t;
+             d["blocks"] = bl;
+             d["lds_budget"] = lds;
+             return d;
+           })
+      .def("set_timing",
+           [](Batch &b, bool on) { check(pq_batch_set_timing(b.b, on ? 1 : 0), "Batch.set_timing"); })
+      .def("last_ms", [](Batch &b) {
+        float ms = 0;
+        check(pq_batch_last_ms(b.b, &ms), "Batch.last_ms");
+        return ms;
       });
 
   m.def("coin_threshold", &pq_coin_threshold);

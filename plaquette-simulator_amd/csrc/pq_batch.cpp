@@ -1,0 +1,387 @@
+// Mixed batches: many states (different qubit and shot counts), each with its
+// own compiled program, run by one persistent kernel over a packed work list
+// of (state, 64-shot word group) items, optionally as a captured hipGraph.
+//
+// Per run: every member's record slots are prepared and its coin/noise
+// sequence numbers are taken and advanced exactly as a standalone run would,
+// so each member's results are identical alone and in the batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "plaquette_amd.h"
+#include "pq_capi_internal.h"
+#include "pq_common.h"
+#include "pq_kernels.h"
+#include "pq_program_internal.h"
+
+using namespace pq;
+
+#define HIPCHK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return pq_set_error(PQ_ERR_HIP,                                             \
+                          (std::string(#expr) + ": " + hipGetErrorString(e_)).c_str()); \
+  } while (0)
+
+struct pq_batch_s {
+  std::vector<std::pair<pq_state, pq_program>> members;
+  hipStream_t stream = nullptr;
+  BatchCircuit *d_circ = nullptr;
+  BatchCircuit *h_circ = nullptr;  // pinned
+  uint64_t *d_items = nullptr;
+  unsigned long long *d_counter = nullptr;
+  uint64_t nitems = 0;
+  bool built = false;
+  uint32_t threads = kProgramThreadsMax;
+  uint32_t blocks = 0;             // 0: CUs x blocks per CU
+  uint64_t lds_budget = kProgramLdsBytes;
+  uint32_t noise_blocks = 64;
+  hipEvent_t ev_upload = nullptr, ev_a = nullptr, ev_b = nullptr;
+  bool timed = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint64_t> graph_gens;  // arena generations at capture
+  ~pq_batch_s();
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
+};
+
+pq_batch_s::~pq_batch_s() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  drop_graph();
+  if (d_circ) (void)hipFree(d_circ);
+  if (h_circ) (void)hipHostFree(h_circ);
+  if (d_items) (void)hipFree(d_items);
+  if (d_counter) (void)hipFree(d_counter);
+  for (hipEvent_t e : {ev_upload, ev_a, ev_b})
+    if (e) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+static int cu_count() {
+  static int cus = 0;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      cus = prop.multiProcessorCount;
+    if (cus <= 0) cus = 1;
+  });
+  return cus;
+}
+
+extern "C" int pq_batch_create(pq_batch *out) {
+  if (!out) return pq_set_error(PQ_ERR_ARG, "null argument");
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess || c < 1)
+    return pq_set_error(PQ_ERR_NODEV, "no HIP device available");
+  auto b = std::make_unique<pq_batch_s>();
+  HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  HIPCHK(hipMalloc(&b->d_counter, sizeof(unsigned long long)));
+  HIPCHK(hipEventCreate(&b->ev_upload));
+  HIPCHK(hipEventCreate(&b->ev_a));
+  HIPCHK(hipEventCreate(&b->ev_b));
+  HIPCHK(prepare_batch_kernel());
+  *out = b.release();
+  return PQ_OK;
+}
+
+extern "C" int pq_batch_destroy(pq_batch b) {
+  delete b;
+  return PQ_OK;
+}
+
+extern "C" int pq_batch_add(pq_batch b, pq_state s, pq_program p) {
+  if (!b || !s || !p) return pq_set_error(PQ_ERR_ARG, "null argument");
+  if (p->n != s->n || p->W != s->W)
+    return pq_set_error(PQ_ERR_ARG, "program was compiled for a state of another shape");
+  if (s->n > measure_max_qubits())
+    return pq_set_error(PQ_ERR_UNSUPPORTED, "batched form supports at most 4095 qubits");
+  for (auto &m : b->members)
+    if (m.first == s) return pq_set_error(PQ_ERR_ARG, "state already in the batch");
+  if (b->members.size() >= (1u << 23)) return pq_set_error(PQ_ERR_UNSUPPORTED, "too many members");
+  b->members.emplace_back(s, p);
+  b->built = false;
+  b->drop_graph();
+  return PQ_OK;
+}
+
+extern "C" int pq_batch_set_launch(pq_batch b, uint32_t threads, uint32_t blocks,
+                                   uint64_t lds_budget) {
+  if (!b) return pq_set_error(PQ_ERR_ARG, "null batch");
+  if (threads) {
+    if (threads % 64 || threads > kProgramThreadsMax)
+      return pq_set_error(PQ_ERR_ARG, "threads must be a multiple of 64, at most 512");
+    b->threads = threads;
+  }
+  b->blocks = blocks;
+  if (lds_budget) {
+    if (lds_budget < 40 * 1024 || lds_budget > kProgramLdsBytes)
+      return pq_set_error(PQ_ERR_ARG, "LDS budget must be in [40 KB, 150 KB]");
+    b->lds_budget = lds_budget;
+  }
+  b->drop_graph();
+  return PQ_OK;
+}
+
+// Packed work list, heaviest circuits first (longest-processing-time order),
+// word groups of one circuit consecutive.
+static int build(pq_batch b) {
+  const uint32_t nc = (uint32_t)b->members.size();
+  std::vector<std::pair<double, uint32_t>> order;
+  uint64_t items = 0;
+  for (uint32_t c = 0; c < nc; ++c) {
+    pq_state s = b->members[c].first;
+    pq_program p = b->members[c].second;
+    const double cost = (double)s->R * ((double)p->instrs.size() + 32.0 * p->nmeas);
+    order.emplace_back(-cost, c);
+    items += s->W;
+  }
+  std::stable_sort(order.begin(), order.end());
+  std::vector<uint64_t> list;
+  list.reserve(items);
+  for (auto &o : order) {
+    const uint32_t c = o.second;
+    for (uint64_t w = 0; w < b->members[c].first->W; ++w)
+      list.push_back(((uint64_t)c << 40) | w);
+  }
+  if (b->d_items) HIPCHK(hipFree(b->d_items));
+  b->d_items = nullptr;
+  if (b->d_circ) HIPCHK(hipFree(b->d_circ));
+  b->d_circ = nullptr;
+  if (b->h_circ) HIPCHK(hipHostFree(b->h_circ));
+  b->h_circ = nullptr;
+  b->nitems = list.size();
+  if (!list.empty()) {
+    HIPCHK(hipMalloc(&b->d_items, list.size() * 8));
+    HIPCHK(hipMemcpy(b->d_items, list.data(), list.size() * 8, hipMemcpyHostToDevice));
+  }
+  if (nc) {
+    HIPCHK(hipMalloc(&b->d_circ, nc * sizeof(BatchCircuit)));
+    HIPCHK(hipHostMalloc(&b->h_circ, nc * sizeof(BatchCircuit), hipHostMallocDefault));
+  }
+  b->built = true;
+  return PQ_OK;
+}
+
+static uint32_t launch_blocks(pq_batch b) {
+  if (b->blocks) return (uint32_t)std::min<uint64_t>(b->blocks, std::max<uint64_t>(1, b->nitems));
+  const uint64_t per_cu = std::max<uint64_t>(1, (160ull * 1024) / (b->lds_budget + 2048));
+  return (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>((uint64_t)cu_count() * per_cu, b->nitems));
+}
+
+static size_t launch_lds(pq_batch b) {
+  size_t lds = 0;
+  for (uint32_t c = 0; c < b->members.size(); ++c)
+    lds = std::max<size_t>(lds, program_lds_bytes((uint32_t)b->members[c].first->n,
+                                                  b->h_circ[c].a.det_waves));
+  return lds;
+}
+
+static int enqueue(pq_batch b) {
+  const uint32_t nc = (uint32_t)b->members.size();
+  uint32_t noise_blocks = 0;
+  for (uint32_t c = 0; c < nc; ++c)
+    if (b->h_circ[c].nnoise) noise_blocks = b->noise_blocks;
+  return launch_batch(b->stream, b->d_circ, nc, b->d_items, b->nitems, b->d_counter,
+                      b->threads, launch_blocks(b), launch_lds(b), noise_blocks) == hipSuccess
+             ? PQ_OK
+             : pq_set_error(PQ_ERR_HIP, "batch launch failed");
+}
+
+extern "C" int pq_batch_run(pq_batch b, int form) {
+  if (!b) return pq_set_error(PQ_ERR_ARG, "null batch");
+  if (form != PQ_RUN_PERSISTENT && form != PQ_RUN_GRAPH)
+    return pq_set_error(PQ_ERR_ARG, "a batch runs in the persistent or graph form");
+  if (b->members.empty()) return PQ_OK;
+  if (!b->built) {
+    int rc = build(b);
+    if (rc) return rc;
+  }
+  // the pinned descriptor buffer is reused: the previous upload must be done
+  HIPCHK(hipEventSynchronize(b->ev_upload));
+  const uint32_t nc = (uint32_t)b->members.size();
+  std::vector<uint64_t> gens(nc);
+  for (uint32_t c = 0; c < nc; ++c) {
+    pq_state s = b->members[c].first;
+    pq_program p = b->members[c].second;
+    int rc = prepare_records(s, p);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));  // the member's own queued work first
+    BatchCircuit &bc = b->h_circ[c];
+    bc.a = program_args(s, p, b->lds_budget);
+    bc.a.census = nullptr;
+    bc.seed = (uint64_t)s->seed;
+    bc.mseq0 = s->meas_seq;
+    bc.nseq0 = s->noise_seq;
+    bc.noise = p->d_noise;
+    bc.nnoise = (uint32_t)p->nnoise;
+    bc.pad = 0;
+    s->meas_seq += p->nmeas;
+    s->noise_seq += p->nnoise;
+    gens[c] = s->arena_gen;
+  }
+  HIPCHK(hipMemcpyAsync(b->d_circ, b->h_circ, nc * sizeof(BatchCircuit),
+                        hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipEventRecord(b->ev_upload, b->stream));
+  if (b->timed) HIPCHK(hipEventRecord(b->ev_a, b->stream));
+  if (form == PQ_RUN_PERSISTENT) {
+    int rc = enqueue(b);
+    if (rc) return rc;
+  } else {
+    if (b->gexec && b->graph_gens != gens) b->drop_graph();
+    if (!b->gexec) {
+      HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      int rc = enqueue(b);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(b->stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      HIPCHK(e);
+      b->graph = g;
+      HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+      b->graph_gens = gens;
+    }
+    HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+  }
+  if (b->timed) HIPCHK(hipEventRecord(b->ev_b, b->stream));
+  return PQ_OK;
+}
+
+extern "C" int pq_batch_wait(pq_batch b) {
+  if (!b) return pq_set_error(PQ_ERR_ARG, "null batch");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return PQ_OK;
+}
+
+extern "C" int pq_batch_set_timing(pq_batch b, int enable) {
+  if (!b) return pq_set_error(PQ_ERR_ARG, "null batch");
+  b->timed = enable != 0;
+  return PQ_OK;
+}
+
+extern "C" int pq_batch_last_ms(pq_batch b, float *ms) {
+  if (!b || !ms) return pq_set_error(PQ_ERR_ARG, "null argument");
+  HIPCHK(hipEventSynchronize(b->ev_b));
+  HIPCHK(hipEventElapsedTime(ms, b->ev_a, b->ev_b));
+  return PQ_OK;
+}
+
+extern "C" int pq_batch_info(pq_batch b, uint64_t *members, uint64_t *items,
+                             uint32_t *threads, uint32_t *blocks, uint64_t *lds_budget) {
+  if (!b) return pq_set_error(PQ_ERR_ARG, "null batch");
+  if (!b->built && !b->members.empty()) {
+    int rc = build(b);
+    if (rc) return rc;
+  }
+  if (members) *members = b->members.size();
+  if (items) *items = b->nitems;
+  if (threads) *threads = b->threads;
+  if (blocks) *blocks = launch_blocks(b);
+  if (lds_budget) *lds_budget = b->lds_budget;
+  return PQ_OK;
+}
+
+// Autotuner: times each launch shape (block size x LDS budget, the latter
+// setting how many workgroups share a CU and how many waves run fully
+// determined measurements in parallel) on the batch itself and keeps the
+// fastest.  Trial runs are real runs: afterwards every member is reset to
+// |0...0> and its coin/noise sequence numbers are restored.  Choices are
+// cached per batch shape (member qubit counts, word-group counts and program
+// sizes) for the life of the process.
+extern "C" int pq_batch_autotune(pq_batch b, int reps, uint32_t *threads_out,
+                                 uint64_t *lds_out, float *ms_out) {
+  if (!b) return pq_set_error(PQ_ERR_ARG, "null batch");
+  if (b->members.empty()) return PQ_OK;
+  static std::mutex mu;
+  static std::map<std::vector<uint64_t>, std::pair<uint32_t, uint64_t>> cache;
+  std::vector<uint64_t> key;
+  for (auto &m : b->members) {
+    key.push_back(m.first->n);
+    key.push_back(m.first->W);
+    key.push_back(m.second->instrs.size());
+  }
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      int rc = pq_batch_set_launch(b, it->second.first, 0, it->second.second);
+      if (threads_out) *threads_out = it->second.first;
+      if (lds_out) *lds_out = it->second.second;
+      if (ms_out) *ms_out = -1.0f;
+      return rc;
+    }
+  }
+  std::vector<std::pair<uint64_t, uint64_t>> saved;
+  for (auto &m : b->members) saved.emplace_back(m.first->meas_seq, m.first->noise_seq);
+  const uint32_t cand_threads[] = {256, 512};
+  const uint64_t cand_lds[] = {kProgramLdsBytes, 72 * 1024, 48 * 1024};
+  float best = 1e30f;
+  uint32_t bt = b->threads;
+  uint64_t bl = b->lds_budget;
+  const bool was_timed = b->timed;
+  b->timed = true;
+  for (uint32_t t : cand_threads) {
+    for (uint64_t l : cand_lds) {
+      // every member must fit: base area + one determined-wave slab
+      bool fits = true;
+      for (auto &m : b->members)
+        if (program_lds_bytes((uint32_t)m.first->n, 1) > l) fits = false;
+      if (!fits) continue;
+      int rc = pq_batch_set_launch(b, t, 0, l);
+      if (rc) return rc;
+      float tot = 0;
+      for (int r = 0; r < std::max(1, reps) + 1; ++r) {
+        rc = pq_batch_run(b, PQ_RUN_PERSISTENT);
+        if (rc) return rc;
+        float ms = 0;
+        rc = pq_batch_last_ms(b, &ms);
+        if (rc) return rc;
+        if (r > 0) tot += ms;  // first run warms up
+      }
+      const float avg = tot / std::max(1, reps);
+      if (avg < best) {
+        best = avg;
+        bt = t;
+        bl = l;
+      }
+    }
+  }
+  b->timed = was_timed;
+  for (size_t c = 0; c < b->members.size(); ++c) {
+    pq_state s = b->members[c].first;
+    int rc = pq_reset(s);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->meas_seq = saved[c].first;
+    s->noise_seq = saved[c].second;
+  }
+  int rc = pq_batch_set_launch(b, bt, 0, bl);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    cache[key] = {bt, bl};
+  }
+  if (threads_out) *threads_out = bt;
+  if (lds_out) *lds_out = bl;
+  if (ms_out) *ms_out = best;
+  return PQ_OK;
+}

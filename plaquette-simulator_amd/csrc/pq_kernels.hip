@@ -897,10 +897,15 @@ __host__ __device__ inline size_t program_lds_words(uint32_t n, uint32_t ndw) {
   return (2ull * n + 1) + ((uint64_t)n + 2) / 2 + kMeasBatch + (uint64_t)ndw * 2 * n;
 }
 
-__global__ __launch_bounds__(kProgramMaxThreads) void k_program(pq::ProgramArgs a,
-                                                  const ProgParams *__restrict__ pp) {
-  extern __shared__ uint64_t dyn[];
-  __shared__ MeasShared sh;
+// Wall-clock ticks per phase (census mode, thread 0).
+struct PhaseTimes {
+  uint64_t gate = 0, check = 0, det = 0, gen = 0;
+};
+
+// The whole compiled program on 64-shot word group w of one state.
+__device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t mseq0,
+                             uint64_t w, uint64_t *dyn, MeasShared &sh,
+                             PhaseTimes *tm) {
   const uint32_t n = a.n;
   uint64_t *s_src = dyn;
   uint32_t *s_list = reinterpret_cast<uint32_t *>(dyn + (2 * n + 1));
@@ -908,101 +913,160 @@ __global__ __launch_bounds__(kProgramMaxThreads) void k_program(pq::ProgramArgs 
   uint64_t *s_slab = s_rm + kMeasBatch;
   const uint64_t C = 2ull * n + 1;
   const uint64_t Ru = a.R;
-  const uint64_t seed = pp->seed, mseq0 = pp->mseq0;
   const uint32_t lane = lane_id(), wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const uint32_t ndw = a.det_waves;
-  // census mode also times the phases (thread 0, wall-clock ticks)
-  const bool timing = a.census != nullptr && threadIdx.x == 0;
-  uint64_t t_gate = 0, t_check = 0, t_det = 0, t_gen = 0, t0 = 0;
-  for (uint64_t wi = blockIdx.x; wi < a.nwords; wi += gridDim.x) {
-    const uint64_t w = a.word0 + wi;
-    uint64_t *T = a.tab + w * C * a.R;
-    for (uint32_t si = 0; si < a.nseg; ++si) {
-      const DSegment sg = ld_segment(a.segs, si);
-      if (sg.kind == 0) {
-        if (timing) t0 = wall_clock64();
-        run_gate_segment(T, w, a, sg, reinterpret_cast<DInstr *>(dyn),
-                         dyn + kStage * (sizeof(DInstr) / 8));
-        if (timing) t_gate += wall_clock64() - t0;
-        continue;
-      }
-      uint32_t k = sg.begin;
-      const uint32_t end = sg.begin + sg.count;
-      uint32_t win = kMeasWindow0;
-      while (k < end) {
-        if (timing) t0 = wall_clock64();
-        // which of the next measurements have a pivot in some shot?
-        // the window grows while every checked measurement is determined and
-        // restarts small after a random one (the state then changes)
-        const uint32_t nm = min(end - k, win);
-        for (uint32_t idx = wv; idx < nm; idx += nw) {
-          const uint64_t *xq = T + (uint64_t)((cinstr_ptr)a.instrs)[k + idx].qa * Ru;
-          uint64_t acc = 0;
-          for (uint32_t p0 = 0; p0 < n; p0 += 4 * 64) {
-            uint64_t v[4];
+  const bool timing = tm != nullptr;
+  uint64_t t0 = 0;
+  uint64_t *T = a.tab + w * C * a.R;
+  for (uint32_t si = 0; si < a.nseg; ++si) {
+    const DSegment sg = ld_segment(a.segs, si);
+    if (sg.kind == 0) {
+      if (timing) t0 = wall_clock64();
+      run_gate_segment(T, w, a, sg, reinterpret_cast<DInstr *>(dyn),
+                       dyn + kStage * (sizeof(DInstr) / 8));
+      if (timing) tm->gate += wall_clock64() - t0;
+      continue;
+    }
+    uint32_t k = sg.begin;
+    const uint32_t end = sg.begin + sg.count;
+    uint32_t win = kMeasWindow0;
+    while (k < end) {
+      if (timing) t0 = wall_clock64();
+      // which of the next measurements have a pivot in some shot?  The window
+      // grows while every checked measurement is determined and restarts
+      // small after a random one (the state then changes)
+      const uint32_t nm = min(end - k, win);
+      for (uint32_t idx = wv; idx < nm; idx += nw) {
+        const uint64_t *xq = T + (uint64_t)((cinstr_ptr)a.instrs)[k + idx].qa * Ru;
+        uint64_t acc = 0;
+        for (uint32_t p0 = 0; p0 < n; p0 += 4 * 64) {
+          uint64_t v[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const uint32_t p = p0 + 64 * u + lane;
-              v[u] = p < n ? xq[n + p] : 0ull;
-            }
-            acc |= v[0] | v[1] | v[2] | v[3];
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t p = p0 + 64 * u + lane;
+            v[u] = p < n ? xq[n + p] : 0ull;
           }
-          acc = wave_or(acc);
-          if (lane == 0) s_rm[idx] = acc;
+          acc |= v[0] | v[1] | v[2] | v[3];
+        }
+        acc = wave_or(acc);
+        if (lane == 0) s_rm[idx] = acc;
+      }
+      __syncthreads();
+      uint32_t jr = nm;
+      for (uint32_t idx = 0; idx < nm; ++idx)
+        if (s_rm[idx]) {
+          jr = idx;
+          break;
+        }
+      win = (jr == nm) ? min(2 * win, kMeasBatch) : kMeasWindow0;
+      if (timing) {
+        const uint64_t t1 = wall_clock64();
+        tm->check += t1 - t0;
+        t0 = t1;
+      }
+      // measurements [k, k+jr) are determined in every shot: one wave each
+      if (jr > 0) {
+        if (wv < ndw) {
+          uint64_t *slab = s_slab + (uint64_t)wv * 2 * n;
+          for (uint32_t idx = wv; idx < jr; idx += ndw) {
+            const DInstr I = ld_instr((cinstr_ptr)a.instrs, k + idx);
+            uint64_t *rec = a.recs + (2ull * I.slot) * a.W;
+            determined_wave(T, w, n, a.R, I.qa, slab, rec, a.W, idx == jr - 1,
+                            a.census);
+          }
         }
         __syncthreads();
-        uint32_t jr = nm;
-        for (uint32_t idx = 0; idx < nm; ++idx)
-          if (s_rm[idx]) {
-            jr = idx;
-            break;
-          }
-        win = (jr == nm) ? min(2 * win, kMeasBatch) : kMeasWindow0;
-        if (timing) {
-          const uint64_t t1 = wall_clock64();
-          t_check += t1 - t0;
-          t0 = t1;
-        }
-        // measurements [k, k+jr) are determined in every shot: one wave each
-        if (jr > 0) {
-          if (wv < ndw) {
-            uint64_t *slab = s_slab + (uint64_t)wv * 2 * n;
-            for (uint32_t idx = wv; idx < jr; idx += ndw) {
-              const DInstr I = ld_instr((cinstr_ptr)a.instrs, k + idx);
-              uint64_t *rec = a.recs + (2ull * I.slot) * a.W;
-              determined_wave(T, w, n, a.R, I.qa, slab, rec, a.W, idx == jr - 1,
-                              a.census);
-            }
-          }
-          __syncthreads();
-        }
-        if (timing) {
-          const uint64_t t1 = wall_clock64();
-          t_det += t1 - t0;
-          t0 = t1;
-        }
-        if (jr < nm) {
-          const DInstr I = ld_instr((cinstr_ptr)a.instrs, k + jr);
-          const uint64_t m = (uint64_t)I.na | ((uint64_t)I.nb << 32);
-          const uint64_t key = stream_key(seed, 2 * (mseq0 + m));
-          const uint64_t *cp = a.coin_planes ? a.coin_planes + m * a.W : nullptr;
-          uint64_t *rec = a.recs + (2ull * I.slot) * a.W;
-          measure_word(T, w, n, a.R, I.qa, I.t0, key, cp, rec, rec + a.W, s_src,
-                       s_list, sh, false, a.census);
-          k += jr + 1;
-          if (timing) t_gen += wall_clock64() - t0;
-        } else {
-          k += nm;
-        }
+      }
+      if (timing) {
+        const uint64_t t1 = wall_clock64();
+        tm->det += t1 - t0;
+        t0 = t1;
+      }
+      if (jr < nm) {
+        const DInstr I = ld_instr((cinstr_ptr)a.instrs, k + jr);
+        const uint64_t m = (uint64_t)I.na | ((uint64_t)I.nb << 32);
+        const uint64_t key = stream_key(seed, 2 * (mseq0 + m));
+        const uint64_t *cp = a.coin_planes ? a.coin_planes + m * a.W : nullptr;
+        uint64_t *rec = a.recs + (2ull * I.slot) * a.W;
+        measure_word(T, w, n, a.R, I.qa, I.t0, key, cp, rec, rec + a.W, s_src,
+                     s_list, sh, false, a.census);
+        k += jr + 1;
+        if (timing) tm->gen += wall_clock64() - t0;
+      } else {
+        k += nm;
       }
     }
   }
-  if (timing) {
-    atomicAdd(a.census + 4, (unsigned long long)t_gate);
-    atomicAdd(a.census + 5, (unsigned long long)t_check);
-    atomicAdd(a.census + 6, (unsigned long long)t_det);
-    atomicAdd(a.census + 7, (unsigned long long)t_gen);
+}
+
+__global__ __launch_bounds__(kProgramMaxThreads) void k_program(pq::ProgramArgs a,
+                                                  const ProgParams *__restrict__ pp) {
+  extern __shared__ uint64_t dyn[];
+  __shared__ MeasShared sh;
+  const uint64_t seed = pp->seed, mseq0 = pp->mseq0;
+  // census mode also times the phases (thread 0, wall-clock ticks)
+  PhaseTimes times;
+  PhaseTimes *tm = (a.census != nullptr && threadIdx.x == 0) ? &times : nullptr;
+  for (uint64_t wi = blockIdx.x; wi < a.nwords; wi += gridDim.x)
+    program_word(a, seed, mseq0, a.word0 + wi, dyn, sh, tm);
+  if (tm) {
+    atomicAdd(a.census + 4, (unsigned long long)times.gate);
+    atomicAdd(a.census + 5, (unsigned long long)times.check);
+    atomicAdd(a.census + 6, (unsigned long long)times.det);
+    atomicAdd(a.census + 7, (unsigned long long)times.gen);
   }
+}
+
+// Mixed batch: many states, each with its own compiled program, in one
+// persistent launch.  Work items (state, word group), heaviest first, are
+// taken from a global counter until the list is exhausted; every workgroup
+// leaves the loop once the counter passes the item count.
+__global__ __launch_bounds__(kProgramMaxThreads) void k_batch(
+    const pq::BatchCircuit *__restrict__ circ, const uint64_t *__restrict__ items,
+    uint64_t nitems, unsigned long long *counter) {
+  extern __shared__ uint64_t dyn[];
+  __shared__ MeasShared sh;
+  __shared__ uint64_t s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1ull);
+    __syncthreads();
+    const uint64_t it = s_item;
+    __syncthreads();
+    if (it >= nitems) break;
+    const uint64_t code = items[it];
+    const uint32_t c = (uint32_t)(code >> 40);
+    const uint64_t w = code & ((1ull << 40) - 1);
+    const pq::BatchCircuit &bc = circ[c];
+    program_word(bc.a, bc.seed, bc.mseq0, w, dyn, sh, nullptr);
+  }
+}
+
+// Noise planes of every circuit of a batch (blockIdx.y = circuit).
+__global__ __launch_bounds__(256) void k_batch_noise(const pq::BatchCircuit *__restrict__ circ) {
+  const pq::BatchCircuit &bc = circ[blockIdx.y];
+  const uint64_t W = bc.a.W;
+  const uint64_t pairs = (uint64_t)bc.nnoise * W;
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
+  const uint64_t wave = blockIdx.x * (uint64_t)(blockDim.x / 64) + (threadIdx.x >> 6);
+  const uint32_t lane = lane_id();
+  uint64_t *planes = const_cast<uint64_t *>(bc.a.planes);
+  for (uint64_t p = wave; p < pairs; p += waves) {
+    const uint64_t k = p / W;
+    const uint64_t w = p - k * W;
+    const NoiseDesc d = bc.noise[k];
+    const uint64_t key = stream_key(bc.seed, 2 * (bc.nseq0 + k) + 1);
+    const uint32_t u = (uint32_t)(draw_keyed(key, 64 * w + lane) >> 32);
+    const uint64_t mx = __ballot(u < d.t1);
+    const uint64_t mz = __ballot((u >= d.t0) & (u < d.t2));
+    if (lane == 0) {
+      planes[(2 * k) * W + w] = mx;
+      planes[(2 * k + 1) * W + w] = mz;
+    }
+  }
+}
+
+__global__ void k_zero_counter(unsigned long long *counter) {
+  if (threadIdx.x == 0) *counter = 0;
 }
 
 // Noise planes of a program: plane k uses stream 2*(nseq0 + k) + 1.
@@ -1214,6 +1278,28 @@ hipError_t prepare_kernels(uint32_t n) {
   if (e == hipSuccess)
     e = ensure_lds_attr((const void *)k_program, kProgramLdsBudget);
   return e;
+}
+
+hipError_t launch_batch(hipStream_t s, const BatchCircuit *circ, uint32_t ncirc,
+                        const uint64_t *items, uint64_t nitems,
+                        unsigned long long *counter, uint32_t threads,
+                        uint32_t blocks, size_t lds_bytes, uint32_t noise_blocks) {
+  if (!nitems) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, s, counter);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (noise_blocks) {
+    hipLaunchKernelGGL(k_batch_noise, dim3(noise_blocks, ncirc), dim3(256), 0, s, circ);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(threads), lds_bytes, s, circ, items,
+                     nitems, counter);
+  return hipGetLastError();
+}
+
+hipError_t prepare_batch_kernel() {
+  return ensure_lds_attr((const void *)k_batch, kProgramLdsBudget);
 }
 
 }  // namespace pq

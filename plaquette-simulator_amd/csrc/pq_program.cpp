@@ -237,7 +237,7 @@ extern "C" int pq_program_set_launch(pq_program p, uint32_t threads, uint32_t bl
 
 // Record slots: M ops write slot <= current count (== count appends), flagged
 // ops read slot < current count, checked in program order.
-static int prepare_records(pq_state s, pq_program p) {
+int prepare_records(pq_state s, pq_program p) {
   uint64_t cur = s->nrec;
   for (uint64_t i = 0; i < p->ops.size(); ++i) {
     const pq_op &o = p->ops[i];
@@ -279,7 +279,7 @@ static int run_plain(pq_state s, pq_program p) {
   return PQ_OK;
 }
 
-static ProgramArgs program_args(pq_state s, pq_program p) {
+ProgramArgs program_args(pq_state s, pq_program p, uint64_t lds_budget) {
   ProgramArgs a{};
   a.tab = s->tab;
   a.recs = s->recs;
@@ -298,7 +298,7 @@ static ProgramArgs program_args(pq_state s, pq_program p) {
   // waves that run fully-determined measurements in parallel, each with a
   // 2n-word LDS slab, within the kernel's dynamic-LDS budget
   const uint64_t waves = p->threads / 64;
-  const uint64_t budget = 150 * 1024;
+  const uint64_t budget = lds_budget;
   const uint64_t base = program_lds_bytes(a.n, 0);
   const uint64_t slab = 16ull * a.n;
   uint64_t ndw = (budget > base && slab) ? (budget - base) / slab : 0;
@@ -320,7 +320,7 @@ static int enqueue_persistent(pq_state s, pq_program p, bool timed) {
     }
     HIPCHK(hipEventRecord(p->ev[p->ev_used], s->stream));
   }
-  HIPCHK(launch_program(s->stream, program_args(s, p), p->threads, p->blocks,
+  HIPCHK(launch_program(s->stream, program_args(s, p, kProgramLdsBytes), p->threads, p->blocks,
                         s->params_dev));
   if (timed) {
     HIPCHK(hipEventRecord(p->ev[p->ev_used + 1], s->stream));
