@@ -169,10 +169,14 @@ ORDER_Z = ("NW", "SW", "NE", "SE")
 
 
 def surface_code_memory_z(d: int, rounds: int, p: float, bias: float = 0.5):
-    """Op list of a rotated surface-code memory-Z experiment (see module doc)."""
+    """Op list of a rotated surface-code memory-Z experiment (see module doc).
+
+    One round is built op by op; the remaining rounds are copies of it with
+    their record slots shifted by the number of ancillas per round."""
     plaq, n = surface_code_layout(d)
     xs = [a for k, a, _ in plaq if k == "X"]
     ancillas = [a for _, a, _ in plaq]
+    na = len(ancillas)
     b = OpList()
 
     def noisy(touched):
@@ -180,32 +184,41 @@ def surface_code_memory_z(d: int, rounds: int, p: float, bias: float = 0.5):
             for q in touched:
                 b.noise(q, p, p, p)
 
-    for _ in range(rounds):
-        for a in xs:
-            b.h(a)
-        noisy(xs)
-        for layer in range(4):
-            touched = []
-            for kind, a, corners in plaq:
-                key = (ORDER_X if kind == "X" else ORDER_Z)[layer]
-                if key not in corners:
-                    continue
-                dq = corners[key]
-                if kind == "X":
-                    b.cx(a, dq)
-                else:
-                    b.cx(dq, a)
-                touched += [a, dq]
-            noisy(touched)
-        for a in xs:
-            b.h(a)
-        noisy(xs)
-        slots = [b.measure(a, bias) for a in ancillas]
-        for a, sl in zip(ancillas, slots):
-            b.flagged(OP_X_FLAG, a, sl)
+    for a in xs:
+        b.h(a)
+    noisy(xs)
+    for layer in range(4):
+        touched = []
+        for kind, a, corners in plaq:
+            key = (ORDER_X if kind == "X" else ORDER_Z)[layer]
+            if key not in corners:
+                continue
+            dq = corners[key]
+            if kind == "X":
+                b.cx(a, dq)
+            else:
+                b.cx(dq, a)
+            touched += [a, dq]
+        noisy(touched)
+    for a in xs:
+        b.h(a)
+    noisy(xs)
+    slots = [b.measure(a, bias) for a in ancillas]
+    for a, sl in zip(ancillas, slots):
+        b.flagged(OP_X_FLAG, a, sl)
+    one = b.array()
+    has_slot = np.isin(one[:, 0], (OP_M, OP_X_FLAG))
+    blocks = []
+    for r in range(rounds):
+        blk = one.copy()
+        blk[has_slot, 3] += np.uint32(r * na)
+        blocks.append(blk)
+    tail = OpList()
+    tail.nrec = rounds * na
     for q in range(d * d):
-        b.measure(q, bias)
-    return b.array(), n
+        tail.measure(q, bias)
+    blocks.append(tail.array())
+    return np.concatenate(blocks, axis=0), n
 
 
 # ---- the measured configurations ----
