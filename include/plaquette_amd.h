@@ -225,6 +225,35 @@ int pq_batch_info(pq_batch b, uint64_t *members, uint64_t *items, uint32_t *thre
 int pq_batch_set_timing(pq_batch b, int enable);
 int pq_batch_last_ms(pq_batch b, float *ms);
 
+/* ---- serving ----
+ * No reference counterpart.  A server keeps requests' tableaux resident in
+ * HBM and runs them concurrently: a worker thread admits queued requests
+ * first-come first-served whenever one of `slots` streams is free and the
+ * request's device bytes (tableau + records + noise planes + program) fit the
+ * budget, runs each as a persistent program on its stream, and on completion
+ * copies the result planes to the host and releases the device memory
+ * stream-ordered.  A request's results depend only on (n, B, seed, ops). */
+typedef struct pq_server_s *pq_server;
+int pq_server_create(uint64_t budget_bytes, uint32_t slots, pq_server *out);
+/* Queue a request; PQ_ERR_OOM if it can never fit the budget. */
+int pq_server_submit(pq_server sv, uint64_t num_qubits, uint64_t batch_size,
+                     int64_t seed, const pq_op *ops, uint64_t nops, uint64_t *ticket);
+/* status: 0 queued, 1 running, 2 done, < 0 failed (PQ_ERR_*); latency_s =
+ * submit -> results on host (-1 until finished); queue_s = submit -> start. */
+int pq_server_status(pq_server sv, uint64_t ticket, int *status, double *latency_s,
+                     double *queue_s);
+/* Result planes of a finished request: words[nrec][2][W] (result bits,
+ * determined bits; bit s of word w = shot 64w+s).  words may be NULL to query
+ * nrec and W.  PQ_ERR_BUSY until finished. */
+int pq_server_result(pq_server sv, uint64_t ticket, uint64_t *words, uint64_t capacity,
+                     uint64_t *nrec, uint64_t *words_per_plane);
+int pq_server_release(pq_server sv, uint64_t ticket);
+/* Block until every submitted request has finished. */
+int pq_server_drain(pq_server sv);
+int pq_server_stats(pq_server sv, uint64_t *completed, uint64_t *peak_bytes,
+                    uint32_t *max_running, uint64_t *queued);
+int pq_server_destroy(pq_server sv);
+
 /* ---- diagnostics: the reference's test-only functors ---- */
 /* RowMultiply(i <- i*k) on every shot (BatchRowMultiplyFunctor,
  * MeasurementFunctors.hpp:168-187). */

@@ -70,6 +70,17 @@ struct Batch {
   }
 };
 
+struct Server {
+  pq_server s = nullptr;
+  Server(uint64_t budget, uint32_t slots) { check(pq_server_create(budget, slots, &s), "Server"); }
+  ~Server() {
+    if (s) {
+      py::gil_scoped_release nogil;
+      pq_server_destroy(s);
+    }
+  }
+};
+
 }  // namespace
 
 PYBIND11_MODULE(plaquette_simulator_bindings, m) {
@@ -328,6 +339,52 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
         float ms = 0;
         check(pq_batch_last_ms(b.b, &ms), "Batch.last_ms");
         return ms;
+      });
+
+  py::class_<Server>(m, "Server")
+      .def(py::init<uint64_t, uint32_t>(), py::arg("budget_bytes"), py::arg("slots") = 8)
+      .def(
+          "submit",
+          [](Server &sv, uint64_t n, uint64_t B, int64_t seed,
+             py::array_t<uint32_t, py::array::c_style | py::array::forcecast> ops) {
+            uint64_t nops = 0, t = 0;
+            const pq_op *o = ops_ptr(ops, &nops);
+            check(pq_server_submit(sv.s, n, B, seed, o, nops, &t), "Server.submit");
+            return t;
+          },
+          py::arg("num_qubits"), py::arg("batch_size"), py::arg("seed"), py::arg("ops"))
+      .def("status",
+           [](Server &sv, uint64_t t) {
+             int st = 0;
+             double lat = 0, q = 0;
+             check(pq_server_status(sv.s, t, &st, &lat, &q), "Server.status");
+             return py::make_tuple(st, lat, q);
+           })
+      .def("result",
+           [](Server &sv, uint64_t t) {
+             uint64_t nrec = 0, W = 0;
+             check(pq_server_result(sv.s, t, nullptr, 0, &nrec, &W), "Server.result");
+             py::array_t<uint64_t> a({(size_t)nrec, (size_t)2, (size_t)W});
+             check(pq_server_result(sv.s, t, a.mutable_data(), nrec * 2 * W, &nrec, &W),
+                   "Server.result");
+             return a;
+           })
+      .def("release", [](Server &sv, uint64_t t) { check(pq_server_release(sv.s, t), "Server.release"); })
+      .def("drain",
+           [](Server &sv) {
+             py::gil_scoped_release nogil;
+             check(pq_server_drain(sv.s), "Server.drain");
+           })
+      .def("stats", [](Server &sv) {
+        uint64_t done = 0, peak = 0, queued = 0;
+        uint32_t maxr = 0;
+        check(pq_server_stats(sv.s, &done, &peak, &maxr, &queued), "Server.stats");
+        py::dict d;
+        d["completed"] = done;
+        d["peak_bytes"] = peak;
+        d["max_running"] = maxr;
+        d["queued"] = queued;
+        return d;
       });
 
   m.def("coin_threshold", &pq_coin_threshold);

@@ -94,17 +94,12 @@ extern "C" uint64_t pq_draw(uint64_t seed, uint64_t stream, uint64_t idx) {
 // state
 // ---------------------------------------------------------------------------
 pq_state_s::~pq_state_s() {
-  if (stream) (void)hipStreamSynchronize(stream);
-  if (tab) (void)hipFree(tab);
-  if (recs) (void)hipFree(recs);
-  if (scratch) (void)hipFree(scratch);
-  if (params_dev) (void)hipFree(params_dev);
-  if (params_host) (void)hipHostFree(params_host);
-  if (csum) (void)hipFree(csum);
-  if (stage) (void)hipFree(stage);
+  if (stream && own_stream) (void)hipStreamSynchronize(stream);
+  for (void *p : {(void *)tab, (void *)recs, (void *)scratch, params_dev, csum, (void *)stage})
+    dev_free(p, alloc_stream);
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
-  if (stream) (void)hipStreamDestroy(stream);
+  if (stream && own_stream) (void)hipStreamDestroy(stream);
 }
 
 static int check_device() {
@@ -114,7 +109,8 @@ static int check_device() {
   return PQ_OK;
 }
 
-static int alloc_state(uint64_t n, uint64_t B, int64_t seed, pq_state *out) {
+static int alloc_state(uint64_t n, uint64_t B, int64_t seed, hipStream_t borrowed,
+                       pq_state *out) {
   ARGCHK(out, "output handle is null");
   int rc = check_device();
   if (rc) return rc;
@@ -127,20 +123,26 @@ static int alloc_state(uint64_t n, uint64_t B, int64_t seed, pq_state *out) {
   s->W = (B + 63) / 64;
   s->seed = seed >= 0 ? seed : kDefaultSeed;
   HIPCHK(hipGetDevice(&s->device));
-  HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  if (borrowed) {
+    s->stream = borrowed;
+    s->own_stream = false;
+    s->alloc_stream = borrowed;
+  } else {
+    HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  }
   if (n > 0 && B > 0) {
     const uint64_t words = s->W * s->C * s->R;
-    if (hipMalloc(&s->tab, words * 8) != hipSuccess) {
+    if (dev_alloc((void **)&s->tab, words * 8, s->alloc_stream) != hipSuccess) {
       (void)hipGetLastError();
       return fail(PQ_ERR_OOM, "tableau allocation failed (" +
                                   std::to_string(words * 8) + " bytes)");
     }
   }
-  HIPCHK(hipMalloc(&s->scratch, std::max<uint64_t>(1, 3 * s->W) * 8));
-  HIPCHK(hipMalloc(&s->params_dev, 64));
-  HIPCHK(hipHostMalloc(&s->params_host, 64, hipHostMallocDefault));
-  HIPCHK(hipMalloc(&s->csum, 8));
-  HIPCHK(prepare_kernels((uint32_t)std::min<uint64_t>(n, measure_max_qubits())));
+  HIPCHK(dev_alloc((void **)&s->scratch, std::max<uint64_t>(1, 3 * s->W) * 8, s->alloc_stream));
+  HIPCHK(dev_alloc(&s->params_dev, 64, s->alloc_stream));
+  HIPCHK(dev_alloc(&s->csum, 8, s->alloc_stream));
+  if (!borrowed)
+    HIPCHK(prepare_kernels((uint32_t)std::min<uint64_t>(n, measure_max_qubits())));
   *out = s.release();
   return PQ_OK;
 }
@@ -149,9 +151,18 @@ Geom geom_of(const pq_state_s *s) {
   return Geom{(uint32_t)s->n, (uint32_t)s->R, (uint32_t)s->C, s->W, s->B};
 }
 
+int create_state_on(uint64_t n, uint64_t B, int64_t seed, hipStream_t stream,
+                    pq_state *out) {
+  int rc = alloc_state(n, B, seed, stream, out);
+  if (rc) return rc;
+  pq_state s = *out;
+  if (s->tab) HIPCHK(launch_fill_identity(s->stream, s->tab, geom_of(s)));
+  return PQ_OK;
+}
+
 extern "C" int pq_create(uint64_t n, uint64_t B, int64_t seed, int init_zero,
                          pq_state *out) {
-  int rc = alloc_state(n, B, seed, out);
+  int rc = alloc_state(n, B, seed, nullptr, out);
   if (rc) return rc;
   pq_state s = *out;
   if (s->tab) {
@@ -232,10 +243,11 @@ int ensure_records(pq_state s, uint64_t count) {
     }
     return PQ_OK;
   }
-  uint64_t cap = std::max<uint64_t>({count, 2 * s->rec_cap, 64});
+  // server-owned states size the arena exactly (they know their program)
+  uint64_t cap = s->alloc_stream ? count : std::max<uint64_t>({count, 2 * s->rec_cap, 64});
   uint64_t *nr = nullptr;
   const uint64_t bytes = cap * 2 * std::max<uint64_t>(s->W, 1) * 8;
-  if (hipMalloc(&nr, bytes) != hipSuccess) {
+  if (dev_alloc((void **)&nr, bytes, s->alloc_stream) != hipSuccess) {
     (void)hipGetLastError();
     return fail(PQ_ERR_OOM, "record arena allocation failed");
   }
@@ -243,8 +255,8 @@ int ensure_records(pq_state s, uint64_t count) {
   if (s->recs) {
     HIPCHK(hipMemcpyAsync(nr, s->recs, s->rec_cap * 2 * s->W * 8,
                           hipMemcpyDeviceToDevice, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    HIPCHK(hipFree(s->recs));
+    if (!s->alloc_stream) HIPCHK(hipStreamSynchronize(s->stream));
+    dev_free(s->recs, s->alloc_stream);
   }
   s->recs = nr;
   s->rec_cap = cap;
@@ -367,7 +379,8 @@ static int zero_scratch_tail(pq_state s) {
 // Copy a host int32[B] vector to the device and turn it into a shot mask
 // plane (nonzero -> bit set).  Synchronous: the staging buffer is reused.
 static int stage_host_ints(pq_state s, const int32_t *v, uint64_t *plane) {
-  if (!s->stage) HIPCHK(hipMalloc(&s->stage, std::max<uint64_t>(s->B, 1) * 4));
+  if (!s->stage)
+    HIPCHK(dev_alloc((void **)&s->stage, std::max<uint64_t>(s->B, 1) * 4, s->alloc_stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   HIPCHK(hipMemcpy(s->stage, v, s->B * 4, hipMemcpyHostToDevice));
   HIPCHK(launch_flags_to_plane(s->stream, s->stage, s->B, plane));

@@ -24,8 +24,12 @@ struct pq_state_s {
   bool scratch_zeroed = false;
   int32_t *stage = nullptr;     // int32[B] host-vector staging
   void *params_dev = nullptr;   // ProgParams of the running program
-  void *params_host = nullptr;  // pinned mirror
   void *csum = nullptr;
+  // States the server creates borrow a slot stream and allocate with the
+  // stream-ordered allocator on it (alloc_stream != nullptr), so freeing
+  // them never waits for work on other streams.
+  hipStream_t alloc_stream = nullptr;
+  bool own_stream = true;
   hipEvent_t ev[8] = {};
   std::vector<std::vector<uint64_t>> rec_host;  // host copies of records
   std::vector<char> rec_valid;
@@ -33,6 +37,21 @@ struct pq_state_s {
 };
 
 int pq_set_error(int code, const char *msg);
+
+// Synchronous (hipMalloc/hipFree) or, with a stream, stream-ordered
+// (hipMallocAsync/hipFreeAsync) device allocation.
+inline hipError_t dev_alloc(void **p, size_t bytes, hipStream_t async_stream) {
+  return async_stream ? hipMallocAsync(p, bytes, async_stream) : hipMalloc(p, bytes);
+}
+inline void dev_free(void *p, hipStream_t async_stream) {
+  if (!p) return;
+  if (async_stream)
+    (void)hipFreeAsync(p, async_stream);
+  else
+    (void)hipFree(p);
+}
+// pq_create on a borrowed stream with stream-ordered allocation (server use).
+int create_state_on(uint64_t n, uint64_t B, int64_t seed, hipStream_t stream, pq_state *out);
 pq::Geom geom_of(const pq_state_s *s);
 int ensure_records(pq_state s, uint64_t count);
 uint64_t *record_ptr(pq_state s, uint64_t id);

@@ -586,16 +586,6 @@ constexpr uint32_t kProgramMaxThreads = 512;  // block-size bound of k_program
 constexpr uint32_t kMeasBatch = 64;   // most measurements pivot-checked per pass
 constexpr uint32_t kMeasWindow0 = 8;  // first window of a run
 
-__device__ __forceinline__ void apply_fused_noise(const uint64_t *__restrict__ planes,
-                                                  uint64_t W, uint64_t w, uint32_t k,
-                                                  uint64_t x, uint64_t z, uint64_t &r) {
-  if (k != kNone) {
-    const uint64_t fx = planes[(2ull * k) * W + w];
-    const uint64_t fz = planes[(2ull * k + 1) * W + w];
-    r ^= (fx & z) ^ (fz & x);
-  }
-}
-
 // Gate segment, processed in chunks of up to kStage instructions.  Each
 // chunk's instructions and the per-word shot masks they need (fused noise
 // planes, flags) are staged in LDS by the whole workgroup in one round trip;

@@ -158,54 +158,62 @@ void pq_program_s::drop_graph() {
 
 pq_program_s::~pq_program_s() {
   drop_graph();
-  if (d_census) (void)hipFree(d_census);
   for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-  if (d_segs) (void)hipFree(d_segs);
-  if (d_instrs) (void)hipFree(d_instrs);
-  if (d_noise) (void)hipFree(d_noise);
-  if (d_planes) (void)hipFree(d_planes);
+  for (void *q : {(void *)d_census, (void *)d_segs, (void *)d_instrs, (void *)d_noise,
+                  (void *)d_planes})
+    dev_free(q, alloc_stream);
 }
 
-extern "C" int pq_program_create(pq_state s, const pq_op *ops, uint64_t nops,
-                                 pq_program *out) {
+template <class T>
+static int upload(T **dst, const std::vector<T> &src, hipStream_t as) {
+  if (src.empty()) return PQ_OK;
+  const size_t bytes = src.size() * sizeof(T);
+  HIPCHK(dev_alloc((void **)dst, bytes, as));
+  if (as)
+    HIPCHK(hipMemcpyAsync(*dst, src.data(), bytes, hipMemcpyHostToDevice, as));
+  else
+    HIPCHK(hipMemcpy(*dst, src.data(), bytes, hipMemcpyHostToDevice));
+  return PQ_OK;
+}
+
+static int create_program(pq_state s, const pq_op *ops, uint64_t nops, hipStream_t as,
+                          pq_program *out) {
   if (!s || !out || (!ops && nops)) return pq_set_error(PQ_ERR_ARG, "null argument");
   int rc = validate(s->n, ops, nops);
   if (rc) return rc;
   auto p = std::make_unique<pq_program_s>();
+  p->alloc_stream = as;
   p->n = s->n;
   p->W = s->W;
   p->R = s->R;
   p->ops.assign(ops, ops + nops);
   rc = compile_program(p.get());
   if (rc) return rc;
-  // upload
-  if (!p->segs.empty()) {
-    HIPCHK(hipMalloc(&p->d_segs, p->segs.size() * sizeof(DSegment)));
-    HIPCHK(hipMemcpy(p->d_segs, p->segs.data(), p->segs.size() * sizeof(DSegment),
-                     hipMemcpyHostToDevice));
-  }
-  if (!p->instrs.empty()) {
-    HIPCHK(hipMalloc(&p->d_instrs, p->instrs.size() * sizeof(DInstr)));
-    HIPCHK(hipMemcpy(p->d_instrs, p->instrs.data(), p->instrs.size() * sizeof(DInstr),
-                     hipMemcpyHostToDevice));
-  }
+  if ((rc = upload(&p->d_segs, p->segs, as))) return rc;
+  if ((rc = upload(&p->d_instrs, p->instrs, as))) return rc;
   if (p->nnoise) {
-    HIPCHK(hipMalloc(&p->d_noise, p->noise.size() * sizeof(NoiseDesc)));
-    HIPCHK(hipMemcpy(p->d_noise, p->noise.data(), p->noise.size() * sizeof(NoiseDesc),
-                     hipMemcpyHostToDevice));
+    if ((rc = upload(&p->d_noise, p->noise, as))) return rc;
     const uint64_t bytes = (uint64_t)p->nnoise * 2 * std::max<uint64_t>(p->W, 1) * 8;
-    if (hipMalloc(&p->d_planes, bytes) != hipSuccess) {
+    if (dev_alloc((void **)&p->d_planes, bytes, as) != hipSuccess) {
       (void)hipGetLastError();
       return pq_set_error(PQ_ERR_OOM, "noise plane allocation failed");
     }
   }
   // the measurement phases use every wave, so the default is the largest block
-  const uint64_t th = kProgramThreadsMax;
-  p->threads = (uint32_t)std::min<uint64_t>(kProgramThreadsMax, std::max<uint64_t>(64, th));
+  p->threads = kProgramThreadsMax;
   p->blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(s->W, 1u << 20));
-  HIPCHK(prepare_kernels((uint32_t)s->n));
+  if (!as) HIPCHK(prepare_kernels((uint32_t)s->n));
   *out = p.release();
   return PQ_OK;
+}
+
+int create_program_on(pq_state s, const pq_op *ops, uint64_t nops, pq_program *out) {
+  return create_program(s, ops, nops, s->stream, out);
+}
+
+extern "C" int pq_program_create(pq_state s, const pq_op *ops, uint64_t nops,
+                                 pq_program *out) {
+  return create_program(s, ops, nops, nullptr, out);
 }
 
 extern "C" int pq_program_destroy(pq_program p) {
@@ -376,7 +384,8 @@ extern "C" int pq_program_run(pq_state s, pq_program p, int form) {
 
 extern "C" int pq_program_set_census(pq_program p, int enable) {
   if (!p) return pq_set_error(PQ_ERR_ARG, "null program");
-  if (enable && !p->d_census) HIPCHK(hipMalloc(&p->d_census, 8 * sizeof(unsigned long long)));
+  if (enable && !p->d_census)
+    HIPCHK(dev_alloc((void **)&p->d_census, 8 * sizeof(unsigned long long), p->alloc_stream));
   if (enable) HIPCHK(hipMemset(p->d_census, 0, 8 * sizeof(unsigned long long)));
   if ((bool)enable != p->census_on) p->drop_graph();
   p->census_on = enable != 0;

@@ -29,7 +29,8 @@ struct pq_program_s {
   hipGraphExec_t gexec = nullptr;
   pq_state graph_state = nullptr;
   uint64_t graph_arena_gen = 0;
-  unsigned long long *d_census = nullptr;  // 4 counters when census is on
+  hipStream_t alloc_stream = nullptr;  // stream-ordered allocation (server use)
+  unsigned long long *d_census = nullptr;  // 8 counters when census is on
   bool census_on = false;
   bool timing_on = false;
   std::vector<hipEvent_t> ev;  // start/end pairs of timed kernel launches
@@ -45,3 +46,6 @@ constexpr uint64_t kProgramLdsBytes = 150 * 1024;  // dynamic-LDS cap (kProgramL
 // Shared with the batch runner.
 int prepare_records(pq_state s, pq_program p);
 pq::ProgramArgs program_args(pq_state s, pq_program p, uint64_t lds_budget);
+// pq_program_create for a server-owned state: stream-ordered allocation and
+// copies on the state's stream.
+int create_program_on(pq_state s, const pq_op *ops, uint64_t nops, pq_program *out);

@@ -1,0 +1,310 @@
+// Serving: circuit-sampling requests arrive at any time; a worker thread
+// admits them first-come first-served as soon as a slot (stream) is free and
+// the request's device bytes fit the HBM budget, runs each as a persistent
+// program on its slot stream (so requests overlap on the GPU), detects
+// completion with events, copies the result planes to the host and releases
+// the device memory stream-ordered (no device-wide synchronisation).
+//
+// A request's results depend only on (n, B, seed, ops): coins and noise are
+// keyed by its own seed, sequence numbers and shot indices, never by launch
+// position, so they are identical alone and in any mix.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "plaquette_amd.h"
+#include "pq_capi_internal.h"
+#include "pq_common.h"
+#include "pq_kernels.h"
+#include "pq_program_internal.h"
+
+namespace {
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Req {
+  uint64_t ticket = 0;
+  uint64_t n = 0, B = 0;
+  int64_t seed = 0;
+  std::vector<pq_op> ops;
+  uint64_t nrec = 0, W = 0, bytes = 0;
+  double t_submit = 0, t_start = 0, t_done = 0;
+  int status = 0;  // 0 queued, 1 running, 2 done, < 0 failed (PQ_ERR_*)
+  int slot = -1;
+  pq_state st = nullptr;
+  pq_program prog = nullptr;
+  hipEvent_t ev = nullptr;
+  std::vector<uint64_t> planes;  // [nrec][2][W] once done
+};
+
+}  // namespace
+
+struct pq_server_s {
+  uint64_t budget = 0, used = 0, peak = 0;
+  std::vector<hipStream_t> streams;
+  std::vector<char> busy;
+  int device = 0;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Req *> pending;
+  std::vector<Req *> running;
+  std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
+  uint64_t next_ticket = 1, completed = 0;
+  uint32_t max_running = 0;
+  std::atomic<bool> stop{false};
+  std::thread worker;
+  void loop();
+  int admit(Req *r);
+  void finish(Req *r);
+};
+
+// Device bytes a request reserves: tableau, records, noise planes, program.
+static uint64_t request_bytes(uint64_t n, uint64_t B, const pq_op *ops, uint64_t nops,
+                              uint64_t *nrec_out) {
+  const uint64_t W = (B + 63) / 64, C = 2 * n + 1;
+  uint64_t nrec = 0, nnoise = 0;
+  for (uint64_t i = 0; i < nops; ++i) {
+    if (ops[i].code == PQ_OP_M && ops[i].slot + 1 > nrec) nrec = ops[i].slot + 1;
+    if (ops[i].code == PQ_OP_NOISE) ++nnoise;
+  }
+  *nrec_out = nrec;
+  return W * C * C * 8 + nrec * 2 * W * 8 + nnoise * (2 * W * 8 + sizeof(pq::NoiseDesc)) +
+         nops * sizeof(pq::DInstr) + 3 * W * 8 + 4096;
+}
+
+int pq_server_s::admit(Req *r) {
+  const hipStream_t s = streams[r->slot];
+  int rc = create_state_on(r->n, r->B, r->seed, s, &r->st);
+  if (rc == PQ_OK) rc = create_program_on(r->st, r->ops.data(), r->ops.size(), &r->prog);
+  if (rc == PQ_OK) rc = pq_program_run(r->st, r->prog, PQ_RUN_PERSISTENT);
+  if (rc == PQ_OK && hipEventRecord(r->ev, s) != hipSuccess) rc = PQ_ERR_HIP;
+  return rc;
+}
+
+void pq_server_s::finish(Req *r) {
+  const hipStream_t s = streams[r->slot];
+  if (r->status == 1 && r->st && r->st->recs && r->nrec) {
+    r->planes.resize(r->nrec * 2 * r->W);
+    if (hipMemcpyAsync(r->planes.data(), r->st->recs, r->planes.size() * 8,
+                       hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      r->status = PQ_ERR_HIP;
+  }
+  if (r->prog) pq_program_destroy(r->prog);
+  if (r->st) pq_destroy(r->st);
+  r->prog = nullptr;
+  r->st = nullptr;
+}
+
+void pq_server_s::loop() {
+  (void)hipSetDevice(device);
+  while (!stop.load()) {
+    std::vector<Req *> done, start;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      for (size_t i = 0; i < running.size();) {
+        hipError_t q = hipEventQuery(running[i]->ev);
+        if (q == hipErrorNotReady) {
+          ++i;
+          continue;
+        }
+        Req *r = running[i];
+        if (q != hipSuccess) r->status = PQ_ERR_HIP;
+        done.push_back(r);
+        running[i] = running.back();
+        running.pop_back();
+      }
+      for (size_t sl = 0; sl < busy.size() && !pending.empty(); ++sl) {
+        if (busy[sl]) continue;
+        Req *r = pending.front();
+        if (used + r->bytes > budget) break;  // first come, first served
+        pending.pop_front();
+        busy[sl] = 1;
+        r->slot = (int)sl;
+        used += r->bytes;
+        peak = std::max(peak, used);
+        start.push_back(r);
+      }
+      if (done.empty() && start.empty()) {
+        cv.wait_for(lk, std::chrono::microseconds(200));
+        continue;
+      }
+    }
+    for (Req *r : start) {
+      r->t_start = now_s();
+      r->status = 1;
+      const int rc = admit(r);
+      if (rc != PQ_OK) {
+        r->status = rc;
+        done.push_back(r);
+      } else {
+        std::lock_guard<std::mutex> g(mu);
+        running.push_back(r);
+        max_running = std::max<uint32_t>(max_running, (uint32_t)running.size());
+      }
+    }
+    for (Req *r : done) {
+      finish(r);
+      std::lock_guard<std::mutex> g(mu);
+      if (r->status == 1) r->status = 2;
+      r->t_done = now_s();
+      busy[r->slot] = 0;
+      used -= r->bytes;
+      ++completed;
+      cv.notify_all();
+    }
+  }
+}
+
+extern "C" int pq_server_create(uint64_t budget_bytes, uint32_t slots, pq_server *out) {
+  if (!out || !budget_bytes || !slots || slots > 64)
+    return pq_set_error(PQ_ERR_ARG, "server needs an output handle, a budget and 1..64 slots");
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess || c < 1)
+    return pq_set_error(PQ_ERR_NODEV, "no HIP device available");
+  auto sv = std::make_unique<pq_server_s>();
+  sv->budget = budget_bytes;
+  if (hipGetDevice(&sv->device) != hipSuccess) return pq_set_error(PQ_ERR_HIP, "hipGetDevice");
+  // keep freed request memory in the stream-ordered pool for reuse
+  hipMemPool_t pool = nullptr;
+  if (hipDeviceGetDefaultMemPool(&pool, sv->device) == hipSuccess && pool) {
+    uint64_t keep = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  }
+  sv->streams.resize(slots);
+  sv->busy.assign(slots, 0);
+  for (auto &s : sv->streams)
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+      return pq_set_error(PQ_ERR_HIP, "stream creation failed");
+  if (pq::prepare_kernels(pq::measure_max_qubits()) != hipSuccess)
+    return pq_set_error(PQ_ERR_HIP, "kernel preparation failed");
+  pq_server_s *raw = sv.get();
+  raw->worker = std::thread([raw] { raw->loop(); });
+  *out = sv.release();
+  return PQ_OK;
+}
+
+extern "C" int pq_server_submit(pq_server sv, uint64_t n, uint64_t B, int64_t seed,
+                                const pq_op *ops, uint64_t nops, uint64_t *ticket) {
+  if (!sv || !ticket || (!ops && nops)) return pq_set_error(PQ_ERR_ARG, "null argument");
+  if (n == 0 || B == 0) return pq_set_error(PQ_ERR_ARG, "request needs qubits and shots");
+  if (n > pq::measure_max_qubits())
+    return pq_set_error(PQ_ERR_UNSUPPORTED, "requests support at most 4095 qubits");
+  for (uint64_t i = 0; i < nops; ++i) {
+    const pq_op &o = ops[i];
+    const bool two = o.code == PQ_OP_CX || o.code == PQ_OP_CZ;
+    if (o.code < PQ_OP_H || o.code > PQ_OP_M || o.q0 >= n ||
+        (two && (o.q1 >= n || o.q1 == o.q0)))
+      return pq_set_error(PQ_ERR_ARG, ("request op " + std::to_string(i) + " is invalid").c_str());
+  }
+  auto r = std::make_unique<Req>();
+  r->n = n;
+  r->B = B;
+  r->W = (B + 63) / 64;
+  r->seed = seed;
+  r->ops.assign(ops, ops + nops);
+  r->bytes = request_bytes(n, B, ops, nops, &r->nrec);
+  if (r->bytes > sv->budget)
+    return pq_set_error(PQ_ERR_OOM, "request can never fit the server's memory budget");
+  if (hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
+    return pq_set_error(PQ_ERR_HIP, "event creation failed");
+  std::lock_guard<std::mutex> g(sv->mu);
+  r->ticket = sv->next_ticket++;
+  r->t_submit = now_s();
+  *ticket = r->ticket;
+  sv->pending.push_back(r.get());
+  sv->reqs[r->ticket] = std::move(r);
+  sv->cv.notify_all();
+  return PQ_OK;
+}
+
+extern "C" int pq_server_status(pq_server sv, uint64_t ticket, int *status, double *latency_s,
+                                double *queue_s) {
+  if (!sv || !status) return pq_set_error(PQ_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> g(sv->mu);
+  auto it = sv->reqs.find(ticket);
+  if (it == sv->reqs.end()) return pq_set_error(PQ_ERR_RANGE, "unknown ticket");
+  const Req &r = *it->second;
+  *status = r.status;
+  const bool fin = r.status == 2 || r.status < 0;
+  if (latency_s) *latency_s = fin ? r.t_done - r.t_submit : -1.0;
+  if (queue_s) *queue_s = r.t_start > 0 ? r.t_start - r.t_submit : -1.0;
+  return PQ_OK;
+}
+
+extern "C" int pq_server_result(pq_server sv, uint64_t ticket, uint64_t *words,
+                                uint64_t capacity, uint64_t *nrec, uint64_t *W) {
+  if (!sv) return pq_set_error(PQ_ERR_ARG, "null server");
+  std::lock_guard<std::mutex> g(sv->mu);
+  auto it = sv->reqs.find(ticket);
+  if (it == sv->reqs.end()) return pq_set_error(PQ_ERR_RANGE, "unknown ticket");
+  const Req &r = *it->second;
+  if (r.status != 2) return pq_set_error(PQ_ERR_BUSY, "request not finished");
+  if (nrec) *nrec = r.nrec;
+  if (W) *W = r.W;
+  if (words) {
+    if (capacity < r.planes.size()) return pq_set_error(PQ_ERR_ARG, "result buffer too small");
+    std::copy(r.planes.begin(), r.planes.end(), words);
+  }
+  return PQ_OK;
+}
+
+extern "C" int pq_server_release(pq_server sv, uint64_t ticket) {
+  if (!sv) return pq_set_error(PQ_ERR_ARG, "null server");
+  std::lock_guard<std::mutex> g(sv->mu);
+  auto it = sv->reqs.find(ticket);
+  if (it == sv->reqs.end()) return pq_set_error(PQ_ERR_RANGE, "unknown ticket");
+  if (it->second->status == 0 || it->second->status == 1)
+    return pq_set_error(PQ_ERR_BUSY, "request not finished");
+  (void)hipEventDestroy(it->second->ev);
+  sv->reqs.erase(it);
+  return PQ_OK;
+}
+
+extern "C" int pq_server_drain(pq_server sv) {
+  if (!sv) return pq_set_error(PQ_ERR_ARG, "null server");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  sv->cv.wait(lk, [sv] { return sv->pending.empty() && sv->running.empty() &&
+                                std::none_of(sv->busy.begin(), sv->busy.end(),
+                                             [](char b) { return b != 0; }); });
+  return PQ_OK;
+}
+
+extern "C" int pq_server_stats(pq_server sv, uint64_t *completed, uint64_t *peak_bytes,
+                               uint32_t *max_running, uint64_t *queued) {
+  if (!sv) return pq_set_error(PQ_ERR_ARG, "null server");
+  std::lock_guard<std::mutex> g(sv->mu);
+  if (completed) *completed = sv->completed;
+  if (peak_bytes) *peak_bytes = sv->peak;
+  if (max_running) *max_running = sv->max_running;
+  if (queued) *queued = sv->pending.size();
+  return PQ_OK;
+}
+
+extern "C" int pq_server_destroy(pq_server sv) {
+  if (!sv) return PQ_OK;
+  (void)pq_server_drain(sv);
+  sv->stop.store(true);
+  sv->cv.notify_all();
+  if (sv->worker.joinable()) sv->worker.join();
+  for (auto &kv : sv->reqs)
+    if (kv.second->ev) (void)hipEventDestroy(kv.second->ev);
+  for (hipStream_t s : sv->streams) {
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+  }
+  delete sv;
+  return PQ_OK;
+}
