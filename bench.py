@@ -179,6 +179,106 @@ def bench_mixed(args):
     print(json.dumps(out))
 
 
+def bench_serving(args):
+    """Poisson arrivals of surface-code sampling requests (d uniform over odd
+    {3..21}, rounds d, shots log-uniform in [256, 65536], p uniform in
+    [5e-4, 5e-3], seed 2024) at rates lambda in {50, 200, 800}/s, served by
+    the engine's request loop with a 200 GB tableau budget.  Latency = arrival
+    -> results on the host; throughput = all gate-shots / (last completion -
+    first arrival)."""
+    import math
+    import numpy as np
+    import plaquette_simulator as ps
+    from plaquette_simulator import workloads as W
+
+    rng = np.random.default_rng(2024)
+    budget = int(args.serving_budget_gb * 1e9)
+    lines = []
+    cache = {}
+    for lam in (50.0, 200.0, 800.0):
+        reqs = []
+        t_arr = 0.0
+        for _ in range(args.serving_requests):
+            t_arr += rng.exponential(1.0 / lam)
+            d = int(rng.choice(list(range(3, 22, 2))))
+            shots = int(round(math.exp(rng.uniform(math.log(256), math.log(65536)))))
+            p = float(rng.uniform(5e-4, 5e-3))
+            seed = int(rng.integers(0, 2 ** 31 - 1))
+            key = (d, round(p, 9))
+            if key not in cache:
+                cache[key] = W.surface_code_memory_z(d, d, p)
+            ops, n = cache[key]
+            gates = int(np.isin(ops[:, 0], W.GATE_CODES).sum())
+            reqs.append((t_arr, n, shots, seed, ops, gates))
+        sv = ps.Server(budget_bytes=budget, slots=args.serving_slots)
+        tickets = []
+        t0 = time.perf_counter()
+        for t_arr, n, shots, seed, ops, gates in reqs:
+            now = time.perf_counter() - t0
+            if t_arr > now:
+                time.sleep(t_arr - now)
+            tickets.append(sv.submit(n, shots, seed, ops))
+        sv.drain()
+        t_end = time.perf_counter() - t0
+        lat = np.array([sv.status(t)[1] for t in tickets])
+        queue = np.array([sv.status(t)[2] for t in tickets])
+        stats = sv.stats()
+        gate_shots = sum(g * s for _, _, s, _, _, g in reqs)
+        # the span from the first arrival to the last result
+        span = t_end - reqs[0][0]
+        lines.append({"lambda_per_s": lam, "requests": len(reqs),
+                      "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
+                      "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
+                      "p50_queue_ms": float(np.percentile(queue, 50) * 1e3),
+                      "gate_shots_per_s": gate_shots / span,
+                      "gate_shots": gate_shots, "span_s": span,
+                      "offered_s": reqs[-1][0],
+                      "max_concurrent": stats["max_running"],
+                      "peak_bytes": stats["peak_bytes"]})
+        for t in tickets:
+            sv.release(t)
+        del sv
+        if lam == 50.0:
+            alone_reqs = reqs
+    # The first rate's requests once more, one at a time on one stream
+    # (state allocation + persistent run + sync per request), and their
+    # algorithmic bytes from the engine's work census.
+    abytes = 0
+    for t_arr, n, shots, seed, ops, gates in alone_reqs:
+        st = ps.CliffordState(n, shots, seed, True)
+        pr = ps.Program(st, ops)
+        pr.set_census(True)
+        pr.run(st, "persistent")
+        abytes += algorithmic_bytes(ops, n, (shots + 63) // 64, pr.read_census())["total"]
+        del pr, st
+    t = time.perf_counter()
+    for t_arr, n, shots, seed, ops, gates in alone_reqs:
+        st = ps.CliffordState(n, shots, seed, True)
+        st.run_program(ops, "persistent")
+        st.synchronize()
+        del st
+    alone_s = time.perf_counter() - t
+    gs0 = lines[0]["gate_shots"]
+    lines[0]["one_at_a_time_gate_shots_per_s"] = gs0 / alone_s
+    lines[0]["algorithmic_bytes"] = abytes
+    lines[0]["hbm_frac_served"] = abytes / lines[0]["span_s"] / (HBM_PEAK_GBS * 1e9)
+    out = {
+        "metric": "serving: p50/p99 latency and aggregate gate-shots/s under Poisson load",
+        "value": max(l["gate_shots_per_s"] for l in lines),
+        "unit": "gate-shots/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "surface_code_requests_poisson", "d": "3..21 odd",
+                   "shots": "log-uniform [256, 65536]", "p": "U[5e-4, 5e-3]", "seed": 2024,
+                   "budget_bytes": budget, "slots": args.serving_slots,
+                   "requests_per_rate": args.serving_requests},
+        "rates": lines,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +292,11 @@ def main():
     ap.add_argument("--mixed", action="store_true",
                     help="measure the 256-circuit mixed batch instead of the d=9 workload")
     ap.add_argument("--mixed-count", type=int, default=256)
+    ap.add_argument("--serving", action="store_true",
+                    help="measure the serving loop under Poisson arrivals instead")
+    ap.add_argument("--serving-requests", type=int, default=100)
+    ap.add_argument("--serving-slots", type=int, default=8)
+    ap.add_argument("--serving-budget-gb", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X "
@@ -201,6 +306,9 @@ def main():
 
     if args.mixed:
         bench_mixed(args)
+        return
+    if args.serving:
+        bench_serving(args)
         return
 
     import plaquette_simulator as ps
