@@ -253,6 +253,15 @@ int pq_server_drain(pq_server sv);
 int pq_server_stats(pq_server sv, uint64_t *completed, uint64_t *peak_bytes,
                     uint32_t *max_running, uint64_t *queued);
 int pq_server_destroy(pq_server sv);
+/* Admission order.  PQ_SERVE_FCFS (the default): arrival order, a request
+ * that does not fit the free budget holds back the ones behind it.
+ * PQ_SERVE_SMALLEST_FIRST: the queued request with the least estimated work
+ * (word groups x rows x ops) that fits goes first, except that a request
+ * queued longer than age_limit_s (> 0) goes before all younger ones, so large
+ * requests are not starved under sustained load. */
+#define PQ_SERVE_FCFS 0
+#define PQ_SERVE_SMALLEST_FIRST 1
+int pq_server_set_policy(pq_server sv, int policy, double age_limit_s);
 
 /* ---- diagnostics: the reference's test-only functors ---- */
 /* RowMultiply(i <- i*k) on every shot (BatchRowMultiplyFunctor,

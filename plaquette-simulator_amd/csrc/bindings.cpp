@@ -370,6 +370,16 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
              return a;
            })
       .def("release", [](Server &sv, uint64_t t) { check(pq_server_release(sv.s, t), "Server.release"); })
+      .def(
+          "set_policy",
+          [](Server &sv, const std::string &policy, double age_limit_s) {
+            int p = -1;
+            if (policy == "fcfs") p = PQ_SERVE_FCFS;
+            if (policy == "smallest_first") p = PQ_SERVE_SMALLEST_FIRST;
+            if (p < 0) throw std::runtime_error("Server.set_policy: 'fcfs' or 'smallest_first'");
+            check(pq_server_set_policy(sv.s, p, age_limit_s), "Server.set_policy");
+          },
+          py::arg("policy"), py::arg("age_limit_s") = 0.0)
       .def("drain",
            [](Server &sv) {
              py::gil_scoped_release nogil;

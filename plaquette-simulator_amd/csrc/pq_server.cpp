@@ -1,6 +1,7 @@
 // Serving: circuit-sampling requests arrive at any time; a worker thread
-// admits them first-come first-served as soon as a slot (stream) is free and
-// the request's device bytes fit the HBM budget, runs each as a persistent
+// admits them (first-come first-served, or smallest estimated work first with
+// an age limit) as soon as a slot (stream) is free and the request's device
+// bytes fit the HBM budget, runs each as a persistent
 // program on its slot stream (so requests overlap on the GPU), detects
 // completion with events, copies the result planes to the host and releases
 // the device memory stream-ordered (no device-wide synchronisation).
@@ -40,6 +41,7 @@ struct Req {
   int64_t seed = 0;
   std::vector<pq_op> ops;
   uint64_t nrec = 0, W = 0, bytes = 0;
+  double cost = 0;  // estimated work: word groups x rows x ops
   double t_submit = 0, t_start = 0, t_done = 0;
   int status = 0;  // 0 queued, 1 running, 2 done, < 0 failed (PQ_ERR_*)
   int slot = -1;
@@ -63,9 +65,12 @@ struct pq_server_s {
   std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
   uint64_t next_ticket = 1, completed = 0;
   uint32_t max_running = 0;
+  int policy = PQ_SERVE_FCFS;
+  double age_limit = 0;
   std::atomic<bool> stop{false};
   std::thread worker;
   void loop();
+  Req *next_fitting(double now);
   int admit(Req *r);
   void finish(Req *r);
 };
@@ -82,6 +87,27 @@ static uint64_t request_bytes(uint64_t n, uint64_t B, const pq_op *ops, uint64_t
   *nrec_out = nrec;
   return W * C * C * 8 + nrec * 2 * W * 8 + nnoise * (2 * W * 8 + sizeof(pq::NoiseDesc)) +
          nops * sizeof(pq::DInstr) + 3 * W * 8 + 4096;
+}
+
+// The queued request to start next under the policy, removed from `pending`,
+// or nullptr if the policy's choice does not fit the free budget.
+Req *pq_server_s::next_fitting(double now) {
+  if (pending.empty()) return nullptr;
+  auto pick = pending.begin();
+  if (policy == PQ_SERVE_SMALLEST_FIRST) {
+    const bool aged = age_limit > 0 && now - pending.front()->t_submit > age_limit;
+    if (!aged) {
+      pick = pending.end();
+      for (auto it = pending.begin(); it != pending.end(); ++it)
+        if (used + (*it)->bytes <= budget && (pick == pending.end() || (*it)->cost < (*pick)->cost))
+          pick = it;
+      if (pick == pending.end()) return nullptr;
+    }
+  }
+  if (used + (*pick)->bytes > budget) return nullptr;
+  Req *r = *pick;
+  pending.erase(pick);
+  return r;
 }
 
 int pq_server_s::admit(Req *r) {
@@ -126,11 +152,11 @@ void pq_server_s::loop() {
         running[i] = running.back();
         running.pop_back();
       }
+      const double t = now_s();
       for (size_t sl = 0; sl < busy.size() && !pending.empty(); ++sl) {
         if (busy[sl]) continue;
-        Req *r = pending.front();
-        if (used + r->bytes > budget) break;  // first come, first served
-        pending.pop_front();
+        Req *r = next_fitting(t);
+        if (!r) break;
         busy[sl] = 1;
         r->slot = (int)sl;
         used += r->bytes;
@@ -216,6 +242,7 @@ extern "C" int pq_server_submit(pq_server sv, uint64_t n, uint64_t B, int64_t se
   r->seed = seed;
   r->ops.assign(ops, ops + nops);
   r->bytes = request_bytes(n, B, ops, nops, &r->nrec);
+  r->cost = (double)r->W * (double)(2 * n + 1) * (double)(nops + 1);
   if (r->bytes > sv->budget)
     return pq_set_error(PQ_ERR_OOM, "request can never fit the server's memory budget");
   if (hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
@@ -290,6 +317,17 @@ extern "C" int pq_server_stats(pq_server sv, uint64_t *completed, uint64_t *peak
   if (peak_bytes) *peak_bytes = sv->peak;
   if (max_running) *max_running = sv->max_running;
   if (queued) *queued = sv->pending.size();
+  return PQ_OK;
+}
+
+extern "C" int pq_server_set_policy(pq_server sv, int policy, double age_limit_s) {
+  if (!sv) return pq_set_error(PQ_ERR_ARG, "null server");
+  if (policy != PQ_SERVE_FCFS && policy != PQ_SERVE_SMALLEST_FIRST)
+    return pq_set_error(PQ_ERR_ARG, "unknown serving policy");
+  if (!(age_limit_s >= 0)) return pq_set_error(PQ_ERR_ARG, "age limit must be >= 0");
+  std::lock_guard<std::mutex> g(sv->mu);
+  sv->policy = policy;
+  sv->age_limit = age_limit_s;
   return PQ_OK;
 }
 

@@ -48,11 +48,13 @@ def alone(ps, n, B, seed, ops):
     return [st.get_record(i) for i in range(st.num_records)]
 
 
-def test_server_results_equal_standalone(ps):
+@pytest.mark.parametrize("policy", ["fcfs", "smallest_first"])
+def test_server_results_equal_standalone(ps, policy):
     reqs = requests()
     biggest = max(((B + 63) // 64) * (2 * n + 1) ** 2 * 8 for n, B, _, _ in reqs)
     sv = ps.Server(budget_bytes=3 * biggest + (64 << 20), slots=4)
-    say(f"[case] server {len(reqs)} requests, budget {3 * biggest}")
+    sv.set_policy(policy, 0.05)
+    say(f"[case] server {len(reqs)} requests, budget {3 * biggest}, {policy}")
     tickets = [sv.submit(n, B, seed, ops) for n, B, seed, ops in reqs]
     sv.drain()
     stats = sv.stats()
@@ -68,6 +70,32 @@ def test_server_results_equal_standalone(ps):
         for i, (r, dd) in enumerate(ref):
             assert np.array_equal(res[i], r) and np.array_equal(det[i], dd), (t, i)
         sv.release(t)
+
+
+def test_server_smallest_first_order(ps):
+    """With one slot held by a running request, the smallest-first policy
+    starts the queued requests in increasing size."""
+    import time
+    from plaquette_simulator import workloads as W
+    big_ops, big_n = W.surface_code_memory_z(11, 11, 0.0)
+    ops, n = W.surface_code_memory_z(5, 5, 0.0)
+    sv = ps.Server(budget_bytes=1 << 36, slots=1)
+    sv.set_policy("smallest_first", 0.0)
+    blocker = sv.submit(big_n, 65536, 1, big_ops)
+    t0 = time.time()
+    while sv.status(blocker)[0] == 0 and time.time() - t0 < 30:
+        time.sleep(1e-4)
+    sizes = [4096, 2048, 64, 1024, 128]
+    tickets = [sv.submit(n, B, 3, ops) for B in sizes]
+    queued_behind = sv.status(blocker)[0] == 1
+    sv.drain()
+    starts = [sv.status(t)[2] for t in tickets]  # submits are microseconds apart
+    order = [sizes[i] for i in sorted(range(len(sizes)), key=lambda i: starts[i])]
+    say(f"[order] {order} (queued behind the blocker: {queued_behind})")
+    if queued_behind:
+        assert order == sorted(sizes)
+    with pytest.raises(RuntimeError):
+        sv.set_policy("lifo")
 
 
 def test_server_rejects_oversized(ps):
