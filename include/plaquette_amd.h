@@ -74,6 +74,21 @@ typedef struct pq_op {
   uint32_t code, q0, q1, slot, t0, t1, t2, reserved;
 } pq_op;
 
+/* ---- circuit text front end ----
+ * The line format of CliffordCircuit (CliffordCircuit.hpp:32-66): each line
+ * is a name (H S X Y Z CX CZ M) and qubit indices; H..Z give one op per
+ * qubit, CX/CZ one op per (control, target) pair, M one measurement per
+ * qubit (bias 1/2, MeasureQubits :543-549) into records first_record,
+ * first_record+1, ...  Any other name (a blank line's included) fails with
+ * "Unsupported operation: <name>" as the reference throws.  Host only.
+ * ops may be NULL (or capacity too small) to count: *nops = ops produced,
+ * *nlines = lines read (GetNumOperations, :127-129), *qubits_used = 1 + the
+ * largest index named.  Indices are checked against a state when the ops
+ * are run. */
+int pq_circuit_parse(const char *text, uint64_t len, uint32_t first_record, pq_op *ops,
+                     uint64_t capacity, uint64_t *nops, uint64_t *nlines,
+                     uint64_t *qubits_used);
+
 /* Execution forms of pq_run_program. */
 #define PQ_RUN_PLAIN 0      /* one kernel launch per op (reference style) */
 #define PQ_RUN_PERSISTENT 1 /* one persistent kernel for the whole op list */

@@ -397,6 +397,22 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
         return d;
       });
 
+  m.def(
+      "parse_circuit",
+      [](const std::string &text, uint32_t first_record) {
+        uint64_t nops = 0, nlines = 0, used = 0;
+        if (pq_circuit_parse(text.data(), text.size(), first_record, nullptr, 0, &nops, &nlines,
+                             &used) != PQ_OK)
+          throw std::runtime_error(pq_last_error());
+        py::array_t<uint32_t> a({(size_t)nops, (size_t)8});
+        if (pq_circuit_parse(text.data(), text.size(), first_record,
+                             reinterpret_cast<pq_op *>(a.mutable_data()), nops, &nops, &nlines,
+                             &used) != PQ_OK)
+          throw std::runtime_error(pq_last_error());
+        return py::make_tuple(a, nlines, used);
+      },
+      py::arg("text"), py::arg("first_record") = 0,
+      "Parse CliffordCircuit text into (ops uint32[nops, 8], lines, qubits used).");
   m.def("coin_threshold", &pq_coin_threshold);
   m.def("draw", &pq_draw);
   m.def("backend_name", &pq_backend_name);

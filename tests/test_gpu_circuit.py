@@ -1,0 +1,117 @@
+"""GPU parity of the circuit text front end: CliffordCircuit files run on the
+engine equal the oracle on the same parsed ops, bit-exact (full tableau and
+every record), for the reference's 59 benchmark ring circuits; the
+reference's CliffordCircuit tests (Test_CliffordCircuit.hpp:21-99); and the
+run_circuit timing tool (benchmarks/run_circuit.cpp)."""
+import json
+import os
+import subprocess
+import sys
+import zlib
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RING = json.load(open(os.path.join(ROOT, "tests", "golden", "ring_circuits.json")))["circuits"]
+TOOL = os.path.join(ROOT, "plaquette-simulator_amd", "run_circuit")
+
+
+@pytest.fixture(scope="module")
+def ps():
+    import plaquette_simulator as ps
+    import plaquette_simulator_bindings as b
+    if b.device_count() < 1:
+        pytest.skip("no GPU")
+    return ps
+
+
+def say(msg):
+    sys.stderr.write(msg + "\n")
+    sys.stderr.flush()
+
+
+def assert_same(st, orc):
+    x, z, r = st.device_to_host_numpy()
+    assert np.array_equal(x, orc.x) and np.array_equal(z, orc.z) and np.array_equal(r, orc.r)
+    assert st.num_records == len(orc.records)
+    for i, (res, det) in enumerate(orc.records):
+        gres, gdet = st.get_record(i)
+        assert np.array_equal(gres, res) and np.array_equal(gdet, det), f"record {i}"
+
+
+@pytest.mark.parametrize("form", ["persistent", "plain"])
+def test_ring_circuits_match_oracle(ps, form):
+    names = sorted(RING) if form == "persistent" else sorted(RING)[::6]
+    for name in names:
+        n = int(name.split("_")[2])
+        B, seed = 1000, zlib.crc32(name.encode()) & 0x7FFFFFFF
+        say(f"[case] {name} n={n} B={B} form={form}")
+        c = ps.CliffordCircuit.from_text(RING[name], n, B, seed)
+        c.execute(form)
+        orc = O.OracleState(n, B, seed)
+        orc.run_program(c.ops.astype(np.int32))
+        assert_same(c.get_clifford_state(), orc)
+
+
+def test_execute_twice_appends_records(ps):
+    name = "width_depth_6_9.circuit"
+    c = ps.CliffordCircuit.from_text(RING[name], 6, 300, 5)
+    c.execute()
+    first = c.get_clifford_state().num_records
+    c.execute("graph")
+    orc = O.OracleState(6, 300, 5)
+    ops = c.ops.astype(np.int32)
+    orc.run_program(ops)
+    ops2 = ops.copy()
+    ops2[ops2[:, 0] == 14, 3] += first
+    orc.run_program(ops2)
+    assert c.get_clifford_state().num_records == 2 * first
+    assert_same(c.get_clifford_state(), orc)
+
+
+def test_reference_circuit_tests(ps, tmp_path):
+    f = tmp_path / "testfile.txt"
+    f.write_text("")
+    c = ps.CliffordCircuit(str(f), 3, 1)
+    assert (c.get_num_qubits(), c.get_batch_size(), c.get_num_operations()) == (3, 1, 0)
+    c.execute()
+    # missing files read as empty circuits, as the reference's ifstream does
+    assert ps.CliffordCircuit(str(tmp_path / "absent.txt"), 3, 1).get_num_operations() == 0
+
+    f.write_text("H 0\n S 0\n S 0\n H 0\n M 0")
+    c = ps.CliffordCircuit(str(f), 3, 1)
+    c.execute()
+    assert c.get_num_operations() == 5
+    assert c.get_clifford_state().get_measurement(0, 0) == (1, 1)
+
+    # "M 0 1 2 3" on 3 qubits: the reference reads past the tableau; the
+    # engine rejects qubit 3 when the circuit runs
+    f.write_text("M 0 1 2 3\n")
+    c = ps.CliffordCircuit(str(f), 3, 1)
+    assert c.get_num_operations() == 1
+    with pytest.raises(RuntimeError):
+        c.execute()
+
+    f.write_text("H 0\nT 0\n")
+    with pytest.raises(RuntimeError, match="Unsupported operation: T"):
+        ps.CliffordCircuit(str(f), 3, 1)
+
+
+def test_run_circuit_tool(ps, tmp_path):
+    assert os.path.exists(TOOL), "build the engine first"
+    f = tmp_path / "width_depth_8_15.circuit"
+    f.write_text(RING["width_depth_8_15.circuit"])
+    for form in ("persistent", "plain"):
+        out = subprocess.run([TOOL, str(f), "8", "4096", form], capture_output=True, text=True,
+                             timeout=120)
+        say(f"[run_circuit {form}] {out.stdout.strip()} {out.stderr.strip()}")
+        assert out.returncode == 0, out.stderr
+        fname, secs = out.stdout.split()
+        assert fname == str(f) and float(secs) > 0
+    bad = subprocess.run([TOOL, str(f), "8"], capture_output=True, text=True, timeout=60)
+    assert bad.returncode == 1 and "Usage" in bad.stdout
