@@ -189,6 +189,9 @@ int pq_program_destroy(pq_program p);
 /* Logical gates (H..CZ), measurements, noise ops, device instructions. */
 int pq_program_info(pq_program p, uint64_t *gates, uint64_t *measurements,
                     uint64_t *noise_ops, uint64_t *instructions);
+/* Compiled layout: segments (runs of gates or of measurements) and the gate
+ * segments that run row-tiled out of LDS when the launch's LDS allows. */
+int pq_program_layout(pq_program p, uint64_t *segments, uint64_t *tiled_segments);
 /* Launch shape of the persistent kernel (0 keeps the current value). */
 int pq_program_set_launch(pq_program p, uint32_t threads, uint32_t blocks);
 /* Work census of persistent/graph runs (enable zeroes the counters):

@@ -250,6 +250,10 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
              d["measurements"] = mm;
              d["noise_ops"] = nz;
              d["instructions"] = ni;
+             uint64_t ns = 0, nt = 0;
+             check(pq_program_layout(p.p, &ns, &nt), "info");
+             d["segments"] = ns;
+             d["tiled_segments"] = nt;
              return d;
            })
       .def("set_launch", [](Program &p, uint32_t threads, uint32_t blocks) {

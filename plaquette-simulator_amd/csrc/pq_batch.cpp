@@ -186,11 +186,12 @@ static uint32_t launch_blocks(pq_batch b) {
       1, std::min<uint64_t>((uint64_t)cu_count() * per_cu, b->nitems));
 }
 
+// Dynamic LDS of the launch: the largest any member asks for (members'
+// arguments carry it, so row-tiled gate segments size their tiles to it).
 static size_t launch_lds(pq_batch b) {
   size_t lds = 0;
   for (uint32_t c = 0; c < b->members.size(); ++c)
-    lds = std::max<size_t>(lds, program_lds_bytes((uint32_t)b->members[c].first->n,
-                                                  b->h_circ[c].a.det_waves));
+    lds = std::max<size_t>(lds, b->h_circ[c].a.lds_words * 8);
   return lds;
 }
 
@@ -237,6 +238,8 @@ extern "C" int pq_batch_run(pq_batch b, int form) {
     s->noise_seq += p->nnoise;
     gens[c] = s->arena_gen;
   }
+  const uint64_t lds_words = launch_lds(b) / 8;
+  for (uint32_t c = 0; c < nc; ++c) b->h_circ[c].a.lds_words = lds_words;
   HIPCHK(hipMemcpyAsync(b->d_circ, b->h_circ, nc * sizeof(BatchCircuit),
                         hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipEventRecord(b->ev_upload, b->stream));

@@ -61,19 +61,32 @@ PQ_HD void add_mod4(uint64_t &a0, uint64_t &a1, uint64_t b0, uint64_t b1) {
 // Device-side instruction of a compiled program (32 bytes).
 //  gates: code, qa, qb; flagged ops: slot = record id;
 //  na / nb: noise plane fused after the op on qa / qb (kNone if none);
+//  t0 (gate segments): local column indices of qa (low 16 bits) and qb
+//  (high 16 bits) in the segment's qubit list;
+//  pad (gate segments): on the first op of a batch, the batch length (bits
+//  0-7) and kLayerStart if the batch opens a new layer (all ops of one layer
+//  act on pairwise distinct qubits); on the first op of a run -- a layer,
+//  or the part of a layer inside one staging chunk -- also the run length
+//  (bits 8-19);
 //  PQ_OP_NOISE alone: qa, na;  M: qa, slot = record id, t0 = coin
 //  threshold, (na, nb) = coin stream (low, high 32 bits).
 struct DInstr {
   uint32_t code, qa, qb, slot, na, nb, t0, pad;
 };
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr uint32_t kLayerStart = 0x80000000u;
+constexpr uint32_t kBatchLenMask = 0xFFu;
+constexpr uint32_t kRunShift = 8;
+constexpr uint32_t kRunMask = 0xFFFu;
 
 // Segment of a compiled program: a run of gate batches or a run of
-// measurements.  Gate segments hold `count` instructions in batches; batch
-// boundaries are flagged in DInstr::pad (bit 0 = first op of a batch).
+// measurements.  Gate segments hold `count` instructions in batches.  A gate
+// segment with nq > 0 may run row-tiled out of LDS: its nq distinct qubits
+// are listed at segq[qoff .. qoff+nq) in first-use order (the local column
+// indices of its instructions refer to this list).
 struct DSegment {
   uint32_t kind;  // 0 = gates, 1 = measurements
-  uint32_t begin, count, pad;
+  uint32_t begin, count, nq, qoff, pad0, pad1, pad2;
 };
 
 // Noise plane descriptor: stream and cumulative X/Y/Z thresholds.

@@ -579,7 +579,9 @@ __device__ __forceinline__ DSegment ld_segment(const DSegment *g, uint32_t i) {
   sg.kind = p[i].kind;
   sg.begin = p[i].begin;
   sg.count = p[i].count;
-  sg.pad = p[i].pad;
+  sg.nq = p[i].nq;
+  sg.qoff = p[i].qoff;
+  sg.pad0 = sg.pad1 = sg.pad2 = 0;
   return sg;
 }
 constexpr uint32_t kProgramMaxThreads = 512;  // block-size bound of k_program
@@ -594,44 +596,53 @@ constexpr uint32_t kMeasWindow0 = 8;  // first window of a run
 constexpr uint32_t kStage = 512;
 constexpr uint32_t kStageWords = kStage * (sizeof(DInstr) / 8 + 4);
 
+// Stages instructions [cb, cb+cn) of a gate segment and the per-word masks
+// they need (fused noise planes of qa and qb, or the flag of a flagged op)
+// in LDS.  No barrier.
+__device__ __forceinline__ void stage_chunk(const pq::ProgramArgs &a, uint64_t w,
+                                            uint32_t cb, uint32_t cn, DInstr *s_ins,
+                                            uint64_t *s_msk) {
+  const uint64_t W = a.W;
+  cinstr_ptr ins = (cinstr_ptr)a.instrs;
+  cword_ptr planes = (cword_ptr)a.planes;
+  const uint64_t *__restrict__ recs = a.recs;
+  for (uint32_t t = threadIdx.x; t < cn; t += blockDim.x) {
+    const DInstr I = ld_instr(ins, cb + t);
+    uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+    if (I.na != kNone) {
+      m0 = planes[(2ull * I.na) * W + w];
+      m1 = planes[(2ull * I.na + 1) * W + w];
+    }
+    if (I.nb != kNone) {
+      m2 = planes[(2ull * I.nb) * W + w];
+      m3 = planes[(2ull * I.nb + 1) * W + w];
+    } else if (I.code >= PQ_OP_H_FLAG && I.code != PQ_OP_NOISE) {
+      m2 = recs[(2ull * I.slot) * W + w];  // flag of a flagged op
+    }
+    s_ins[t] = I;
+    s_msk[4 * t + 0] = m0;
+    s_msk[4 * t + 1] = m1;
+    s_msk[4 * t + 2] = m2;
+    s_msk[4 * t + 3] = m3;
+  }
+}
+
 __device__ void run_gate_segment(uint64_t *__restrict__ T, uint64_t w,
                                  const pq::ProgramArgs &a, const DSegment &sg,
                                  DInstr *s_ins, uint64_t *s_msk) {
   const uint64_t Ru = a.R;
   const uint32_t n = a.n;
-  const uint64_t W = a.W;
-  cinstr_ptr ins = (cinstr_ptr)a.instrs;
-  cword_ptr planes = (cword_ptr)a.planes;
-  const uint64_t *__restrict__ recs = a.recs;
   const uint32_t seg_end = sg.begin + sg.count;
   for (uint32_t cb = sg.begin; cb < seg_end; cb += kStage) {
     const uint32_t cn = min(kStage, seg_end - cb);
-    for (uint32_t t = threadIdx.x; t < cn; t += blockDim.x) {
-      const DInstr I = ld_instr(ins, cb + t);
-      uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
-      if (I.na != kNone) {
-        m0 = planes[(2ull * I.na) * W + w];
-        m1 = planes[(2ull * I.na + 1) * W + w];
-      }
-      if (I.nb != kNone) {
-        m2 = planes[(2ull * I.nb) * W + w];
-        m3 = planes[(2ull * I.nb + 1) * W + w];
-      } else if (I.code >= PQ_OP_H_FLAG && I.code != PQ_OP_NOISE) {
-        m2 = recs[(2ull * I.slot) * W + w];  // flag of a flagged op
-      }
-      s_ins[t] = I;
-      s_msk[4 * t + 0] = m0;
-      s_msk[4 * t + 1] = m1;
-      s_msk[4 * t + 2] = m2;
-      s_msk[4 * t + 3] = m3;
-    }
+    stage_chunk(a, w, cb, cn, s_ins, s_msk);
     __syncthreads();
     for (uint32_t row = threadIdx.x; row < a.R; row += blockDim.x) {
       uint64_t *Tr = T + row;
       uint64_t r = Tr[(uint64_t)(2 * n) * Ru];
       uint32_t k = 0;
       while (k < cn) {
-        const uint32_t len = s_ins[k].pad;
+        const uint32_t len = s_ins[k].pad & kBatchLenMask;
         uint64_t xa[kBatch], za[kBatch], xb[kBatch], zb[kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
@@ -711,6 +722,357 @@ __device__ void run_gate_segment(uint64_t *__restrict__ T, uint64_t w,
         k += len;
       }
       Tr[(uint64_t)(2 * n) * Ru] = r;
+    }
+    __syncthreads();
+  }
+}
+
+#ifndef PQ_TILE_LOADS
+#define PQ_TILE_LOADS 4
+#endif
+#ifndef PQ_TILE_OPS
+#define PQ_TILE_OPS 2
+#endif
+#ifndef PQ_TILE_OPS_UNIFORM
+#define PQ_TILE_OPS_UNIFORM 1
+#endif
+
+// Rows of a row-tiled gate segment: the most, up to kTileMaxRows, whose
+// tile fits in the launch's LDS next to the staging area, the qubit list and
+// the tile's r words (0 if fewer than kTileMinRows fit: the direct form
+// runs).  The tile is [2 nq][TR] words, rounded up to whole 32-word DMA
+// chunks.
+constexpr uint32_t kTileMinRows = 8;
+constexpr uint32_t kTileMaxRows = 64;
+constexpr uint32_t kDmaWords = 32;  // words one wave-wide 4-byte LDS DMA moves
+constexpr int kTileLoads = PQ_TILE_LOADS;
+constexpr int kTileOps = PQ_TILE_OPS;
+constexpr int kTileOpsUniform = PQ_TILE_OPS_UNIFORM;
+
+__device__ __forceinline__ uint64_t tile_words(uint32_t nq, uint32_t tr) {
+  const uint64_t w = 2ull * nq * tr;
+  return (w + kDmaWords - 1) / kDmaWords * kDmaWords;
+}
+__device__ __forceinline__ uint32_t tile_rows(uint64_t lds_words, uint32_t nq) {
+  if (!nq) return 0;
+  const uint64_t fixed = kStageWords + (nq + 1) / 2;
+  if (lds_words <= fixed + kDmaWords) return 0;
+  const uint64_t tr = min<uint64_t>((lds_words - fixed - kDmaWords) / (2ull * nq + 1),
+                                    kTileMaxRows);
+  return tr >= kTileMinRows ? (uint32_t)tr : 0;
+}
+
+// Branch-free form of every gate-segment op, for lanes that may hold
+// different ops.  Per op code, 8 coefficient bits: A, Bs, Cxz, Cz, Cx (the
+// one-qubit update below), F (the op is flagged), CX, CZ.
+//   x' = x ^ (A & (x ^ z))          z' = z ^ (A & (x ^ z)) ^ (Bs & x)
+//   r ^= (Cxz & x & z) ^ (Cz & z) ^ (Cx & x)
+// where each coefficient is the shot flag for a flagged op and all ones
+// otherwise: H = A Cxz, S = Bs Cxz, X = Cz, Y = Cz Cx, Z = Cx; NOISE has
+// none (only its fused planes).
+enum : uint32_t {
+  kCoefA = 1, kCoefBs = 2, kCoefCxz = 4, kCoefCz = 8, kCoefCx = 16,
+  kCoefF = 32, kCoefCX = 64, kCoefCZ = 128
+};
+constexpr uint64_t coef_byte(uint32_t code) {
+  return code == PQ_OP_H ? (kCoefA | kCoefCxz)
+       : code == PQ_OP_S ? (kCoefBs | kCoefCxz)
+       : code == PQ_OP_X ? kCoefCz
+       : code == PQ_OP_Y ? (kCoefCz | kCoefCx)
+       : code == PQ_OP_Z ? kCoefCx
+       : code == PQ_OP_CX ? kCoefCX
+       : code == PQ_OP_CZ ? kCoefCZ
+       : code == PQ_OP_H_FLAG ? (kCoefA | kCoefCxz | kCoefF)
+       : code == PQ_OP_S_FLAG ? (kCoefBs | kCoefCxz | kCoefF)
+       : code == PQ_OP_X_FLAG ? (kCoefCz | kCoefF)
+       : code == PQ_OP_Y_FLAG ? (kCoefCz | kCoefCx | kCoefF)
+       : code == PQ_OP_Z_FLAG ? (kCoefCx | kCoefF)
+       : 0;
+}
+constexpr uint64_t coef_table(uint32_t c0) {
+  uint64_t t = 0;
+  for (uint32_t c = 0; c < 8; ++c) t |= coef_byte(c0 + c) << (8 * c);
+  return t;
+}
+constexpr uint64_t kCoefLo = coef_table(0), kCoefHi = coef_table(8);
+__device__ __forceinline__ uint64_t bitmask(uint32_t bits, uint32_t bit) {
+  return 0ull - (uint64_t)((bits >> bit) & 1u);
+}
+
+// Branch-free op on the row a lane holds: xa/za and xb/zb point at the
+// row's entries of the op's two columns in LDS (a one-qubit op names its own
+// column as b too) and x, z, xt, zt are their current values; m0..m3 are the
+// op's per-word masks (noise planes of a, noise planes of b or the flag).
+// Returns the phase contribution.
+__device__ __forceinline__ uint64_t tile_op_any(uint32_t code, uint64_t *pxa, uint64_t *pza,
+                                                uint64_t *pxb, uint64_t *pzb, uint64_t x,
+                                                uint64_t z, uint64_t xt, uint64_t zt,
+                                                uint64_t m0, uint64_t m1, uint64_t m2,
+                                                uint64_t m3) {
+  const uint32_t cf =
+      (uint32_t)(((code < 8 ? kCoefLo : kCoefHi) >> (8 * (code & 7))) & 0xFFu);
+  const uint64_t F = m2 | ~bitmask(cf, 5);  // the flag, or all ones
+  const uint64_t CX = bitmask(cf, 6), CZ = bitmask(cf, 7), two = CX | CZ;
+  const uint64_t d = bitmask(cf, 0) & F & (x ^ z);
+  const uint64_t x2 = x ^ d;
+  const uint64_t z2 = z ^ d ^ (bitmask(cf, 1) & F & x) ^ (CX & zt) ^ (CZ & xt);
+  const uint64_t xt2 = xt ^ (CX & x), zt2 = zt ^ (CZ & x);
+  uint64_t rp = F & ((bitmask(cf, 2) & x & z) ^ (bitmask(cf, 3) & z) ^ (bitmask(cf, 4) & x));
+  rp ^= (CX & x & zt & ~(xt ^ z)) ^ (CZ & x & xt & (zt ^ z));
+  rp ^= (m0 & z2) ^ (m1 & x2) ^ (two & ((m2 & zt2) ^ (m3 & xt2)));
+  *pxb = xt2;
+  *pzb = zt2;
+  *pxa = x2;
+  *pza = z2;
+  return rp;
+}
+
+// The same op when every lane of the wave holds it: `code` is wave-uniform,
+// so the switch compiles to scalar branches.  x, z (and xt, zt) are the
+// row's current values of the op's columns, already loaded.
+__device__ __forceinline__ uint64_t tile_op_uniform(uint32_t code, uint64_t *pxa, uint64_t *pza,
+                                                    uint64_t *pxb, uint64_t *pzb, uint64_t x,
+                                                    uint64_t z, uint64_t xt, uint64_t zt,
+                                                    uint64_t m0, uint64_t m1, uint64_t m2,
+                                                    uint64_t m3) {
+  uint64_t rp = 0;
+  const uint64_t f = m2;
+  switch (code) {
+    case PQ_OP_H:
+      rp = x & z;
+      *pxa = z; *pza = x;
+      { const uint64_t t = x; x = z; z = t; }
+      break;
+    case PQ_OP_S:
+      rp = x & z;
+      z ^= x;
+      *pza = z;
+      break;
+    case PQ_OP_X: rp = z; break;
+    case PQ_OP_Y: rp = z ^ x; break;
+    case PQ_OP_Z: rp = x; break;
+    case PQ_OP_CX:
+      rp = x & zt & ~(xt ^ z);
+      xt ^= x;
+      z ^= zt;
+      *pxb = xt;
+      *pza = z;
+      rp ^= (m2 & zt) ^ (m3 & xt);
+      break;
+    case PQ_OP_CZ:
+      rp = x & xt & (zt ^ z);
+      zt ^= x;
+      z ^= xt;
+      *pzb = zt;
+      *pza = z;
+      rp ^= (m2 & zt) ^ (m3 & xt);
+      break;
+    case PQ_OP_H_FLAG: {
+      const uint64_t d = f & (x ^ z);
+      rp = f & x & z;
+      x ^= d; z ^= d;
+      *pxa = x; *pza = z;
+      break;
+    }
+    case PQ_OP_S_FLAG:
+      rp = f & x & z;
+      z ^= f & x;
+      *pza = z;
+      break;
+    case PQ_OP_X_FLAG: rp = f & z; break;
+    case PQ_OP_Y_FLAG: rp = f & (z ^ x); break;
+    case PQ_OP_Z_FLAG: rp = f & x; break;
+    default: break;  // PQ_OP_NOISE: only the fused plane below
+  }
+  return rp ^ (m0 & z) ^ (m1 & x);
+}
+
+// Row-tiled gate segment.  Gates act on every row independently, so the
+// segment runs tile by tile over rows [r0, r0+TR): the x and z columns of
+// the segment's nq qubits (local column 2j = x, 2j+1 = z of qubit segq[j])
+// are loaded into LDS once, every op of the segment is applied there, and
+// the columns are written back once.  A wave holds P ops' worth of rows (one
+// op for TR >= 32, else 64 / TR): lane l works on row l % TR of op slot
+// wave * P + l / TR, and the
+// S = waves * P slots take ops slot, slot + S, ... of each run (a layer, or
+// its part in one staging chunk); ops of one layer touch distinct qubits, so
+// a run needs one barrier.  Phase contributions never feed back into x or
+// z, so each lane accumulates its own and they are XOR-combined into r at
+// the end of the tile.
+__device__ void run_gate_segment_tiled(uint64_t *__restrict__ T, uint64_t w,
+                                       const pq::ProgramArgs &a, const DSegment &sg,
+                                       uint64_t *dyn, uint32_t TR) {
+  const uint64_t Ru = a.R;
+  const uint32_t n = a.n;
+  const uint32_t nq = sg.nq;
+  DInstr *s_ins = reinterpret_cast<DInstr *>(dyn);
+  uint64_t *s_msk = dyn + kStage * (sizeof(DInstr) / 8);
+  uint32_t *s_q = reinterpret_cast<uint32_t *>(dyn + kStageWords);
+  uint64_t *s_r = dyn + kStageWords + (nq + 1) / 2;
+  uint64_t *tile = s_r + TR;
+  const uint32_t seg_end = sg.begin + sg.count;
+  const bool one_chunk = sg.count <= kStage;
+  for (uint32_t j = threadIdx.x; j < nq; j += blockDim.x) s_q[j] = a.segq[sg.qoff + j];
+  if (one_chunk) stage_chunk(a, w, sg.begin, sg.count, s_ins, s_msk);
+  const uint32_t lane = lane_id();
+  // tiles of 32+ rows give each wave one op (wave-uniform control flow)
+  const uint32_t P = TR >= 32 ? 1 : 64 / TR;
+  const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nwv = blockDim.x >> 6;
+  const uint32_t sub = lane / TR, ti = lane - sub * TR;
+  const uint32_t slot = (threadIdx.x >> 6) * P + sub;
+  const uint32_t S = (blockDim.x >> 6) * P;
+  const uint32_t ncol = 2 * nq;
+  __syncthreads();
+  for (uint32_t r0 = 0; r0 < a.R; r0 += TR) {
+    const uint32_t TRv = min(TR, a.R - r0);
+    const bool active = sub < P && ti < TRv;
+    // The tile arrives by LDS DMA: each wave-wide 4-byte global_load_lds
+    // fills one 32-word chunk of the tile (lane l carries half l&1 of word
+    // l/2), with no register round trip, so a wave keeps dozens of chunk
+    // loads in flight.  Rows past the tableau's last row repeat that row
+    // (they are never stored back).
+    {
+      const uint32_t nchunk = (uint32_t)(tile_words(nq, TR) / kDmaWords);
+      const uint32_t lrow = lane >> 1;  // word within the chunk
+      const uint32_t last = a.R - 1 - r0;
+      for (uint32_t ch = wv; ch < nchunk; ch += nwv) {
+        const uint32_t t = ch * kDmaWords + lrow;  // tile word of this lane
+        const uint32_t cc = min(t / TR, ncol - 1);
+        const uint32_t j = min(t % TR, last);
+        const uint32_t q = s_q[cc >> 1];
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(
+                                  T + (uint64_t)((cc & 1) ? n + q : q) * Ru + r0 + j) +
+                              (lane & 1);
+        __builtin_amdgcn_global_load_lds(
+            src, (__attribute__((address_space(3))) void *)(tile + (uint64_t)ch * kDmaWords), 4,
+            0, 0);
+      }
+      if (slot == 0 && active) s_r[ti] = T[(uint64_t)(2 * n) * Ru + r0 + ti];
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    }
+    __syncthreads();
+    uint64_t rp = 0;
+    uint64_t *Tl = tile + ti;
+    for (uint32_t cb = sg.begin; cb < seg_end; cb += kStage) {
+      const uint32_t cn = min(kStage, seg_end - cb);
+      if (!one_chunk) {
+        stage_chunk(a, w, cb, cn, s_ins, s_msk);
+        __syncthreads();
+      }
+      uint32_t k = 0;
+      while (k < cn) {
+        const uint32_t e = k + ((s_ins[k].pad >> kRunShift) & kRunMask);
+        {
+          // several ops per pass: their column reads are all issued before
+          // any write (ops of a run are disjoint), hiding LDS latency
+          if (P == 1) {
+            // One op per wave at a time, wave-uniform.  The wave's ops of
+            // the run (k + wv, k + wv + S, ...) are fetched in one LDS round
+            // trip, one per lane, then broadcast with readlane, so each op
+            // costs one round trip for its columns, overlapped kTileOps ways.
+            const uint32_t nmine = (e - k > wv) ? (e - k - wv + S - 1) / S : 0;
+            for (uint32_t base = 0; base < nmine; base += 64) {
+              uint32_t my_code = 0, my_t0 = 0;
+              if (base + lane < nmine) {
+                const uint32_t idx = k + wv + (base + lane) * S;
+                my_code = s_ins[idx].code;
+                my_t0 = s_ins[idx].t0;
+              }
+              const uint32_t cnt = min(64u, nmine - base);
+              for (uint32_t o = 0; active && o < cnt; o += kTileOpsUniform) {
+                uint32_t code[kTileOpsUniform];
+                uint64_t *pa[kTileOpsUniform], *pb[kTileOpsUniform];
+                uint64_t vx[kTileOpsUniform], vz[kTileOpsUniform], vxt[kTileOpsUniform],
+                    vzt[kTileOpsUniform];
+#pragma unroll
+                for (int u = 0; u < kTileOpsUniform; ++u) {
+                  code[u] = 0;
+                  if (o + u < cnt) {
+                    code[u] = __builtin_amdgcn_readlane(my_code, o + u);
+                    const uint32_t t0 = __builtin_amdgcn_readlane(my_t0, o + u);
+                    pa[u] = Tl + (uint64_t)(2 * (t0 & 0xFFFFu)) * TR;
+                    pb[u] = Tl + (uint64_t)(2 * (t0 >> 16)) * TR;
+                    vx[u] = pa[u][0];
+                    vz[u] = pa[u][TR];
+                    if (code[u] == PQ_OP_CX || code[u] == PQ_OP_CZ) {
+                      vxt[u] = pb[u][0];
+                      vzt[u] = pb[u][TR];
+                    }
+                  }
+                }
+#pragma unroll
+                for (int u = 0; u < kTileOpsUniform; ++u) {
+                  if (o + u < cnt) {
+                    const uint32_t ln = o + u;
+                    // the op's masks: one broadcast LDS read
+                    const uint64_t *M = s_msk + 4 * (k + wv + (base + ln) * S);
+                    rp ^= tile_op_uniform(code[u], pa[u], pa[u] + TR, pb[u], pb[u] + TR, vx[u],
+                                          vz[u], vxt[u], vzt[u], M[0], M[1], M[2], M[3]);
+                  }
+                }
+              }
+            }
+          } else if (active) {
+            for (uint32_t i0 = k + slot; i0 < e; i0 += kTileOps * S) {
+              uint32_t code[kTileOps], idxs[kTileOps];
+              uint64_t *pa[kTileOps], *pb[kTileOps];
+              uint64_t vx[kTileOps], vz[kTileOps], vxt[kTileOps], vzt[kTileOps];
+#pragma unroll
+              for (int u = 0; u < kTileOps; ++u) {
+                idxs[u] = i0 + u * S;
+                code[u] = 0;
+                if (idxs[u] < e) {
+                  code[u] = s_ins[idxs[u]].code;
+                  const uint32_t t0 = s_ins[idxs[u]].t0;
+                  pa[u] = Tl + (uint64_t)(2 * (t0 & 0xFFFFu)) * TR;
+                  pb[u] = Tl + (uint64_t)(2 * (t0 >> 16)) * TR;
+                  vx[u] = pa[u][0];
+                  vz[u] = pa[u][TR];
+                  vxt[u] = pb[u][0];
+                  vzt[u] = pb[u][TR];
+                }
+              }
+#pragma unroll
+              for (int u = 0; u < kTileOps; ++u) {
+                if (idxs[u] < e) {
+                  const uint64_t *M = s_msk + 4 * idxs[u];
+                  rp ^= tile_op_any(code[u], pa[u], pa[u] + TR, pb[u], pb[u] + TR, vx[u], vz[u],
+                                    vxt[u], vzt[u], M[0], M[1], M[2], M[3]);
+                }
+              }
+            }
+          }
+        }
+        __syncthreads();
+        k = e;
+      }
+    }
+    if (active && rp) atomicXor(reinterpret_cast<unsigned long long *>(s_r + ti), rp);
+    __syncthreads();
+    {
+      // write back: consecutive threads take consecutive tile words
+      const uint32_t nw = 2 * nq * TR;
+      for (uint32_t t0 = threadIdx.x; t0 < nw; t0 += kTileLoads * blockDim.x) {
+        uint64_t v[kTileLoads];
+#pragma unroll
+        for (int u = 0; u < kTileLoads; ++u) {
+          const uint32_t t = t0 + u * blockDim.x;
+          if (t < nw) v[u] = tile[t];
+        }
+#pragma unroll
+        for (int u = 0; u < kTileLoads; ++u) {
+          const uint32_t t = t0 + u * blockDim.x;
+          const uint32_t cc = t / TR, j = t % TR;
+          if (t < nw && j < TRv) {
+            const uint32_t q = s_q[cc >> 1];
+            T[(uint64_t)((cc & 1) ? n + q : q) * Ru + r0 + j] = v[u];
+          }
+        }
+      }
+    }
+    if (active) {
+      uint64_t *Tw = T + r0 + ti;
+      if (slot == 0) Tw[(uint64_t)(2 * n) * Ru] = s_r[ti];
     }
     __syncthreads();
   }
@@ -912,8 +1274,12 @@ __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t m
     const DSegment sg = ld_segment(a.segs, si);
     if (sg.kind == 0) {
       if (timing) t0 = wall_clock64();
-      run_gate_segment(T, w, a, sg, reinterpret_cast<DInstr *>(dyn),
-                       dyn + kStage * (sizeof(DInstr) / 8));
+      const uint32_t TR = tile_rows(a.lds_words, sg.nq);
+      if (TR)
+        run_gate_segment_tiled(T, w, a, sg, dyn, TR);
+      else
+        run_gate_segment(T, w, a, sg, reinterpret_cast<DInstr *>(dyn),
+                         dyn + kStage * (sizeof(DInstr) / 8));
       if (timing) tm->gate += wall_clock64() - t0;
       continue;
     }
@@ -1226,7 +1592,7 @@ hipError_t launch_record_to_int(hipStream_t s, const uint64_t *rec, uint64_t B,
 hipError_t launch_program(hipStream_t s, const ProgramArgs &a, uint32_t threads,
                           uint32_t blocks, const void *params) {
   if (!a.nwords) return hipSuccess;
-  const size_t lds = program_lds_bytes(a.n, a.det_waves);
+  const size_t lds = a.lds_words * 8;
   hipLaunchKernelGGL(k_program, dim3(blocks), dim3(threads), lds, s, a,
                      (const ProgParams *)params);
   return hipGetLastError();

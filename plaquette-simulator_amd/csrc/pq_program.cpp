@@ -62,6 +62,65 @@ int validate(uint64_t n, const pq_op *ops, uint64_t nops) {
 }  // namespace
 
 // Compile `ops` into segments/instructions (host side only).
+// Run lengths of a gate segment: a run starts at every layer start and at
+// every staging-chunk start and ends before the next one; its length goes
+// into bits 8-19 of its first instruction's pad.
+static void mark_runs(pq_program_s *p, const DSegment &sg) {
+  const uint32_t end = (uint32_t)p->instrs.size();
+  uint32_t start = sg.begin;
+  for (uint32_t k = sg.begin + 1; k <= end; ++k) {
+    const bool brk = k == end || (p->instrs[k].pad & kLayerStart) ||
+                     (k - sg.begin) % kProgramStage == 0;
+    if (brk) {
+      p->instrs[start].pad |= (k - start) << kRunShift;
+      start = k;
+    }
+  }
+}
+
+// Row tiling of a gate segment.  Gates act on each tableau row on its own,
+// so a tile of rows restricted to the columns the segment touches can be
+// loaded into LDS once, carry every op of the segment there, and be written
+// back once.  Per row that moves 4 words per touched qubit (x and z, in and
+// out); issuing the ops one by one against global memory moves the columns
+// each op reads and writes (H 4, S 3, CX/CZ 6, Paulis and noise 2).  The
+// segment is marked for tiling when that is at least 1.5x the tiled traffic;
+// the kernel still falls back to the direct form when too few rows fit.
+static void plan_tiling(pq_program_s *p, DSegment &sg) {
+  const uint32_t end = (uint32_t)p->instrs.size();
+  std::vector<int32_t> local(p->n, -1);
+  std::vector<uint32_t> qs;
+  uint64_t direct = 0;
+  auto loc = [&](uint32_t q) -> uint32_t {
+    if (local[q] < 0) {
+      local[q] = (int32_t)qs.size();
+      qs.push_back(q);
+    }
+    return (uint32_t)local[q];
+  };
+  for (uint32_t k = sg.begin; k < end; ++k) {
+    DInstr &I = p->instrs[k];
+    // one-qubit ops name their own column twice (the kernel's branch-free
+    // update then rewrites it with the final value last)
+    const uint32_t la = loc(I.qa);
+    const uint32_t lb = is_two(I.code) ? loc(I.qb) : la;
+    I.t0 = la | (lb << 16);
+    switch (I.code) {
+      case PQ_OP_H: case PQ_OP_H_FLAG: direct += 4; break;
+      case PQ_OP_S: case PQ_OP_S_FLAG: direct += 3; break;
+      case PQ_OP_CX: case PQ_OP_CZ: direct += 6; break;
+      default: direct += 2; break;
+    }
+  }
+  const uint64_t tiled = 4ull * qs.size() + 2;
+  if (qs.size() <= 0xFFFFu && 2 * direct >= 3 * tiled) {
+    sg.nq = (uint32_t)qs.size();
+    sg.qoff = (uint32_t)p->segq.size();
+    p->segq.insert(p->segq.end(), qs.begin(), qs.end());
+    p->tiled = true;
+  }
+}
+
 int compile_program(pq_program_s *p) {
   const pq_op *ops = p->ops.data();
   const uint64_t nops = p->ops.size();
@@ -71,7 +130,7 @@ int compile_program(pq_program_s *p) {
     const bool meas = ops[i].code == PQ_OP_M;
     uint64_t j = i;
     while (j < nops && (ops[j].code == PQ_OP_M) == meas) ++j;
-    DSegment sg{meas ? 1u : 0u, (uint32_t)p->instrs.size(), 0, 0};
+    DSegment sg{meas ? 1u : 0u, (uint32_t)p->instrs.size(), 0, 0, 0, 0, 0, 0};
     if (meas) {
       for (uint64_t k = i; k < j; ++k) {
         DInstr I{PQ_OP_M, ops[k].q0, 0, ops[k].slot, nmeas, 0, ops[k].t0, 0};
@@ -132,13 +191,15 @@ int compile_program(pq_program_s *p) {
               std::min<uint32_t>({kProgramBatch, e - b, kProgramStage - pos});
           for (uint32_t u = 0; u < len; ++u) {
             DInstr I = seg[order[b + u]];
-            I.pad = (u == 0) ? len : 0;
+            I.pad = (u == 0) ? (len | (b == k ? kLayerStart : 0u)) : 0u;
             p->instrs.push_back(I);
           }
           b += len;
         }
         k = e;
       }
+      mark_runs(p, sg);
+      plan_tiling(p, sg);
     }
     sg.count = (uint32_t)(p->instrs.size() - sg.begin);
     if (sg.count) p->segs.push_back(sg);
@@ -159,8 +220,8 @@ void pq_program_s::drop_graph() {
 pq_program_s::~pq_program_s() {
   drop_graph();
   for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-  for (void *q : {(void *)d_census, (void *)d_segs, (void *)d_instrs, (void *)d_noise,
-                  (void *)d_planes})
+  for (void *q : {(void *)d_census, (void *)d_segs, (void *)d_segq, (void *)d_instrs,
+                  (void *)d_noise, (void *)d_planes})
     dev_free(q, alloc_stream);
 }
 
@@ -191,6 +252,7 @@ static int create_program(pq_state s, const pq_op *ops, uint64_t nops, hipStream
   if (rc) return rc;
   if ((rc = upload(&p->d_segs, p->segs, as))) return rc;
   if ((rc = upload(&p->d_instrs, p->instrs, as))) return rc;
+  if ((rc = upload(&p->d_segq, p->segq, as))) return rc;
   if (p->nnoise) {
     if ((rc = upload(&p->d_noise, p->noise, as))) return rc;
     const uint64_t bytes = (uint64_t)p->nnoise * 2 * std::max<uint64_t>(p->W, 1) * 8;
@@ -228,6 +290,17 @@ extern "C" int pq_program_info(pq_program p, uint64_t *gates, uint64_t *measurem
   if (measurements) *measurements = p->nmeas;
   if (noise_ops) *noise_ops = p->nnoise;
   if (instructions) *instructions = p->instrs.size();
+  return PQ_OK;
+}
+
+extern "C" int pq_program_layout(pq_program p, uint64_t *segments, uint64_t *tiled_segments) {
+  if (!p) return pq_set_error(PQ_ERR_ARG, "null program");
+  if (segments) *segments = p->segs.size();
+  if (tiled_segments) {
+    uint64_t t = 0;
+    for (const DSegment &sg : p->segs) t += (sg.kind == 0 && sg.nq > 0);
+    *tiled_segments = t;
+  }
   return PQ_OK;
 }
 
@@ -311,6 +384,10 @@ ProgramArgs program_args(pq_state s, pq_program p, uint64_t lds_budget) {
   const uint64_t slab = 16ull * a.n;
   uint64_t ndw = (budget > base && slab) ? (budget - base) / slab : 0;
   a.det_waves = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(waves, ndw));
+  a.segq = p->d_segq;
+  // row-tiled gate segments use whatever LDS the budget allows
+  const uint64_t need = program_lds_bytes(a.n, a.det_waves);
+  a.lds_words = (p->tiled ? std::max<uint64_t>(need, budget) : need) / 8;
   return a;
 }
 

@@ -20,7 +20,10 @@ struct pq_program_s {
   std::vector<pq::DSegment> segs;
   std::vector<pq::DInstr> instrs;
   std::vector<pq::NoiseDesc> noise;
+  std::vector<uint32_t> segq;  // qubit lists of row-tiled gate segments
+  bool tiled = false;          // some gate segment runs row-tiled
   pq::DSegment *d_segs = nullptr;
+  uint32_t *d_segq = nullptr;
   pq::DInstr *d_instrs = nullptr;
   pq::NoiseDesc *d_noise = nullptr;
   uint64_t *d_planes = nullptr;  // noise planes [nnoise][2][W]

@@ -218,6 +218,28 @@ def test_random_programs(ps, W, form, seed, n, B):
     assert_same(st, orc)
 
 
+@pytest.mark.parametrize("form", ["persistent", "graph"])
+@pytest.mark.parametrize("seed,n,B,nops,pmeas", [(11, 5, 200, 1500, 0.001), (12, 40, 300, 2000, 0.01),
+                                                 (13, 64, 129, 1200, 0.004), (14, 200, 64, 1500, 0.003),
+                                                 (15, 600, 100, 2500, 0.002),
+                                                 (16, 300, 70, 2000, 0.002)])
+def test_long_gate_segments(ps, W, form, seed, n, B, nops, pmeas):
+    """Long runs of gates (flags, noise, CZ included) between measurements:
+    the compiler marks them row-tiled, segments over 512 instructions are
+    staged in several chunks, and n=300 and n=600 leave room for so few rows
+    per tile that one wave holds several ops (lanes with different ops)."""
+    rng = np.random.default_rng(seed)
+    ops = random_program(W, rng, n, nops, pflag=0.03, pmeas=pmeas, pnoise=0.1)
+    if not any(ops[:, 0] == W.OP_M):
+        ops = np.vstack([np.array([[W.OP_M, 0, 0, 0, O.coin_threshold(0.5), 0, 0, 0]], np.uint32), ops])
+    st0 = ps.CliffordState(n, B, seed, True)
+    info = ps.Program(st0, ops).info()
+    say(f"[layout] {info}")
+    assert info["tiled_segments"] >= 1
+    st, orc = run_both(ps, n, B, seed, ops, form)
+    assert_same(st, orc)
+
+
 def test_divergent_pivots(ps, W):
     """Per-shot flagged H makes X/Z differ across the 64 lanes of a word, so
     one measurement sees many pivot groups and mixed random/determined."""
