@@ -428,6 +428,11 @@ def main():
     if os.path.exists(traffic_file):
         tr = json.load(open(traffic_file))
         out["roofline"]["traffic"] = tr.get("bytes_per_launch")
+        # measured HBM bytes (rocprof, same kernel and workload) per launch time
+        if tr.get("bytes_per_launch"):
+            gbs = tr["bytes_per_launch"] / (kernel_ms / 1e3) / 1e9
+            out["roofline"]["traffic_gbs"] = gbs
+            out["roofline"]["traffic_frac"] = gbs / HBM_PEAK_GBS
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
     print(json.dumps(out))

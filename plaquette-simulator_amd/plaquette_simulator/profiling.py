@@ -68,6 +68,18 @@ def kernels(d, out):
             name[:60], r["Calls"], float(r["TotalDurationNs"]) / 1e6,
             float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3,
             float(r["MaxNs"]) / 1e3, float(r["Percentage"])))
+    # per-launch durations of the persistent kernel, in launch order
+    try:
+        trace = list(csv.DictReader(open(_find(d, "*kernel_trace.csv"))))
+    except (FileNotFoundError, StopIteration, IndexError):
+        trace = []
+    durs = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+            for r in sorted(trace, key=lambda r: int(r["Start_Timestamp"]))
+            if "k_program" in r["Kernel_Name"] and "noise" not in r["Kernel_Name"]]
+    if durs:
+        srt = sorted(durs)
+        lines += ["", "k_program launches in order (us): " + ", ".join("%.1f" % v for v in durs),
+                  "k_program median: %.1f us over %d launches" % (srt[len(srt) // 2], len(srt))]
     with open(out, "w") as f:
         f.write("\n".join(lines) + "\n")
     print("\n".join(lines))
