@@ -598,7 +598,9 @@ constexpr uint32_t kStageWords = kStage * (sizeof(DInstr) / 8 + 4);
 
 // Stages instructions [cb, cb+cn) of a gate segment and the per-word masks
 // they need (fused noise planes of qa and qb, or the flag of a flagged op)
-// in LDS.  No barrier.
+// in LDS; a staged instruction's `slot` (read only here) is replaced by the
+// kOp* flags below.  No barrier.
+constexpr uint32_t kOpNoisy = 1, kOpIdentity = 2;
 __device__ __forceinline__ void stage_chunk(const pq::ProgramArgs &a, uint64_t w,
                                             uint32_t cb, uint32_t cn, DInstr *s_ins,
                                             uint64_t *s_msk) {
@@ -619,7 +621,16 @@ __device__ __forceinline__ void stage_chunk(const pq::ProgramArgs &a, uint64_t w
     } else if (I.code >= PQ_OP_H_FLAG && I.code != PQ_OP_NOISE) {
       m2 = recs[(2ull * I.slot) * W + w];  // flag of a flagged op
     }
-    s_ins[t] = I;
+    // for the row-tiled path: whether the op's noise masks are non-zero on
+    // this word group, and whether the op is the identity here (a noise op
+    // drawing nothing, a flagged op whose flag is clear in all 64 shots)
+    const bool two = I.code == PQ_OP_CX || I.code == PQ_OP_CZ;
+    const bool flagged = I.code >= PQ_OP_H_FLAG && I.code != PQ_OP_NOISE;
+    const bool noisy = (m0 | m1) != 0 || (two && (m2 | m3) != 0);
+    const bool identity = !noisy && (I.code == PQ_OP_NOISE || (flagged && m2 == 0));
+    DInstr J = I;
+    J.slot = (noisy ? kOpNoisy : 0u) | (identity ? kOpIdentity : 0u);
+    s_ins[t] = J;
     s_msk[4 * t + 0] = m0;
     s_msk[4 * t + 1] = m1;
     s_msk[4 * t + 2] = m2;
@@ -727,37 +738,31 @@ __device__ void run_gate_segment(uint64_t *__restrict__ T, uint64_t w,
   }
 }
 
-#ifndef PQ_TILE_LOADS
-#define PQ_TILE_LOADS 4
-#endif
 #ifndef PQ_TILE_OPS
 #define PQ_TILE_OPS 2
 #endif
-#ifndef PQ_TILE_OPS_UNIFORM
-#define PQ_TILE_OPS_UNIFORM 1
-#endif
+
 
 // Rows of a row-tiled gate segment: the most, up to kTileMaxRows, whose
 // tile fits in the launch's LDS next to the staging area, the qubit list and
 // the tile's r words (0 if fewer than kTileMinRows fit: the direct form
 // runs).  The tile is [2 nq][TR] words, rounded up to whole 32-word DMA
-// chunks.
+// chunks; each tile column has TR + 1 words (a spare word per column).
 constexpr uint32_t kTileMinRows = 8;
 constexpr uint32_t kTileMaxRows = 64;
 constexpr uint32_t kDmaWords = 32;  // words one wave-wide 4-byte LDS DMA moves
-constexpr int kTileLoads = PQ_TILE_LOADS;
 constexpr int kTileOps = PQ_TILE_OPS;
-constexpr int kTileOpsUniform = PQ_TILE_OPS_UNIFORM;
 
-__device__ __forceinline__ uint64_t tile_words(uint32_t nq, uint32_t tr) {
-  const uint64_t w = 2ull * nq * tr;
+__device__ __forceinline__ uint64_t tile_words(uint32_t nq, uint32_t col_words) {
+  const uint64_t w = 2ull * nq * col_words;
   return (w + kDmaWords - 1) / kDmaWords * kDmaWords;
 }
 __device__ __forceinline__ uint32_t tile_rows(uint64_t lds_words, uint32_t nq) {
   if (!nq) return 0;
   const uint64_t fixed = kStageWords + (nq + 1) / 2;
-  if (lds_words <= fixed + kDmaWords) return 0;
-  const uint64_t tr = min<uint64_t>((lds_words - fixed - kDmaWords) / (2ull * nq + 1),
+  if (lds_words <= fixed + kDmaWords + 2ull * nq) return 0;
+  // TR r words + 2 nq (TR + 1) tile words + up to a chunk of rounding
+  const uint64_t tr = min<uint64_t>((lds_words - fixed - kDmaWords - 2ull * nq) / (2ull * nq + 1),
                                     kTileMaxRows);
   return tr >= kTileMinRows ? (uint32_t)tr : 0;
 }
@@ -804,8 +809,9 @@ __device__ __forceinline__ uint64_t bitmask(uint32_t bits, uint32_t bit) {
 // column as b too) and x, z, xt, zt are their current values; m0..m3 are the
 // op's per-word masks (noise planes of a, noise planes of b or the flag).
 // Returns the phase contribution.
-__device__ __forceinline__ uint64_t tile_op_any(uint32_t code, uint64_t *pxa, uint64_t *pza,
-                                                uint64_t *pxb, uint64_t *pzb, uint64_t x,
+template <class Ptr>
+__device__ __forceinline__ uint64_t tile_op_any(uint32_t code, Ptr pxa, Ptr pza,
+                                                Ptr pxb, Ptr pzb, uint64_t x,
                                                 uint64_t z, uint64_t xt, uint64_t zt,
                                                 uint64_t m0, uint64_t m1, uint64_t m2,
                                                 uint64_t m3) {
@@ -830,8 +836,9 @@ __device__ __forceinline__ uint64_t tile_op_any(uint32_t code, uint64_t *pxa, ui
 // The same op when every lane of the wave holds it: `code` is wave-uniform,
 // so the switch compiles to scalar branches.  x, z (and xt, zt) are the
 // row's current values of the op's columns, already loaded.
-__device__ __forceinline__ uint64_t tile_op_uniform(uint32_t code, uint64_t *pxa, uint64_t *pza,
-                                                    uint64_t *pxb, uint64_t *pzb, uint64_t x,
+template <bool kNoise, class Ptr>
+__device__ __forceinline__ uint64_t tile_op_uniform(uint32_t code, Ptr pxa, Ptr pza,
+                                                    Ptr pxb, Ptr pzb, uint64_t x,
                                                     uint64_t z, uint64_t xt, uint64_t zt,
                                                     uint64_t m0, uint64_t m1, uint64_t m2,
                                                     uint64_t m3) {
@@ -857,7 +864,7 @@ __device__ __forceinline__ uint64_t tile_op_uniform(uint32_t code, uint64_t *pxa
       z ^= zt;
       *pxb = xt;
       *pza = z;
-      rp ^= (m2 & zt) ^ (m3 & xt);
+      if (kNoise) rp ^= (m2 & zt) ^ (m3 & xt);
       break;
     case PQ_OP_CZ:
       rp = x & xt & (zt ^ z);
@@ -865,7 +872,7 @@ __device__ __forceinline__ uint64_t tile_op_uniform(uint32_t code, uint64_t *pxa
       z ^= xt;
       *pzb = zt;
       *pza = z;
-      rp ^= (m2 & zt) ^ (m3 & xt);
+      if (kNoise) rp ^= (m2 & zt) ^ (m3 & xt);
       break;
     case PQ_OP_H_FLAG: {
       const uint64_t d = f & (x ^ z);
@@ -884,32 +891,75 @@ __device__ __forceinline__ uint64_t tile_op_uniform(uint32_t code, uint64_t *pxa
     case PQ_OP_Z_FLAG: rp = f & x; break;
     default: break;  // PQ_OP_NOISE: only the fused plane below
   }
-  return rp ^ (m0 & z) ^ (m1 & x);
+  return kNoise ? rp ^ (m0 & z) ^ (m1 & x) : rp;
+}
+
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
+
+// A wave's share of one run, one op at a time on every lane's tile row (the
+// spare word for lanes without a row).  Op o of the share has code and flags
+// in lane o of my_code and local columns in lane o of my_t0; its masks are
+// at s_msk[4 * (i0 + o * S)].  CODE != 0 when every op of the share has that
+// code: the dispatch is then resolved at compile time.
+template <uint32_t CODE>
+__device__ __forceinline__ uint64_t tile_run_ops(lds_u64 *tl, uint32_t TRp, uint32_t lr,
+                                                 uint32_t my_code, uint32_t my_t0, uint32_t cnt,
+                                                 const uint64_t *s_msk, uint32_t i0,
+                                                 uint32_t S) {
+  uint64_t rp = 0;
+  for (uint32_t o = 0; o < cnt; ++o) {
+    const uint32_t cw = __builtin_amdgcn_readlane(my_code, o);
+    if (cw & (kOpIdentity << 16)) continue;  // nothing to do here
+    const uint32_t code = CODE ? CODE : (cw & 0xFFFFu);
+    const uint32_t t0 = __builtin_amdgcn_readlane(my_t0, o);
+    lds_u64 *pa = tl + (2 * (t0 & 0xFFFFu) * TRp + lr);
+    lds_u64 *pb = tl + (2 * (t0 >> 16) * TRp + lr);
+    const uint64_t vx = pa[0], vz = pa[TRp];
+    uint64_t vxt = 0, vzt = 0;
+    if (code == PQ_OP_CX || code == PQ_OP_CZ) {
+      vxt = pb[0];
+      vzt = pb[TRp];
+    }
+    const uint64_t *M = s_msk + 4 * (i0 + o * S);
+    if (cw & (kOpNoisy << 16)) {
+      rp ^= tile_op_uniform<true>(code, pa, pa + TRp, pb, pb + TRp, vx, vz, vxt, vzt, M[0], M[1],
+                                  M[2], M[3]);
+    } else {
+      // flagged ops still need their flag
+      const uint64_t f = (code >= PQ_OP_H_FLAG && code != PQ_OP_NOISE) ? M[2] : 0ull;
+      rp ^= tile_op_uniform<false>(code, pa, pa + TRp, pb, pb + TRp, vx, vz, vxt, vzt, 0ull,
+                                   0ull, f, 0ull);
+    }
+  }
+  return rp;
 }
 
 // Row-tiled gate segment.  Gates act on every row independently, so the
 // segment runs tile by tile over rows [r0, r0+TR): the x and z columns of
 // the segment's nq qubits (local column 2j = x, 2j+1 = z of qubit segq[j])
 // are loaded into LDS once, every op of the segment is applied there, and
-// the columns are written back once.  A wave holds P ops' worth of rows (one
+// the columns are written back once.  Each column holds TR + 1 words: TR
+// rows and a spare word that lanes without a row of this tile work on, so
+// the op loop needs no lane masking.  A wave holds P ops' worth of rows (one
 // op for TR >= 32, else 64 / TR): lane l works on row l % TR of op slot
-// wave * P + l / TR, and the
-// S = waves * P slots take ops slot, slot + S, ... of each run (a layer, or
-// its part in one staging chunk); ops of one layer touch distinct qubits, so
-// a run needs one barrier.  Phase contributions never feed back into x or
-// z, so each lane accumulates its own and they are XOR-combined into r at
-// the end of the tile.
+// wave * P + l / TR, and the S = waves * P slots take ops slot, slot + S,
+// ... of each run (a layer, or its part in one staging chunk); ops of one
+// layer touch distinct qubits, so a run needs one barrier.  Phase
+// contributions never feed back into x or z, so each lane accumulates its
+// own and they are XOR-combined into r at the end of the tile.
 __device__ void run_gate_segment_tiled(uint64_t *__restrict__ T, uint64_t w,
                                        const pq::ProgramArgs &a, const DSegment &sg,
                                        uint64_t *dyn, uint32_t TR) {
   const uint64_t Ru = a.R;
   const uint32_t n = a.n;
   const uint32_t nq = sg.nq;
+  const uint32_t TRp = TR + 1;  // words per tile column
   DInstr *s_ins = reinterpret_cast<DInstr *>(dyn);
   uint64_t *s_msk = dyn + kStage * (sizeof(DInstr) / 8);
   uint32_t *s_q = reinterpret_cast<uint32_t *>(dyn + kStageWords);
   uint64_t *s_r = dyn + kStageWords + (nq + 1) / 2;
   uint64_t *tile = s_r + TR;
+  lds_u64 *tl = (lds_u64 *)tile;
   const uint32_t seg_end = sg.begin + sg.count;
   const bool one_chunk = sg.count <= kStage;
   for (uint32_t j = threadIdx.x; j < nq; j += blockDim.x) s_q[j] = a.segq[sg.qoff + j];
@@ -919,40 +969,47 @@ __device__ void run_gate_segment_tiled(uint64_t *__restrict__ T, uint64_t w,
   const uint32_t P = TR >= 32 ? 1 : 64 / TR;
   const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nwv = blockDim.x >> 6;
   const uint32_t sub = lane / TR, ti = lane - sub * TR;
-  const uint32_t slot = (threadIdx.x >> 6) * P + sub;
-  const uint32_t S = (blockDim.x >> 6) * P;
+  const uint32_t slot = wv * P + sub;
+  const uint32_t S = nwv * P;
   const uint32_t ncol = 2 * nq;
+  const uint32_t nwords = ncol * TRp;  // tile words
   __syncthreads();
   for (uint32_t r0 = 0; r0 < a.R; r0 += TR) {
     const uint32_t TRv = min(TR, a.R - r0);
     const bool active = sub < P && ti < TRv;
+    const uint32_t lr = active ? ti : TR;  // this lane's word in a column
     // The tile arrives by LDS DMA: each wave-wide 4-byte global_load_lds
     // fills one 32-word chunk of the tile (lane l carries half l&1 of word
     // l/2), with no register round trip, so a wave keeps dozens of chunk
-    // loads in flight.  Rows past the tableau's last row repeat that row
-    // (they are never stored back).
+    // loads in flight.  Spare words and rows past the tableau's last row
+    // load that last row (they are never stored back).
     {
-      const uint32_t nchunk = (uint32_t)(tile_words(nq, TR) / kDmaWords);
-      const uint32_t lrow = lane >> 1;  // word within the chunk
+      const uint32_t nchunk = (uint32_t)(tile_words(nq, TRp) / kDmaWords);
       const uint32_t last = a.R - 1 - r0;
+      const uint32_t step = kDmaWords * nwv;  // tile words between a lane's chunks
+      const uint32_t dcc = step / TRp, dj = step - dcc * TRp;
+      uint32_t t = wv * kDmaWords + (lane >> 1);
+      uint32_t cc = t / TRp, j = t - cc * TRp;
       for (uint32_t ch = wv; ch < nchunk; ch += nwv) {
-        const uint32_t t = ch * kDmaWords + lrow;  // tile word of this lane
-        const uint32_t cc = min(t / TR, ncol - 1);
-        const uint32_t j = min(t % TR, last);
-        const uint32_t q = s_q[cc >> 1];
-        const uint32_t *src = reinterpret_cast<const uint32_t *>(
-                                  T + (uint64_t)((cc & 1) ? n + q : q) * Ru + r0 + j) +
-                              (lane & 1);
+        const uint32_t c = min(cc, ncol - 1);
+        const uint32_t q = s_q[c >> 1];
+        const uint32_t off = ((c & 1) ? n + q : q) * (uint32_t)Ru + r0 + min(j, last);
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(T + off) + (lane & 1);
         __builtin_amdgcn_global_load_lds(
             src, (__attribute__((address_space(3))) void *)(tile + (uint64_t)ch * kDmaWords), 4,
             0, 0);
+        cc += dcc;
+        j += dj;
+        if (j >= TRp) {
+          j -= TRp;
+          ++cc;
+        }
       }
       if (slot == 0 && active) s_r[ti] = T[(uint64_t)(2 * n) * Ru + r0 + ti];
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     }
     __syncthreads();
     uint64_t rp = 0;
-    uint64_t *Tl = tile + ti;
     for (uint32_t cb = sg.begin; cb < seg_end; cb += kStage) {
       const uint32_t cn = min(kStage, seg_end - cb);
       if (!one_chunk) {
@@ -962,83 +1019,69 @@ __device__ void run_gate_segment_tiled(uint64_t *__restrict__ T, uint64_t w,
       uint32_t k = 0;
       while (k < cn) {
         const uint32_t e = k + ((s_ins[k].pad >> kRunShift) & kRunMask);
-        {
-          // several ops per pass: their column reads are all issued before
-          // any write (ops of a run are disjoint), hiding LDS latency
-          if (P == 1) {
-            // One op per wave at a time, wave-uniform.  The wave's ops of
-            // the run (k + wv, k + wv + S, ...) are fetched in one LDS round
-            // trip, one per lane, then broadcast with readlane, so each op
-            // costs one round trip for its columns, overlapped kTileOps ways.
-            const uint32_t nmine = (e - k > wv) ? (e - k - wv + S - 1) / S : 0;
-            for (uint32_t base = 0; base < nmine; base += 64) {
-              uint32_t my_code = 0, my_t0 = 0;
-              if (base + lane < nmine) {
-                const uint32_t idx = k + wv + (base + lane) * S;
-                my_code = s_ins[idx].code;
-                my_t0 = s_ins[idx].t0;
-              }
-              const uint32_t cnt = min(64u, nmine - base);
-              for (uint32_t o = 0; active && o < cnt; o += kTileOpsUniform) {
-                uint32_t code[kTileOpsUniform];
-                uint64_t *pa[kTileOpsUniform], *pb[kTileOpsUniform];
-                uint64_t vx[kTileOpsUniform], vz[kTileOpsUniform], vxt[kTileOpsUniform],
-                    vzt[kTileOpsUniform];
+        if (P == 1) {
+          // One op per wave at a time, wave-uniform.  The wave's ops of the
+          // run (k + wv, k + wv + S, ...) are fetched in one LDS round trip,
+          // one per lane, then broadcast with readlane; ops that are the
+          // identity on this word group are skipped and ops whose noise drew
+          // nothing here skip the noise terms.  Every lane runs every op
+          // (lanes without a row on the spare word).
+          const uint32_t nmine = (e - k > wv) ? (e - k - wv + S - 1) / S : 0;
+          for (uint32_t base = 0; base < nmine; base += 64) {
+            uint32_t my_code = 0, my_t0 = 0;
+            if (base + lane < nmine) {
+              const uint32_t idx = k + wv + (base + lane) * S;
+              my_code = s_ins[idx].code | (s_ins[idx].slot << 16);
+              my_t0 = s_ins[idx].t0;
+            }
+            const uint32_t cnt = min(64u, nmine - base);
+            const uint32_t i0 = k + wv + base * S;
+            // one code for the whole share (the usual case: a layer of one
+            // gate kind): dispatch once, outside the op loop
+            const uint32_t c0 = __builtin_amdgcn_readlane(my_code, 0) & 0xFFFFu;
+            const bool uni = __ballot(lane < cnt && (my_code & 0xFFFFu) != c0) == 0;
+            if (uni && c0 == PQ_OP_CX)
+              rp ^= tile_run_ops<PQ_OP_CX>(tl, TRp, lr, my_code, my_t0, cnt, s_msk, i0, S);
+            else if (uni && c0 == PQ_OP_H)
+              rp ^= tile_run_ops<PQ_OP_H>(tl, TRp, lr, my_code, my_t0, cnt, s_msk, i0, S);
+            else if (uni && c0 == PQ_OP_CZ)
+              rp ^= tile_run_ops<PQ_OP_CZ>(tl, TRp, lr, my_code, my_t0, cnt, s_msk, i0, S);
+            else if (uni && c0 == PQ_OP_S)
+              rp ^= tile_run_ops<PQ_OP_S>(tl, TRp, lr, my_code, my_t0, cnt, s_msk, i0, S);
+            else if (uni && c0 == PQ_OP_X_FLAG)
+              rp ^= tile_run_ops<PQ_OP_X_FLAG>(tl, TRp, lr, my_code, my_t0, cnt, s_msk, i0, S);
+            else if (uni && c0 == PQ_OP_NOISE)
+              rp ^= tile_run_ops<PQ_OP_NOISE>(tl, TRp, lr, my_code, my_t0, cnt, s_msk, i0, S);
+            else
+              rp ^= tile_run_ops<0>(tl, TRp, lr, my_code, my_t0, cnt, s_msk, i0, S);
+          }
+        } else if (active) {
+          lds_u64 *Tl = tl + ti;
+          for (uint32_t i0 = k + slot; i0 < e; i0 += kTileOps * S) {
+            uint32_t code[kTileOps], idxs[kTileOps];
+            lds_u64 *pa[kTileOps], *pb[kTileOps];
+            uint64_t vx[kTileOps], vz[kTileOps], vxt[kTileOps], vzt[kTileOps];
 #pragma unroll
-                for (int u = 0; u < kTileOpsUniform; ++u) {
-                  code[u] = 0;
-                  if (o + u < cnt) {
-                    code[u] = __builtin_amdgcn_readlane(my_code, o + u);
-                    const uint32_t t0 = __builtin_amdgcn_readlane(my_t0, o + u);
-                    pa[u] = Tl + (uint64_t)(2 * (t0 & 0xFFFFu)) * TR;
-                    pb[u] = Tl + (uint64_t)(2 * (t0 >> 16)) * TR;
-                    vx[u] = pa[u][0];
-                    vz[u] = pa[u][TR];
-                    if (code[u] == PQ_OP_CX || code[u] == PQ_OP_CZ) {
-                      vxt[u] = pb[u][0];
-                      vzt[u] = pb[u][TR];
-                    }
-                  }
-                }
-#pragma unroll
-                for (int u = 0; u < kTileOpsUniform; ++u) {
-                  if (o + u < cnt) {
-                    const uint32_t ln = o + u;
-                    // the op's masks: one broadcast LDS read
-                    const uint64_t *M = s_msk + 4 * (k + wv + (base + ln) * S);
-                    rp ^= tile_op_uniform(code[u], pa[u], pa[u] + TR, pb[u], pb[u] + TR, vx[u],
-                                          vz[u], vxt[u], vzt[u], M[0], M[1], M[2], M[3]);
-                  }
-                }
+            for (int u = 0; u < kTileOps; ++u) {
+              idxs[u] = i0 + u * S;
+              code[u] = 0;
+              if (idxs[u] < e) {
+                code[u] = s_ins[idxs[u]].code;
+                const uint32_t t0 = s_ins[idxs[u]].t0;
+                pa[u] = Tl + 2 * (t0 & 0xFFFFu) * TRp;
+                pb[u] = Tl + 2 * (t0 >> 16) * TRp;
+                vx[u] = pa[u][0];
+                vz[u] = pa[u][TRp];
+                vxt[u] = pb[u][0];
+                vzt[u] = pb[u][TRp];
               }
             }
-          } else if (active) {
-            for (uint32_t i0 = k + slot; i0 < e; i0 += kTileOps * S) {
-              uint32_t code[kTileOps], idxs[kTileOps];
-              uint64_t *pa[kTileOps], *pb[kTileOps];
-              uint64_t vx[kTileOps], vz[kTileOps], vxt[kTileOps], vzt[kTileOps];
 #pragma unroll
-              for (int u = 0; u < kTileOps; ++u) {
-                idxs[u] = i0 + u * S;
-                code[u] = 0;
-                if (idxs[u] < e) {
-                  code[u] = s_ins[idxs[u]].code;
-                  const uint32_t t0 = s_ins[idxs[u]].t0;
-                  pa[u] = Tl + (uint64_t)(2 * (t0 & 0xFFFFu)) * TR;
-                  pb[u] = Tl + (uint64_t)(2 * (t0 >> 16)) * TR;
-                  vx[u] = pa[u][0];
-                  vz[u] = pa[u][TR];
-                  vxt[u] = pb[u][0];
-                  vzt[u] = pb[u][TR];
-                }
-              }
-#pragma unroll
-              for (int u = 0; u < kTileOps; ++u) {
-                if (idxs[u] < e) {
-                  const uint64_t *M = s_msk + 4 * idxs[u];
-                  rp ^= tile_op_any(code[u], pa[u], pa[u] + TR, pb[u], pb[u] + TR, vx[u], vz[u],
-                                    vxt[u], vzt[u], M[0], M[1], M[2], M[3]);
-                }
+            for (int u = 0; u < kTileOps; ++u) {
+              if (idxs[u] < e) {
+                const uint64_t *M = s_msk + 4 * idxs[u];
+                rp ^= tile_op_any(code[u], pa[u], pa[u] + TRp, pb[u], pb[u] + TRp, vx[u], vz[u],
+                                  vxt[u], vzt[u], M[0], M[1], M[2], M[3]);
               }
             }
           }
@@ -1051,28 +1094,22 @@ __device__ void run_gate_segment_tiled(uint64_t *__restrict__ T, uint64_t w,
     __syncthreads();
     {
       // write back: consecutive threads take consecutive tile words
-      const uint32_t nw = 2 * nq * TR;
-      for (uint32_t t0 = threadIdx.x; t0 < nw; t0 += kTileLoads * blockDim.x) {
-        uint64_t v[kTileLoads];
-#pragma unroll
-        for (int u = 0; u < kTileLoads; ++u) {
-          const uint32_t t = t0 + u * blockDim.x;
-          if (t < nw) v[u] = tile[t];
+      const uint32_t step = blockDim.x;
+      const uint32_t dcc = step / TRp, dj = step - dcc * TRp;
+      uint32_t cc = threadIdx.x / TRp, j = threadIdx.x - cc * TRp;
+      for (uint32_t t = threadIdx.x; t < nwords; t += step) {
+        if (j < TRv) {
+          const uint32_t q = s_q[cc >> 1];
+          T[((cc & 1) ? n + q : q) * (uint32_t)Ru + r0 + j] = tl[t];
         }
-#pragma unroll
-        for (int u = 0; u < kTileLoads; ++u) {
-          const uint32_t t = t0 + u * blockDim.x;
-          const uint32_t cc = t / TR, j = t % TR;
-          if (t < nw && j < TRv) {
-            const uint32_t q = s_q[cc >> 1];
-            T[(uint64_t)((cc & 1) ? n + q : q) * Ru + r0 + j] = v[u];
-          }
+        cc += dcc;
+        j += dj;
+        if (j >= TRp) {
+          j -= TRp;
+          ++cc;
         }
       }
-    }
-    if (active) {
-      uint64_t *Tw = T + r0 + ti;
-      if (slot == 0) Tw[(uint64_t)(2 * n) * Ru] = s_r[ti];
+      if (slot == 0 && active) T[(uint64_t)(2 * n) * Ru + r0 + ti] = s_r[ti];
     }
     __syncthreads();
   }
