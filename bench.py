@@ -298,6 +298,90 @@ def bench_serving(args):
     print(json.dumps(out))
 
 
+def cpu_baseline_prefix(w, target_s, shots=64):
+    """The CPU oracle on `shots` shots and a prefix of the op list, the prefix
+    scaled from a 1000-op calibration run to about `target_s` seconds (the
+    ops are uniform random, so a prefix has the circuit's mix)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    O.set_threads(cores)
+    ops32 = w.ops.astype(np.int32)
+    cal = O.OracleState(w.num_qubits, shots, w.seed)
+    k0 = min(1000, len(ops32))
+    t = time.perf_counter()
+    cal.run_program(ops32[:k0])
+    t0 = time.perf_counter() - t
+    k = int(min(len(ops32), max(k0, k0 * target_s / max(t0, 1e-3))))
+    st = O.OracleState(w.num_qubits, shots, w.seed)
+    t = time.perf_counter()
+    st.run_program(ops32[:k])
+    dt = time.perf_counter() - t
+    codes = ops32[:k, 0]
+    gates = int(np.isin(codes, (1, 2, 3, 4, 5, 6, 7)).sum())
+    meas = int((codes == 14).sum())
+    return {"value": gates * shots / dt, "unit": "gate-shots/s", "cores": O.get_threads(),
+            "kind": "port",
+            "sample": f"{w.name}: first {k} ops ({gates} gates, {meas} measurements) on "
+                      f"shots [0,{shots}), {dt:.2f} s, int32 [B][2n+1][n] layout",
+            "meas_shots_per_s": meas * shots / dt}
+
+
+def bench_random_n1024(args):
+    """Random Clifford circuit on 1024 qubits, 4096 shots, 100000 ops (10%
+    measurements of uniform qubits, else H/S/CX on uniform qubits), seed 42:
+    the measurement-bound shape.  One step = reset + whole circuit in the
+    persistent form; census run first for the algorithmic bytes."""
+    import numpy as np
+    import plaquette_simulator as ps
+    from plaquette_simulator import workloads as W
+
+    w = W.config_random_n1024()
+    n, B = w.num_qubits, w.batch_size
+    words = (B + 63) // 64
+    st = ps.CliffordState(n, B, w.seed, True)
+    prog = ps.Program(st, w.ops)
+    prog.set_census(True)
+    st.reset_clifford_state()
+    prog.run(st, "persistent")
+    census = prog.read_census()
+    prog.set_census(False)
+    abytes = algorithmic_bytes(w.ops, n, words, census)
+    steps = max(1, min(args.steps, 3))
+    prog.kernel_ms()
+    prog.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st.reset_clifford_state()
+        prog.run(st, "persistent")
+    st.synchronize()
+    wall = time.perf_counter() - t0
+    kms, launches = prog.kernel_ms()
+    kernel_ms = kms / max(1, launches)
+    ms_step = 1e3 * wall / steps
+    achieved = abytes["total"] / (kernel_ms / 1e3) / 1e9
+    out = {
+        "metric": "gate-shots/s and measurement-shots/s, random Clifford circuit n=1024",
+        "value": w.gates * B / (ms_step / 1e3),
+        "unit": "gate-shots/s",
+        "n_gpus": 1, "steps": steps, "warmup": 0, "ms_per_step": ms_step,
+        "higher_is_better": True, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "random_clifford_n1024", "qubits": n, "shots": B,
+                   "ops": int(len(w.ops)), "gates": w.gates, "measurements": w.measurements,
+                   "seed": w.seed, "word_groups": words},
+        "meas_shots_per_s": w.measurements * B / (ms_step / 1e3),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_program",
+                     "kernel_ms": kernel_ms, "algorithmic_bytes": abytes["total"]},
+        "algorithmic_bytes": abytes,
+        "census": census,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_prefix(w, args.cpu_seconds)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,6 +395,8 @@ def main():
     ap.add_argument("--mixed", action="store_true",
                     help="measure the 256-circuit mixed batch instead of the d=9 workload")
     ap.add_argument("--mixed-count", type=int, default=256)
+    ap.add_argument("--random-n1024", action="store_true",
+                    help="measure the n=1024 random Clifford circuit instead")
     ap.add_argument("--serving", action="store_true",
                     help="measure the serving loop under Poisson arrivals instead")
     ap.add_argument("--serving-requests", type=int, default=100)
@@ -330,6 +416,9 @@ def main():
         return
     if args.serving:
         bench_serving(args)
+        return
+    if args.random_n1024:
+        bench_random_n1024(args)
         return
 
     import plaquette_simulator as ps

@@ -126,7 +126,9 @@ extern "C" int pq_batch_set_launch(pq_batch b, uint32_t threads, uint32_t blocks
   if (!b) return pq_set_error(PQ_ERR_ARG, "null batch");
   if (threads) {
     if (threads % 64 || threads > kProgramThreadsMax)
-      return pq_set_error(PQ_ERR_ARG, "threads must be a multiple of 64, at most 512");
+      return pq_set_error(PQ_ERR_ARG,
+                          "threads must be a multiple of 64, at most 768 (512 when a member "
+                          "has more than 192 qubits: larger blocks are clamped)");
     b->threads = threads;
   }
   b->blocks = blocks;
@@ -200,8 +202,11 @@ static int enqueue(pq_batch b) {
   uint32_t noise_blocks = 0;
   for (uint32_t c = 0; c < nc; ++c)
     if (b->h_circ[c].nnoise) noise_blocks = b->noise_blocks;
+  uint32_t max_n = 0;
+  for (uint32_t c = 0; c < nc; ++c) max_n = std::max<uint32_t>(max_n, b->h_circ[c].a.n);
   return launch_batch(b->stream, b->d_circ, nc, b->d_items, b->nitems, b->d_counter,
-                      b->threads, launch_blocks(b), launch_lds(b), noise_blocks) == hipSuccess
+                      b->threads, launch_blocks(b), launch_lds(b), noise_blocks,
+                      max_n) == hipSuccess
              ? PQ_OK
              : pq_set_error(PQ_ERR_HIP, "batch launch failed");
 }
@@ -335,7 +340,7 @@ extern "C" int pq_batch_autotune(pq_batch b, int reps, uint32_t *threads_out,
   }
   std::vector<std::pair<uint64_t, uint64_t>> saved;
   for (auto &m : b->members) saved.emplace_back(m.first->meas_seq, m.first->noise_seq);
-  const uint32_t cand_threads[] = {256, 512};
+  const uint32_t cand_threads[] = {256, 512, 768};
   const uint64_t cand_lds[] = {kProgramLdsBytes, 72 * 1024, 48 * 1024};
   float best = 1e30f;
   uint32_t bt = b->threads;

@@ -584,7 +584,9 @@ __device__ __forceinline__ DSegment ld_segment(const DSegment *g, uint32_t i) {
   sg.pad0 = sg.pad1 = sg.pad2 = 0;
   return sg;
 }
-constexpr uint32_t kProgramMaxThreads = 512;  // block-size bound of k_program
+constexpr uint32_t kSmallQubits = 192;         // the small build's state size bound
+constexpr int kSmallRegs = 3, kSmallThreads = 768;
+constexpr int kLargeRegs = 8, kLargeThreads = 512;
 constexpr uint32_t kMeasBatch = 64;   // most measurements pivot-checked per pass
 constexpr uint32_t kMeasWindow0 = 8;  // first window of a run
 
@@ -1128,11 +1130,11 @@ __device__ __forceinline__ uint64_t wave_or(uint64_t v) {
 // that step's mod-4 phase across the wave.  Only the last measurement of a
 // batch writes its scratch row back to the tableau.
 //
-// n <= 64*kDetRegs: each lane owns at most kDetRegs columns, the scratch row
-// lives in registers and the next step's source words are loaded while the
+// n <= 64*NREG: each lane owns at most NREG columns, the scratch row lives
+// in registers and the next step's source words are loaded while the
 // current step is folded in.  Larger n: scratch row in the wave's LDS slab.
-constexpr int kDetRegs = 8;
-
+// NREG sets the kernel's register footprint (see k_program_t).
+template <int NREG>
 __device__ void determined_wave(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
                                 uint32_t R, uint32_t q, uint64_t *slab,
                                 uint64_t *rec, uint64_t W, bool write_scratch,
@@ -1144,7 +1146,7 @@ __device__ void determined_wave(uint64_t *__restrict__ T, uint64_t w, uint32_t n
   const uint64_t *rq = T + scratch * Ru;
   uint64_t racc = 0;
   uint32_t steps = 0;
-  if (n <= 64u * kDetRegs) {
+  if (n <= 64u * NREG) {
     // chain list into the wave's slab: rows in slab[0..], masks in slab[n..]
     uint32_t count = 0;
     for (uint32_t base = 0; base < n; base += 4 * 64) {
@@ -1166,15 +1168,15 @@ __device__ void determined_wave(uint64_t *__restrict__ T, uint64_t w, uint32_t n
       }
     }
     steps = count;
-    uint64_t sx[kDetRegs], sz[kDetRegs], nx[kDetRegs], nz[kDetRegs];
+    uint64_t sx[NREG], sz[NREG], nx[NREG], nz[NREG];
 #pragma unroll
-    for (int u = 0; u < kDetRegs; ++u) sx[u] = sz[u] = nx[u] = nz[u] = 0;
+    for (int u = 0; u < NREG; ++u) sx[u] = sz[u] = nx[u] = nz[u] = 0;
     uint64_t nm = 0, nr = 0;
     if (count) {
       const uint64_t s = n + slab[0];
       nm = slab[n];
 #pragma unroll
-      for (int u = 0; u < kDetRegs; ++u) {
+      for (int u = 0; u < NREG; ++u) {
         const uint32_t j = lane + 64 * u;
         if (j < n) {
           nx[u] = T[(uint64_t)j * Ru + s];
@@ -1184,9 +1186,9 @@ __device__ void determined_wave(uint64_t *__restrict__ T, uint64_t w, uint32_t n
       nr = rq[s];
     }
     for (uint32_t t = 0; t < count; ++t) {
-      uint64_t cx[kDetRegs], cz[kDetRegs];
+      uint64_t cx[NREG], cz[NREG];
 #pragma unroll
-      for (int u = 0; u < kDetRegs; ++u) {
+      for (int u = 0; u < NREG; ++u) {
         cx[u] = nx[u];
         cz[u] = nz[u];
       }
@@ -1195,7 +1197,7 @@ __device__ void determined_wave(uint64_t *__restrict__ T, uint64_t w, uint32_t n
         const uint64_t s = n + slab[t + 1];
         nm = slab[n + t + 1];
 #pragma unroll
-        for (int u = 0; u < kDetRegs; ++u) {
+        for (int u = 0; u < NREG; ++u) {
           const uint32_t j = lane + 64 * u;
           if (j < n) {
             nx[u] = T[(uint64_t)j * Ru + s];
@@ -1206,7 +1208,7 @@ __device__ void determined_wave(uint64_t *__restrict__ T, uint64_t w, uint32_t n
       }
       uint64_t c0 = 0, c1 = 0;
 #pragma unroll
-      for (int u = 0; u < kDetRegs; ++u) {
+      for (int u = 0; u < NREG; ++u) {
         phase_acc(sx[u], sz[u], cx[u], cz[u], c0, c1);
         sx[u] ^= m & cx[u];
         sz[u] ^= m & cz[u];
@@ -1221,7 +1223,7 @@ __device__ void determined_wave(uint64_t *__restrict__ T, uint64_t w, uint32_t n
     }
     if (write_scratch) {
 #pragma unroll
-      for (int u = 0; u < kDetRegs; ++u) {
+      for (int u = 0; u < NREG; ++u) {
         const uint32_t j = lane + 64 * u;
         if (j < n) {
           T[(uint64_t)j * Ru + scratch] = sx[u];
@@ -1292,6 +1294,7 @@ struct PhaseTimes {
 };
 
 // The whole compiled program on 64-shot word group w of one state.
+template <int NREG>
 __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t mseq0,
                              uint64_t w, uint64_t *dyn, MeasShared &sh,
                              PhaseTimes *tm) {
@@ -1303,7 +1306,9 @@ __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t m
   const uint64_t C = 2ull * n + 1;
   const uint64_t Ru = a.R;
   const uint32_t lane = lane_id(), wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  const uint32_t ndw = a.det_waves;
+  // waves running determined measurements: as many as the arguments size
+  // LDS slabs for, but no more than the launched block has
+  const uint32_t ndw = min(a.det_waves, nw);
   const bool timing = tm != nullptr;
   uint64_t t0 = 0;
   uint64_t *T = a.tab + w * C * a.R;
@@ -1364,8 +1369,8 @@ __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t m
           for (uint32_t idx = wv; idx < jr; idx += ndw) {
             const DInstr I = ld_instr((cinstr_ptr)a.instrs, k + idx);
             uint64_t *rec = a.recs + (2ull * I.slot) * a.W;
-            determined_wave(T, w, n, a.R, I.qa, slab, rec, a.W, idx == jr - 1,
-                            a.census);
+            determined_wave<NREG>(T, w, n, a.R, I.qa, slab, rec, a.W, idx == jr - 1,
+                                  a.census);
           }
         }
         __syncthreads();
@@ -1392,8 +1397,13 @@ __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t m
   }
 }
 
-__global__ __launch_bounds__(kProgramMaxThreads) void k_program(pq::ProgramArgs a,
-                                                  const ProgParams *__restrict__ pp) {
+// Two builds of the persistent kernels: up to 192 qubits, determined chains
+// keep 3 columns per lane in registers, which leaves room for 768-thread
+// workgroups (12 waves); larger states keep 8 per lane (n <= 512) in
+// 512-thread workgroups.
+template <int NREG, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_program_t(pq::ProgramArgs a,
+                                                   const ProgParams *__restrict__ pp) {
   extern __shared__ uint64_t dyn[];
   __shared__ MeasShared sh;
   const uint64_t seed = pp->seed, mseq0 = pp->mseq0;
@@ -1401,7 +1411,7 @@ __global__ __launch_bounds__(kProgramMaxThreads) void k_program(pq::ProgramArgs 
   PhaseTimes times;
   PhaseTimes *tm = (a.census != nullptr && threadIdx.x == 0) ? &times : nullptr;
   for (uint64_t wi = blockIdx.x; wi < a.nwords; wi += gridDim.x)
-    program_word(a, seed, mseq0, a.word0 + wi, dyn, sh, tm);
+    program_word<NREG>(a, seed, mseq0, a.word0 + wi, dyn, sh, tm);
   if (tm) {
     atomicAdd(a.census + 4, (unsigned long long)times.gate);
     atomicAdd(a.census + 5, (unsigned long long)times.check);
@@ -1414,7 +1424,8 @@ __global__ __launch_bounds__(kProgramMaxThreads) void k_program(pq::ProgramArgs 
 // persistent launch.  Work items (state, word group), heaviest first, are
 // taken from a global counter until the list is exhausted; every workgroup
 // leaves the loop once the counter passes the item count.
-__global__ __launch_bounds__(kProgramMaxThreads) void k_batch(
+template <int NREG, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_batch_t(
     const pq::BatchCircuit *__restrict__ circ, const uint64_t *__restrict__ items,
     uint64_t nitems, unsigned long long *counter) {
   extern __shared__ uint64_t dyn[];
@@ -1430,7 +1441,7 @@ __global__ __launch_bounds__(kProgramMaxThreads) void k_batch(
     const uint32_t c = (uint32_t)(code >> 40);
     const uint64_t w = code & ((1ull << 40) - 1);
     const pq::BatchCircuit &bc = circ[c];
-    program_word(bc.a, bc.seed, bc.mseq0, w, dyn, sh, nullptr);
+    program_word<NREG>(bc.a, bc.seed, bc.mseq0, w, dyn, sh, nullptr);
   }
 }
 
@@ -1630,9 +1641,19 @@ hipError_t launch_program(hipStream_t s, const ProgramArgs &a, uint32_t threads,
                           uint32_t blocks, const void *params) {
   if (!a.nwords) return hipSuccess;
   const size_t lds = a.lds_words * 8;
-  hipLaunchKernelGGL(k_program, dim3(blocks), dim3(threads), lds, s, a,
-                     (const ProgParams *)params);
+  if (a.n <= kSmallQubits)
+    hipLaunchKernelGGL((k_program_t<kSmallRegs, kSmallThreads>), dim3(blocks),
+                       dim3(std::min<uint32_t>(threads, kSmallThreads)), lds, s, a,
+                       (const ProgParams *)params);
+  else
+    hipLaunchKernelGGL((k_program_t<kLargeRegs, kLargeThreads>), dim3(blocks),
+                       dim3(std::min<uint32_t>(threads, kLargeThreads)), lds, s, a,
+                       (const ProgParams *)params);
   return hipGetLastError();
+}
+
+uint32_t program_max_threads(uint32_t n) {
+  return n <= kSmallQubits ? kSmallThreads : kLargeThreads;
 }
 
 size_t program_lds_bytes(uint32_t n, uint32_t det_waves) {
@@ -1669,14 +1690,17 @@ hipError_t launch_set_params(hipStream_t s, void *params, uint64_t seed,
 hipError_t prepare_kernels(uint32_t n) {
   hipError_t e = ensure_lds_attr((const void *)k_measure, measure_lds_bytes(n));
   if (e == hipSuccess)
-    e = ensure_lds_attr((const void *)k_program, kProgramLdsBudget);
+    e = ensure_lds_attr((const void *)k_program_t<kSmallRegs, kSmallThreads>, kProgramLdsBudget);
+  if (e == hipSuccess)
+    e = ensure_lds_attr((const void *)k_program_t<kLargeRegs, kLargeThreads>, kProgramLdsBudget);
   return e;
 }
 
 hipError_t launch_batch(hipStream_t s, const BatchCircuit *circ, uint32_t ncirc,
                         const uint64_t *items, uint64_t nitems,
                         unsigned long long *counter, uint32_t threads,
-                        uint32_t blocks, size_t lds_bytes, uint32_t noise_blocks) {
+                        uint32_t blocks, size_t lds_bytes, uint32_t noise_blocks,
+                        uint32_t max_qubits) {
   if (!nitems) return hipSuccess;
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, s, counter);
   hipError_t e = hipGetLastError();
@@ -1686,13 +1710,23 @@ hipError_t launch_batch(hipStream_t s, const BatchCircuit *circ, uint32_t ncirc,
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(threads), lds_bytes, s, circ, items,
-                     nitems, counter);
+  if (max_qubits <= kSmallQubits)
+    hipLaunchKernelGGL((k_batch_t<kSmallRegs, kSmallThreads>), dim3(blocks),
+                       dim3(std::min<uint32_t>(threads, kSmallThreads)), lds_bytes, s, circ,
+                       items, nitems, counter);
+  else
+    hipLaunchKernelGGL((k_batch_t<kLargeRegs, kLargeThreads>), dim3(blocks),
+                       dim3(std::min<uint32_t>(threads, kLargeThreads)), lds_bytes, s, circ,
+                       items, nitems, counter);
   return hipGetLastError();
 }
 
 hipError_t prepare_batch_kernel() {
-  return ensure_lds_attr((const void *)k_batch, kProgramLdsBudget);
+  hipError_t e =
+      ensure_lds_attr((const void *)k_batch_t<kSmallRegs, kSmallThreads>, kProgramLdsBudget);
+  if (e == hipSuccess)
+    e = ensure_lds_attr((const void *)k_batch_t<kLargeRegs, kLargeThreads>, kProgramLdsBudget);
+  return e;
 }
 
 }  // namespace pq

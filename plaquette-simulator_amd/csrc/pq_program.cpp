@@ -262,7 +262,7 @@ static int create_program(pq_state s, const pq_op *ops, uint64_t nops, hipStream
     }
   }
   // the measurement phases use every wave, so the default is the largest block
-  p->threads = kProgramThreadsMax;
+  p->threads = program_max_threads((uint32_t)s->n);
   p->blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(s->W, 1u << 20));
   if (!as) HIPCHK(prepare_kernels((uint32_t)s->n));
   *out = p.release();
@@ -307,8 +307,10 @@ extern "C" int pq_program_layout(pq_program p, uint64_t *segments, uint64_t *til
 extern "C" int pq_program_set_launch(pq_program p, uint32_t threads, uint32_t blocks) {
   if (!p) return pq_set_error(PQ_ERR_ARG, "null program");
   if (threads) {
-    if (threads % 64 || threads > kProgramThreadsMax)
-      return pq_set_error(PQ_ERR_ARG, "threads must be a multiple of 64, at most 512");
+    if (threads % 64 || threads > program_max_threads((uint32_t)p->n))
+      return pq_set_error(PQ_ERR_ARG, ("threads must be a multiple of 64, at most " +
+                                       std::to_string(program_max_threads((uint32_t)p->n)) +
+                                       " for this state size").c_str());
     p->threads = threads;
   }
   if (blocks) p->blocks = blocks;

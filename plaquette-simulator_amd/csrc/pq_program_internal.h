@@ -10,7 +10,7 @@
 #include "pq_common.h"
 
 constexpr uint32_t kProgramBatch = 8;  // must equal kBatch in pq_kernels.hip
-constexpr uint32_t kProgramThreadsMax = 512;  // must equal kProgramMaxThreads there
+constexpr uint32_t kProgramThreadsMax = 768;  // largest block of the persistent kernel builds
 constexpr uint32_t kProgramStage = 512;       // must equal kStage there
 
 struct pq_program_s {
