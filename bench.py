@@ -24,6 +24,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "plaquette-simulator_amd"))
 
 HBM_PEAK_GBS = 8000.0
+
+
+def progress(msg):
+    """One progress line on stderr (long modes print one per phase)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 METRIC = "gate-shots/s on d=9 surface code (%HBM peak); batched vs plain speedup"
 
 # bytes per op per (row x word group), unfused, at bit-plane density
@@ -108,6 +113,7 @@ def bench_mixed(args):
         states.append(st)
         progs.append(pr)
     setup_s = time.perf_counter() - t0
+    progress(f"mixed: {len(ws)} circuits built in {setup_s:.1f} s")
     gate_shots = sum(w.gates * w.batch_size for w in ws)
     meas_shots = sum(w.measurements * w.batch_size for w in ws)
 
@@ -122,14 +128,17 @@ def bench_mixed(args):
         abytes += algorithmic_bytes(w.ops, w.num_qubits, (w.batch_size + 63) // 64, c)["total"]
     for st in states:
         st.synchronize()
+    progress("mixed: census done")
     t = time.perf_counter()
     for st, pr in zip(states, progs):
         st.reset_clifford_state()
         pr.run(st, "persistent")
         st.synchronize()
     seq_ms = 1e3 * (time.perf_counter() - t)
+    progress(f"mixed: one launch per circuit {seq_ms:.0f} ms")
 
     choice = batch.autotune(1)
+    progress(f"mixed: autotune {choice}")
     batch.run("graph")
     for _ in range(args.warmup):
         for st in states:
@@ -138,11 +147,13 @@ def bench_mixed(args):
     batch.set_timing(True)
     kms = []
     t = time.perf_counter()
-    for _ in range(args.steps):
+    for i in range(args.steps):
         for st in states:
             st.reset_clifford_state()
         batch.run("graph")
         kms.append(batch.last_ms())
+        if i % 5 == 4:
+            progress(f"mixed: step {i + 1}/{args.steps}")
     wall = time.perf_counter() - t
     ms_step = 1e3 * wall / args.steps
     kernel_ms = float(np.mean(kms))
@@ -347,6 +358,7 @@ def bench_random_n1024(args):
     prog.run(st, "persistent")
     census = prog.read_census()
     prog.set_census(False)
+    progress("n1024: census run done")
     abytes = algorithmic_bytes(w.ops, n, words, census)
     steps = max(1, min(args.steps, 3))
     prog.kernel_ms()
