@@ -1445,26 +1445,32 @@ __global__ __launch_bounds__(MAXT) void k_batch_t(
   }
 }
 
-// Noise planes of every circuit of a batch (blockIdx.y = circuit).
+// Noise planes of every circuit of a batch (blockIdx.y = circuit).  The
+// block row's waves are laid out as (noise-op lane) x (word-group lane): one
+// stream key per op and wave, one draw per lane per (op, word group).
 __global__ __launch_bounds__(256) void k_batch_noise(const pq::BatchCircuit *__restrict__ circ) {
   const pq::BatchCircuit &bc = circ[blockIdx.y];
   const uint64_t W = bc.a.W;
-  const uint64_t pairs = (uint64_t)bc.nnoise * W;
-  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
+  const uint64_t nwv = (uint64_t)gridDim.x * (blockDim.x / 64);
   const uint64_t wave = blockIdx.x * (uint64_t)(blockDim.x / 64) + (threadIdx.x >> 6);
+  // waves along word groups (about 16 groups each, so one stream key serves
+  // many draws) and along noise ops
+  const uint64_t ww = W / 16 < 1 ? 1 : (W / 16 < nwv ? W / 16 : nwv);
+  const uint64_t kstride = nwv / ww;
+  if (!W || wave >= kstride * ww) return;
   const uint32_t lane = lane_id();
   uint64_t *planes = const_cast<uint64_t *>(bc.a.planes);
-  for (uint64_t p = wave; p < pairs; p += waves) {
-    const uint64_t k = p / W;
-    const uint64_t w = p - k * W;
+  for (uint64_t k = wave / ww; k < bc.nnoise; k += kstride) {
     const NoiseDesc d = bc.noise[k];
     const uint64_t key = stream_key(bc.seed, 2 * (bc.nseq0 + k) + 1);
-    const uint32_t u = (uint32_t)(draw_keyed(key, 64 * w + lane) >> 32);
-    const uint64_t mx = __ballot(u < d.t1);
-    const uint64_t mz = __ballot((u >= d.t0) & (u < d.t2));
-    if (lane == 0) {
-      planes[(2 * k) * W + w] = mx;
-      planes[(2 * k + 1) * W + w] = mz;
+    for (uint64_t w = wave % ww; w < W; w += ww) {
+      const uint32_t u = (uint32_t)(draw_keyed(key, 64 * w + lane) >> 32);
+      const uint64_t mx = __ballot(u < d.t1);
+      const uint64_t mz = __ballot((u >= d.t0) & (u < d.t2));
+      if (lane == 0) {
+        planes[(2 * k) * W + w] = mx;
+        planes[(2 * k + 1) * W + w] = mz;
+      }
     }
   }
 }
@@ -1477,21 +1483,24 @@ __global__ void k_zero_counter(unsigned long long *counter) {
 __global__ __launch_bounds__(256) void k_program_noise(
     uint64_t *__restrict__ planes, const NoiseDesc *__restrict__ desc,
     uint32_t nk, uint64_t W, const ProgParams *__restrict__ pp) {
-  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
-  const uint64_t wave = blockIdx.x * (uint64_t)(blockDim.x / 64) + (threadIdx.x >> 6);
+  // blockIdx.y strides over noise ops (one stream key per op), the waves of
+  // blockIdx.x over word groups: one draw per lane per (op, word group)
   const uint32_t lane = lane_id();
   const uint64_t seed = pp->seed, nseq0 = pp->nseq0;
-  for (uint64_t p = wave; p < (uint64_t)nk * W; p += waves) {
-    const uint64_t k = p / W;
-    const uint64_t w = p - k * W;
+  const uint64_t wv = blockIdx.x * (uint64_t)(blockDim.x / 64) + (threadIdx.x >> 6);
+  const uint64_t nwv = (uint64_t)gridDim.x * (blockDim.x / 64);
+  for (uint32_t k = blockIdx.y; k < nk; k += gridDim.y) {
     const NoiseDesc d = desc[k];
     const uint64_t key = stream_key(seed, 2 * (nseq0 + k) + 1);
-    const uint32_t u = (uint32_t)(draw_keyed(key, 64 * w + lane) >> 32);
-    const uint64_t mx = __ballot(u < d.t1);
-    const uint64_t mz = __ballot((u >= d.t0) & (u < d.t2));
-    if (lane == 0) {
-      planes[(2 * k) * W + w] = mx;
-      planes[(2 * k + 1) * W + w] = mz;
+    uint64_t *px = planes + (2ull * k) * W, *pz = px + W;
+    for (uint64_t w = wv; w < W; w += nwv) {
+      const uint32_t u = (uint32_t)(draw_keyed(key, 64 * w + lane) >> 32);
+      const uint64_t mx = __ballot(u < d.t1);
+      const uint64_t mz = __ballot((u >= d.t0) & (u < d.t2));
+      if (lane == 0) {
+        px[w] = mx;
+        pz[w] = mz;
+      }
     }
   }
 }
@@ -1666,8 +1675,11 @@ hipError_t launch_program_noise(hipStream_t s, uint64_t *planes,
                                 const void *params) {
   const uint64_t items = (uint64_t)nk * W;
   if (!items) return hipSuccess;
-  hipLaunchKernelGGL(k_program_noise, dim3(grid_for(items * 64, 256, 16384)),
-                     dim3(256), 0, s, planes, desc, nk, W,
+  // about 64 word groups per wave, so the stream key (a full SplitMix64
+  // round of 64-bit multiplies) is amortized over many draws
+  const uint32_t gx = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(W / 256, 64));
+  const uint32_t gy = std::min<uint32_t>(nk, 65535);
+  hipLaunchKernelGGL(k_program_noise, dim3(gx, gy), dim3(256), 0, s, planes, desc, nk, W,
                      (const ProgParams *)params);
   return hipGetLastError();
 }
