@@ -1406,17 +1406,31 @@ __global__ __launch_bounds__(MAXT) void k_program_t(pq::ProgramArgs a,
                                                    const ProgParams *__restrict__ pp) {
   extern __shared__ uint64_t dyn[];
   __shared__ MeasShared sh;
+  __shared__ unsigned long long s_census[4];
   const uint64_t seed = pp->seed, mseq0 = pp->mseq0;
-  // census mode also times the phases (thread 0, wall-clock ticks)
+  // census mode: work counts gather in LDS (one global add per workgroup at
+  // the end, so counting does not contend) and thread 0 times the phases
+  // (wall-clock ticks)
+  unsigned long long *census = a.census;
+  pq::ProgramArgs la = a;
+  if (census) {
+    if (threadIdx.x < 4) s_census[threadIdx.x] = 0;
+    __syncthreads();
+    la.census = s_census;
+  }
   PhaseTimes times;
-  PhaseTimes *tm = (a.census != nullptr && threadIdx.x == 0) ? &times : nullptr;
+  PhaseTimes *tm = (census != nullptr && threadIdx.x == 0) ? &times : nullptr;
   for (uint64_t wi = blockIdx.x; wi < a.nwords; wi += gridDim.x)
-    program_word<NREG>(a, seed, mseq0, a.word0 + wi, dyn, sh, tm);
+    program_word<NREG>(la, seed, mseq0, a.word0 + wi, dyn, sh, tm);
+  if (census) {
+    __syncthreads();
+    if (threadIdx.x < 4) atomicAdd(census + threadIdx.x, s_census[threadIdx.x]);
+  }
   if (tm) {
-    atomicAdd(a.census + 4, (unsigned long long)times.gate);
-    atomicAdd(a.census + 5, (unsigned long long)times.check);
-    atomicAdd(a.census + 6, (unsigned long long)times.det);
-    atomicAdd(a.census + 7, (unsigned long long)times.gen);
+    atomicAdd(census + 4, (unsigned long long)times.gate);
+    atomicAdd(census + 5, (unsigned long long)times.check);
+    atomicAdd(census + 6, (unsigned long long)times.det);
+    atomicAdd(census + 7, (unsigned long long)times.gen);
   }
 }
 
