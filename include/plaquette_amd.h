@@ -229,7 +229,7 @@ int pq_batch_run(pq_batch b, int form);
 int pq_batch_wait(pq_batch b);
 /* Launch shape: threads per workgroup (multiple of 64, <= 512), workgroups
  * (0 = CUs x workgroups per CU) and the dynamic-LDS budget per workgroup
- * (40 KB..150 KB; it sets workgroups per CU and parallel determined waves). */
+ * (40 KB..158 KB; it sets workgroups per CU and parallel determined waves). */
 int pq_batch_set_launch(pq_batch b, uint32_t threads, uint32_t blocks,
                         uint64_t lds_budget);
 /* Times candidate launch shapes on the batch and keeps the fastest (cached

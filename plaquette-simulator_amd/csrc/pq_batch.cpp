@@ -134,7 +134,7 @@ extern "C" int pq_batch_set_launch(pq_batch b, uint32_t threads, uint32_t blocks
   b->blocks = blocks;
   if (lds_budget) {
     if (lds_budget < 40 * 1024 || lds_budget > kProgramLdsBytes)
-      return pq_set_error(PQ_ERR_ARG, "LDS budget must be in [40 KB, 150 KB]");
+      return pq_set_error(PQ_ERR_ARG, "LDS budget must be in [40 KB, 158 KB]");
     b->lds_budget = lds_budget;
   }
   b->drop_graph();

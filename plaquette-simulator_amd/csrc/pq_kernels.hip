@@ -1594,7 +1594,7 @@ size_t measure_lds_bytes(uint32_t n) {
 
 uint32_t measure_max_qubits() { return kMeasMaxQubits; }
 
-constexpr size_t kProgramLdsBudget = 150 * 1024;  // dynamic LDS cap of k_program
+constexpr size_t kProgramLdsBudget = 158 * 1024;  // dynamic LDS cap of k_program (160 KB less static LDS)
 
 static hipError_t ensure_lds_attr(const void *fn, size_t bytes) {
   if (bytes <= 65536) return hipSuccess;

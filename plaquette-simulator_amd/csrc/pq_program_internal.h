@@ -44,7 +44,7 @@ struct pq_program_s {
 
 int compile_program(pq_program_s *p);
 
-constexpr uint64_t kProgramLdsBytes = 150 * 1024;  // dynamic-LDS cap (kProgramLdsBudget there)
+constexpr uint64_t kProgramLdsBytes = 158 * 1024;  // dynamic-LDS cap (kProgramLdsBudget there)
 
 // Shared with the batch runner.
 int prepare_records(pq_state s, pq_program p);
