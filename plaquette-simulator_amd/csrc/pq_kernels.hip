@@ -1350,12 +1350,10 @@ __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t m
         if (lane == 0) s_rm[idx] = acc;
       }
       __syncthreads();
-      uint32_t jr = nm;
-      for (uint32_t idx = 0; idx < nm; ++idx)
-        if (s_rm[idx]) {
-          jr = idx;
-          break;
-        }
+      // first measurement with a pivot: one LDS read per lane and a ballot
+      // (nm <= kMeasBatch = 64 = lanes)
+      const uint64_t piv = __ballot(lane < nm && s_rm[lane] != 0);
+      const uint32_t jr = piv ? (uint32_t)__ffsll((long long)piv) - 1 : nm;
       win = (jr == nm) ? min(2 * win, kMeasBatch) : kMeasWindow0;
       if (timing) {
         const uint64_t t1 = wall_clock64();
