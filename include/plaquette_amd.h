@@ -57,11 +57,23 @@ extern "C" {
 #define PQ_OP_Y_FLAG 12 /* Y on those shots */
 #define PQ_OP_Z_FLAG 13 /* Z on those shots */
 #define PQ_OP_M 14      /* MeasureQubit(q0) into record `slot`       :587 */
+#define PQ_OP_CX_FLAG 15 /* CX(q0 control, q1 target) on flagged shots (no reference op) */
+#define PQ_OP_CZ_FLAG 16 /* CZ(q0, q1) on flagged shots (no reference op) */
+
+/* `slot` of a flagged op whose flag is a per-shot Bernoulli draw instead of
+ * a record: the op takes the state's next noise number k, and shot b is
+ * flagged iff (draw(seed, 2k+1, b) >> 40) < t0, t0 = pq_coin_threshold(p).
+ * With H..CZ this is the per-shot form of the reference's WithProb functors
+ * (GateFunctors.hpp:38-75, 96-124, 172-202, 256-288, 330-357, 379-406),
+ * which draw once per (shot, row) and so do not apply a gate per shot. */
+#define PQ_FLAG_RANDOM 0xFFFFFFFFu
 
 /*
  * One circuit operation (32 bytes).
  *  H..CZ:      q0 (control of CX/CZ), q1 (target of CX/CZ)
- *  *_FLAG:     q0, slot = record whose result bit is the per-shot flag
+ *  *_FLAG:     q0 (q1: target of CX_FLAG/CZ_FLAG), slot = record whose
+ *              result bit is the per-shot flag, or PQ_FLAG_RANDOM with
+ *              t0 = flag probability threshold (see above)
  *  NOISE:      q0; each shot draws u = draw(seed, 2k+1, shot) >> 32 for the
  *              state's k-th noise op and gets X if u < t0, Y if u < t1,
  *              Z if u < t2 (t0 <= t1 <= t2 are cumulative 32-bit thresholds)
@@ -136,6 +148,20 @@ int pq_gate_flag_record(pq_state s, uint32_t code, uint64_t q, uint64_t rec);
 /* Same with a host flag vector int32[B] (nonzero = apply). */
 int pq_gate_flag_host(pq_state s, uint32_t code, uint64_t q,
                       const int32_t *flags);
+/* Flagged CX/CZ (PQ_OP_CX_FLAG, PQ_OP_CZ_FLAG: q0 control, q1 target) with
+ * the flag from record `rec` or from a host int32[B] vector.  No reference
+ * counterpart: the reference flags H and S only (CliffordState.hpp:348-372);
+ * these extend that API to the two-qubit gates. */
+int pq_gate2_flag_record(pq_state s, uint32_t code, uint64_t q0, uint64_t q1, uint64_t rec);
+int pq_gate2_flag_host(pq_state s, uint32_t code, uint64_t q0, uint64_t q1,
+                       const int32_t *flags);
+/* Gate `code` (H..CZ) on each shot independently with probability `prob`:
+ * the per-shot form of the WithProb functors (GateFunctors.hpp:38-406,
+ * e.g. BatchHadamardGateWithProbFunctor :96-124), which are not reachable
+ * from CliffordState and draw per (shot, row).  Takes the state's next
+ * noise number k; shot b applies the gate iff (draw(seed, 2k+1, b) >> 40) <
+ * pq_coin_threshold(prob), the reference's frand(0,1) < prob. */
+int pq_gate_prob(pq_state s, uint32_t code, uint64_t q0, uint64_t q1, float prob);
 /* Per-shot Pauli channel (PQ_OP_NOISE semantics); no reference counterpart:
  * the reference reaches per-shot Paulis only through the flagged H/S API. */
 int pq_noise(pq_state s, uint64_t q, uint32_t t0, uint32_t t1, uint32_t t2);

@@ -131,6 +131,16 @@ void orc_gate_flag(int64_t n, int64_t B, int32_t *x, int32_t *z, int32_t *r,
     }
 }
 
+/* The unflagged CX/CZ rule on each flagged shot's own [R][n] slice. */
+void orc_gate2_flag(int64_t n, int64_t B, int32_t *x, int32_t *z, int32_t *r,
+                    int code, int64_t q0, int64_t q1, const int32_t *flag) {
+  const int64_t R = 2 * n + 1;
+  const int gate = code == ORC_CX_FLAG ? ORC_CX : ORC_CZ;
+  for (int64_t b = 0; b < B; ++b)
+    if (flag[b])
+      orc_gate(n, 1, x + IDX3(b, 0, 0), z + IDX3(b, 0, 0), r + IDX2(b, 0), gate, q0, q1);
+}
+
 /* X rule (GateFunctors.hpp:306) where fx[b], Z rule (:375) where fz[b]. */
 void orc_pauli_flag(int64_t n, int64_t B, int32_t *x, int32_t *z, int32_t *r,
                     int64_t q, const int32_t *fx, const int32_t *fz) {
@@ -284,8 +294,9 @@ int orc_run_program(int64_t n, int64_t B, int32_t *x, int32_t *z, int32_t *r,
     int code = op[0];
     int64_t q0 = op[1], q1 = op[2], slot = op[3];
     uint32_t t0 = (uint32_t)op[4], t1 = (uint32_t)op[5], t2 = (uint32_t)op[6];
-    if (q0 < 0 || q0 >= n || ((code == ORC_CX || code == ORC_CZ) &&
-                              (q1 < 0 || q1 >= n || q1 == q0))) {
+    const int two = code == ORC_CX || code == ORC_CZ || code == ORC_CX_FLAG ||
+                    code == ORC_CZ_FLAG;
+    if (q0 < 0 || q0 >= n || (two && (q1 < 0 || q1 >= n || q1 == q0))) {
       rc = -2;
       break;
     }
@@ -294,11 +305,23 @@ int orc_run_program(int64_t n, int64_t B, int32_t *x, int32_t *z, int32_t *r,
       case ORC_CX: case ORC_CZ:
         orc_gate(n, B, x, z, r, code, q0, q1);
         break;
-      case ORC_H_FLAG: case ORC_S_FLAG:
-      case ORC_X_FLAG: case ORC_Y_FLAG: case ORC_Z_FLAG: {
-        if (slot < 0 || slot >= nrec) { rc = -3; break; }
-        const int32_t *flag = records + (size_t)slot * 2 * B;
-        if (code == ORC_H_FLAG || code == ORC_S_FLAG)
+      case ORC_H_FLAG: case ORC_S_FLAG: case ORC_X_FLAG: case ORC_Y_FLAG:
+      case ORC_Z_FLAG: case ORC_CX_FLAG: case ORC_CZ_FLAG: {
+        const int32_t *flag;
+        if (slot == ORC_FLAG_RANDOM) {
+          uint64_t kk = (uint64_t)(*noise_seq)++;
+          for (int64_t b = 0; b < B; ++b)
+            c[b] = (uint32_t)(orc_draw(seed, 2 * kk + 1, (uint64_t)(shot0 + b)) >> 40) < t0;
+          flag = c;
+        } else if (slot < 0 || slot >= nrec) {
+          rc = -3;
+          break;
+        } else {
+          flag = records + (size_t)slot * 2 * B;
+        }
+        if (code == ORC_CX_FLAG || code == ORC_CZ_FLAG)
+          orc_gate2_flag(n, B, x, z, r, code, q0, q1, flag);
+        else if (code == ORC_H_FLAG || code == ORC_S_FLAG)
           orc_gate_flag(n, B, x, z, r, code, q0, flag);
         else
           orc_pauli_flag(n, B, x, z, r, q0,

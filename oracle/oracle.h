@@ -45,8 +45,11 @@ int64_t orc_checksum(int64_t n, int64_t B, const int32_t *x, const int32_t *z,
 enum {
   ORC_H = 1, ORC_S = 2, ORC_X = 3, ORC_Y = 4, ORC_Z = 5, ORC_CX = 6, ORC_CZ = 7,
   ORC_H_FLAG = 8, ORC_S_FLAG = 9, ORC_NOISE = 10, ORC_X_FLAG = 11,
-  ORC_Y_FLAG = 12, ORC_Z_FLAG = 13, ORC_M = 14
+  ORC_Y_FLAG = 12, ORC_Z_FLAG = 13, ORC_M = 14, ORC_CX_FLAG = 15,
+  ORC_CZ_FLAG = 16
 };
+/* Slot value of a flagged op whose flag is a per-shot Bernoulli draw. */
+#define ORC_FLAG_RANDOM (-1)
 
 /* One unconditional gate over every shot and all 2n+1 rows
  * (GateFunctors.hpp: CZ :29-35, H :89-93, S :151-170, CX :247-253,
@@ -58,6 +61,14 @@ void orc_gate(int64_t n, int64_t B, int32_t *x, int32_t *z, int32_t *r,
  * where flag[b] != 0; code is ORC_H_FLAG or ORC_S_FLAG. */
 void orc_gate_flag(int64_t n, int64_t B, int32_t *x, int32_t *z, int32_t *r,
                    int code, int64_t q, const int32_t *flag);
+
+/* Shot-conditional two-qubit gates: the CX (code ORC_CX_FLAG) or CZ
+ * (ORC_CZ_FLAG) rule on q0 (control), q1 (target) where flag[b] != 0.  The
+ * reference has no flagged CX/CZ; its per-draw variants are the WithProb
+ * functors (GateFunctors.hpp:38-75, 256-288).  Each flagged shot is the
+ * unflagged rule of orc_gate applied to that shot alone. */
+void orc_gate2_flag(int64_t n, int64_t B, int32_t *x, int32_t *z, int32_t *r,
+                    int code, int64_t q0, int64_t q1, const int32_t *flag);
 
 /* Per-shot Pauli: shot b gets X^fx[b] Z^fz[b] on qubit q, i.e. the
  * reference's X rule (r ^= z_q) where fx[b] and Z rule (r ^= x_q) where
@@ -109,7 +120,10 @@ uint32_t orc_coin_threshold(float bias);
 
 /* One op = 8 int32 fields: {code, q0, q1, slot, t0, t1, t2, reserved}.
  *  gates H..CZ: q0 (control for CX/CZ), q1 (target)
- *  *_FLAG: q0, slot = record whose result bits are the per-shot flag
+ *  *_FLAG: q0 (and q1 for CX/CZ_FLAG), slot = record whose result bits
+ *         are the per-shot flag, or ORC_FLAG_RANDOM: then the op takes the
+ *         state's next noise number k and shot b is flagged iff
+ *         (draw(seed, 2k+1, shot) >> 40) < t0 (t0 a coin threshold)
  *  NOISE: q0, t0 < t1 < t2 cumulative 32-bit thresholds for X, Y, Z;
  *         a shot draws u = draw(seed, 2k+1, shot) >> 32 for the k-th noise
  *         op of the state and gets X if u<t0, Y if u<t1, Z if u<t2

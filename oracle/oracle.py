@@ -20,6 +20,8 @@ _LIB = None
 ORC_H, ORC_S, ORC_X, ORC_Y, ORC_Z, ORC_CX, ORC_CZ = 1, 2, 3, 4, 5, 6, 7
 ORC_H_FLAG, ORC_S_FLAG, ORC_NOISE = 8, 9, 10
 ORC_X_FLAG, ORC_Y_FLAG, ORC_Z_FLAG, ORC_M = 11, 12, 13, 14
+ORC_CX_FLAG, ORC_CZ_FLAG = 15, 16
+ORC_FLAG_RANDOM = 0xFFFFFFFF  # slot of a flagged op with a Bernoulli flag
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -44,6 +46,8 @@ def lib():
     L.orc_checksum.argtypes = [_i64, _i64, _i32p, _i32p, _i32p, _i64, _i64]
     L.orc_gate.argtypes = [_i64, _i64, _i32p, _i32p, _i32p, ctypes.c_int, _i64, _i64]
     L.orc_gate_flag.argtypes = [_i64, _i64, _i32p, _i32p, _i32p, ctypes.c_int, _i64, _i32p]
+    L.orc_gate2_flag.argtypes = [_i64, _i64, _i32p, _i32p, _i32p, ctypes.c_int, _i64, _i64,
+                                 _i32p]
     L.orc_pauli_flag.argtypes = [_i64, _i64, _i32p, _i32p, _i32p, _i64, _i32p, _i32p]
     L.orc_row_product_sign.restype = ctypes.c_int32
     L.orc_row_product_sign.argtypes = [_i64, _i64, _i32p, _i32p, _i32p, _i64, _i64, _i64]
@@ -133,6 +137,14 @@ class OracleState:
         f = np.ascontiguousarray(flag, np.int32)
         lib().orc_gate_flag(*self._args(), ORC_S_FLAG, int(q), _p32(f))
 
+    def apply_control_not_gate_with_apply_flag(self, c, t, flag):
+        f = np.ascontiguousarray(flag, np.int32)
+        lib().orc_gate2_flag(*self._args(), ORC_CX_FLAG, int(c), int(t), _p32(f))
+
+    def apply_control_phase_gate_with_apply_flag(self, c, t, flag):
+        f = np.ascontiguousarray(flag, np.int32)
+        lib().orc_gate2_flag(*self._args(), ORC_CZ_FLAG, int(c), int(t), _p32(f))
+
     def apply_pauli_flag(self, q, fx=None, fz=None):
         ax = np.ascontiguousarray(fx, np.int32) if fx is not None else None
         az = np.ascontiguousarray(fz, np.int32) if fz is not None else None
@@ -189,7 +201,10 @@ class OracleState:
     # -- program interpreter --
     def run_program(self, ops: np.ndarray, nrec_total: int | None = None):
         """Run an op list (int32 [nops][8]); records are created as needed."""
-        ops = np.ascontiguousarray(ops, np.int32).reshape(-1, 8)
+        # fields are uint32 (a random flag's slot is 0xFFFFFFFF); the C side
+        # reads the same bits as int32
+        ops = np.ascontiguousarray(
+            np.asarray(ops, np.int64).astype(np.uint32).view(np.int32)).reshape(-1, 8)
         need = len(self.records)
         for op in ops:
             if op[0] == ORC_M:

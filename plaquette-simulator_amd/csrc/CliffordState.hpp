@@ -168,6 +168,57 @@ template <class Precision> class CliffordState {
   void ApplyPauliWithApplyFlag(uint32_t code, std::size_t q, const RecordPlane &flag) {
     flagged(code, q, flag);
   }
+  void ApplyPauliXGateWithApplyFlag(std::size_t q, const RecordPlane &flag) {
+    flagged(PQ_OP_X_FLAG, q, flag);
+  }
+  void ApplyPauliYGateWithApplyFlag(std::size_t q, const RecordPlane &flag) {
+    flagged(PQ_OP_Y_FLAG, q, flag);
+  }
+  void ApplyPauliZGateWithApplyFlag(std::size_t q, const RecordPlane &flag) {
+    flagged(PQ_OP_Z_FLAG, q, flag);
+  }
+  void ApplyPauliXGateWithApplyFlag(std::size_t q, const std::vector<Precision> &flag) {
+    flagged_host(PQ_OP_X_FLAG, q, flag);
+  }
+  void ApplyPauliYGateWithApplyFlag(std::size_t q, const std::vector<Precision> &flag) {
+    flagged_host(PQ_OP_Y_FLAG, q, flag);
+  }
+  void ApplyPauliZGateWithApplyFlag(std::size_t q, const std::vector<Precision> &flag) {
+    flagged_host(PQ_OP_Z_FLAG, q, flag);
+  }
+  /// CX / CZ on the flagged shots (the H/S flag API of :348-372 extended to
+  /// the two-qubit gates; no reference counterpart).
+  void ApplyControlNotGateWithApplyFlag(std::size_t c, std::size_t t, const RecordPlane &flag) {
+    flagged2(PQ_OP_CX_FLAG, c, t, flag);
+  }
+  void ApplyControlPhaseGateWithApplyFlag(std::size_t c, std::size_t t,
+                                          const RecordPlane &flag) {
+    flagged2(PQ_OP_CZ_FLAG, c, t, flag);
+  }
+  void ApplyControlNotGateWithApplyFlag(std::size_t c, std::size_t t,
+                                        const std::vector<Precision> &flag) {
+    flagged2_host(PQ_OP_CX_FLAG, c, t, flag);
+  }
+  void ApplyControlPhaseGateWithApplyFlag(std::size_t c, std::size_t t,
+                                          const std::vector<Precision> &flag) {
+    flagged2_host(PQ_OP_CZ_FLAG, c, t, flag);
+  }
+
+  // ---- gates applied per shot with a probability ----
+  // The per-shot form of the reference's WithProb functors
+  // (GateFunctors.hpp:38-75, 96-124, 172-202, 256-288, 330-357, 379-406):
+  // each shot applies the gate iff frand(0,1) < prob, one draw per shot.
+  void ApplyHadamardGateWithProb(std::size_t q, float prob) { with_prob(PQ_OP_H, q, 0, prob); }
+  void ApplyPhaseGateWithProb(std::size_t q, float prob) { with_prob(PQ_OP_S, q, 0, prob); }
+  void ApplyPauliXGateWithProb(std::size_t q, float prob) { with_prob(PQ_OP_X, q, 0, prob); }
+  void ApplyPauliYGateWithProb(std::size_t q, float prob) { with_prob(PQ_OP_Y, q, 0, prob); }
+  void ApplyPauliZGateWithProb(std::size_t q, float prob) { with_prob(PQ_OP_Z, q, 0, prob); }
+  void ApplyControlNotGateWithProb(std::size_t c, std::size_t t, float prob) {
+    with_prob(PQ_OP_CX, c, t, prob);
+  }
+  void ApplyControlPhaseGateWithProb(std::size_t c, std::size_t t, float prob) {
+    with_prob(PQ_OP_CZ, c, t, prob);
+  }
 
   // ---- gates (CliffordState.hpp:379-531) ----
   void ApplyHadamardGateToQubits(const std::vector<std::size_t> &t) { each1(PQ_OP_H, t); }
@@ -311,6 +362,20 @@ template <class Precision> class CliffordState {
     PLAQUETTE_ABORT_IF(flag.size() != GetBatchSize(), "flag vector needs batch_size entries");
     std::vector<int32_t> f(flag.begin(), flag.end());
     detail::check(pq_gate_flag_host(h_, code, q, f.data()), "flagged gate");
+  }
+  void flagged2(uint32_t code, std::size_t c, std::size_t t, const RecordPlane &flag) {
+    PLAQUETTE_ABORT_IF(flag.state != h_, "flag record belongs to another state");
+    PLAQUETTE_ABORT_IF(flag.which != 0, "flag must be a results plane");
+    detail::check(pq_gate2_flag_record(h_, code, c, t, flag.record), "flagged gate");
+  }
+  void flagged2_host(uint32_t code, std::size_t c, std::size_t t,
+                     const std::vector<Precision> &flag) {
+    PLAQUETTE_ABORT_IF(flag.size() != GetBatchSize(), "flag vector needs batch_size entries");
+    std::vector<int32_t> f(flag.begin(), flag.end());
+    detail::check(pq_gate2_flag_host(h_, code, c, t, f.data()), "flagged gate");
+  }
+  void with_prob(uint32_t code, std::size_t q0, std::size_t q1, float prob) {
+    detail::check(pq_gate_prob(h_, code, q0, q1, prob), "gate with probability");
   }
 
   pq_state h_ = nullptr;

@@ -156,6 +156,70 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
             s.ApplyPauliWithApplyFlag(code, q, *s.GetMeasurement(id).measurement_results_device);
           },
           py::arg("kind"), py::arg("target_qubit"), py::arg("measurement_id"))
+      .def(
+          "apply_pauli_x_gate_with_apply_flag",
+          [](State &s, size_t q, const py::object &flag) {
+            if (py::isinstance<py::int_>(flag))
+              s.ApplyPauliXGateWithApplyFlag(
+                  q, *s.GetMeasurement(flag.cast<size_t>()).measurement_results_device);
+            else
+              s.ApplyPauliXGateWithApplyFlag(q, flag.cast<std::vector<int>>());
+          },
+          py::arg("target_qubit"), py::arg("flag"))
+      .def(
+          "apply_pauli_y_gate_with_apply_flag",
+          [](State &s, size_t q, const py::object &flag) {
+            if (py::isinstance<py::int_>(flag))
+              s.ApplyPauliYGateWithApplyFlag(
+                  q, *s.GetMeasurement(flag.cast<size_t>()).measurement_results_device);
+            else
+              s.ApplyPauliYGateWithApplyFlag(q, flag.cast<std::vector<int>>());
+          },
+          py::arg("target_qubit"), py::arg("flag"))
+      .def(
+          "apply_pauli_z_gate_with_apply_flag",
+          [](State &s, size_t q, const py::object &flag) {
+            if (py::isinstance<py::int_>(flag))
+              s.ApplyPauliZGateWithApplyFlag(
+                  q, *s.GetMeasurement(flag.cast<size_t>()).measurement_results_device);
+            else
+              s.ApplyPauliZGateWithApplyFlag(q, flag.cast<std::vector<int>>());
+          },
+          py::arg("target_qubit"), py::arg("flag"))
+      .def(
+          "apply_control_not_gate_with_apply_flag",
+          [](State &s, size_t c, size_t t, const py::object &flag) {
+            if (py::isinstance<py::int_>(flag))
+              s.ApplyControlNotGateWithApplyFlag(
+                  c, t, *s.GetMeasurement(flag.cast<size_t>()).measurement_results_device);
+            else
+              s.ApplyControlNotGateWithApplyFlag(c, t, flag.cast<std::vector<int>>());
+          },
+          py::arg("control_qubit"), py::arg("target_qubit"), py::arg("flag"))
+      .def(
+          "apply_control_phase_gate_with_apply_flag",
+          [](State &s, size_t c, size_t t, const py::object &flag) {
+            if (py::isinstance<py::int_>(flag))
+              s.ApplyControlPhaseGateWithApplyFlag(
+                  c, t, *s.GetMeasurement(flag.cast<size_t>()).measurement_results_device);
+            else
+              s.ApplyControlPhaseGateWithApplyFlag(c, t, flag.cast<std::vector<int>>());
+          },
+          py::arg("control_qubit"), py::arg("target_qubit"), py::arg("flag"))
+      .def("apply_hadamard_gate_with_prob", &State::ApplyHadamardGateWithProb,
+           py::arg("target_qubit"), py::arg("prob"))
+      .def("apply_phase_gate_with_prob", &State::ApplyPhaseGateWithProb,
+           py::arg("target_qubit"), py::arg("prob"))
+      .def("apply_pauli_x_gate_with_prob", &State::ApplyPauliXGateWithProb,
+           py::arg("target_qubit"), py::arg("prob"))
+      .def("apply_pauli_y_gate_with_prob", &State::ApplyPauliYGateWithProb,
+           py::arg("target_qubit"), py::arg("prob"))
+      .def("apply_pauli_z_gate_with_prob", &State::ApplyPauliZGateWithProb,
+           py::arg("target_qubit"), py::arg("prob"))
+      .def("apply_control_not_gate_with_prob", &State::ApplyControlNotGateWithProb,
+           py::arg("control_qubit"), py::arg("target_qubit"), py::arg("prob"))
+      .def("apply_control_phase_gate_with_prob", &State::ApplyControlPhaseGateWithProb,
+           py::arg("control_qubit"), py::arg("target_qubit"), py::arg("prob"))
       .def("apply_pauli_noise", &State::ApplyPauliNoise, py::arg("target_qubit"),
            py::arg("px"), py::arg("py"), py::arg("pz"))
       .def("check_sum", &State::CheckSum, py::arg("starting_batch_id") = 0,
@@ -438,5 +502,8 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
   m.attr("OP_X_FLAG") = PQ_OP_X_FLAG;
   m.attr("OP_Y_FLAG") = PQ_OP_Y_FLAG;
   m.attr("OP_Z_FLAG") = PQ_OP_Z_FLAG;
+  m.attr("OP_CX_FLAG") = PQ_OP_CX_FLAG;
+  m.attr("OP_CZ_FLAG") = PQ_OP_CZ_FLAG;
+  m.attr("FLAG_RANDOM") = PQ_FLAG_RANDOM;
   m.attr("OP_M") = PQ_OP_M;
 }

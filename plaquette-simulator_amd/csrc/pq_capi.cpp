@@ -320,6 +320,19 @@ static bool is_flag_op(uint32_t c) {
   return c == PQ_OP_H_FLAG || c == PQ_OP_S_FLAG || c == PQ_OP_X_FLAG ||
          c == PQ_OP_Y_FLAG || c == PQ_OP_Z_FLAG;
 }
+static bool is_flag2_op(uint32_t c) { return c == PQ_OP_CX_FLAG || c == PQ_OP_CZ_FLAG; }
+// The flagged op code of gate `c` (H..CZ).
+static uint32_t flagged_code(uint32_t c) {
+  switch (c) {
+    case PQ_OP_H: return PQ_OP_H_FLAG;
+    case PQ_OP_S: return PQ_OP_S_FLAG;
+    case PQ_OP_X: return PQ_OP_X_FLAG;
+    case PQ_OP_Y: return PQ_OP_Y_FLAG;
+    case PQ_OP_Z: return PQ_OP_Z_FLAG;
+    case PQ_OP_CX: return PQ_OP_CX_FLAG;
+    default: return PQ_OP_CZ_FLAG;
+  }
+}
 
 int plain_gate(pq_state s, uint32_t code, uint64_t q0, uint64_t q1) {
   ARGCHK(s, "null state");
@@ -340,7 +353,8 @@ extern "C" int pq_gate(pq_state s, uint32_t code, uint64_t q0, uint64_t q1) {
 // Flagged op with a device flag plane (W words).
 static int zero_scratch_tail(pq_state s);
 
-int plain_flag_plane(pq_state s, uint32_t code, uint64_t q, const uint64_t *plane) {
+int plain_flag_plane(pq_state s, uint32_t code, uint64_t q, uint64_t q1,
+                     const uint64_t *plane) {
   if (!s->tab) return PQ_OK;
   int rc = zero_scratch_tail(s);
   if (rc) return rc;
@@ -349,6 +363,11 @@ int plain_flag_plane(pq_state s, uint32_t code, uint64_t q, const uint64_t *plan
     case PQ_OP_H_FLAG:
     case PQ_OP_S_FLAG:
       HIPCHK(launch_gate(s->stream, s->tab, g, code, (uint32_t)q, 0, plane, nullptr));
+      break;
+    case PQ_OP_CX_FLAG:
+    case PQ_OP_CZ_FLAG:
+      HIPCHK(launch_gate(s->stream, s->tab, g, code, (uint32_t)q, (uint32_t)q1, plane,
+                         nullptr));
       break;
     case PQ_OP_X_FLAG:
       HIPCHK(launch_gate(s->stream, s->tab, g, PQ_OP_NOISE, (uint32_t)q, 0, plane,
@@ -394,7 +413,17 @@ extern "C" int pq_gate_flag_record(pq_state s, uint32_t code, uint64_t q, uint64
   if (rec >= s->nrec) return fail(PQ_ERR_RANGE, "Measurement id is out of range.");
   int rc = zero_scratch_tail(s);
   if (rc) return rc;
-  return plain_flag_plane(s, code, q, record_ptr(s, rec));
+  return plain_flag_plane(s, code, q, 0, record_ptr(s, rec));
+}
+
+extern "C" int pq_gate2_flag_record(pq_state s, uint32_t code, uint64_t q0, uint64_t q1,
+                                    uint64_t rec) {
+  ARGCHK(s, "null state");
+  ARGCHK(is_flag2_op(code), "not a flagged two-qubit op code");
+  if (q0 >= s->n || q1 >= s->n || q0 == q1)
+    return fail(PQ_ERR_ARG, "two-qubit gate needs distinct qubits in range");
+  if (rec >= s->nrec) return fail(PQ_ERR_RANGE, "Measurement id is out of range.");
+  return plain_flag_plane(s, code, q0, q1, record_ptr(s, rec));
 }
 
 extern "C" int pq_gate_flag_host(pq_state s, uint32_t code, uint64_t q,
@@ -408,7 +437,41 @@ extern "C" int pq_gate_flag_host(pq_state s, uint32_t code, uint64_t q,
   if (rc) return rc;
   rc = stage_host_ints(s, flags, s->scratch);
   if (rc) return rc;
-  return plain_flag_plane(s, code, q, s->scratch);
+  return plain_flag_plane(s, code, q, 0, s->scratch);
+}
+
+extern "C" int pq_gate2_flag_host(pq_state s, uint32_t code, uint64_t q0, uint64_t q1,
+                                  const int32_t *flags) {
+  ARGCHK(s, "null state");
+  ARGCHK(is_flag2_op(code), "not a flagged two-qubit op code");
+  ARGCHK(flags || s->B == 0, "flags is null");
+  if (q0 >= s->n || q1 >= s->n || q0 == q1)
+    return fail(PQ_ERR_ARG, "two-qubit gate needs distinct qubits in range");
+  if (!s->tab) return PQ_OK;
+  int rc = stage_host_ints(s, flags, s->scratch);
+  if (rc) return rc;
+  return plain_flag_plane(s, code, q0, q1, s->scratch);
+}
+
+// Flagged op `code` whose flag is a Bernoulli draw with coin threshold T
+// (PQ_FLAG_RANDOM): the flag plane is drawn into scratch[0, W) from the
+// state's next noise stream, then the flagged op runs on it.
+int plain_random_flag(pq_state s, uint32_t code, uint64_t q0, uint64_t q1, uint32_t T) {
+  const uint64_t k = s->noise_seq++;
+  if (!s->tab) return PQ_OK;
+  NoiseDesc d{stream_key((uint64_t)s->seed, 2 * k + 1), 0, T, 0, kNoiseFlag};
+  HIPCHK(launch_noise_planes(s->stream, s->scratch, d, nullptr, 1, s->W));
+  return plain_flag_plane(s, code, q0, q1, s->scratch);
+}
+
+extern "C" int pq_gate_prob(pq_state s, uint32_t code, uint64_t q0, uint64_t q1, float prob) {
+  ARGCHK(s, "null state");
+  ARGCHK(is_gate(code), "not a gate op code");
+  if (q0 >= s->n) return fail(PQ_ERR_ARG, "qubit index out of range");
+  if (is_two_qubit(code) && (q1 >= s->n || q1 == q0))
+    return fail(PQ_ERR_ARG, "two-qubit gate needs distinct qubits in range");
+  return plain_random_flag(s, flagged_code(code), q0, is_two_qubit(code) ? q1 : 0,
+                           pq_coin_threshold(prob));
 }
 
 int plain_noise(pq_state s, uint64_t q, uint32_t t0, uint32_t t1, uint32_t t2) {
@@ -417,7 +480,7 @@ int plain_noise(pq_state s, uint64_t q, uint32_t t0, uint32_t t1, uint32_t t2) {
   ARGCHK(t0 <= t1 && t1 <= t2, "noise thresholds must be cumulative");
   const uint64_t k = s->noise_seq++;
   if (!s->tab) return PQ_OK;
-  NoiseDesc d{stream_key((uint64_t)s->seed, 2 * k + 1), t0, t1, t2, 0};
+  NoiseDesc d{stream_key((uint64_t)s->seed, 2 * k + 1), t0, t1, t2, kNoisePauli};
   HIPCHK(launch_noise_planes(s->stream, s->scratch, d, nullptr, 1, s->W));
   HIPCHK(launch_gate(s->stream, s->tab, geom_of(s), PQ_OP_NOISE, (uint32_t)q, 0,
                      s->scratch, s->scratch + s->W));

@@ -56,7 +56,9 @@ pq::Geom geom_of(const pq_state_s *s);
 int ensure_records(pq_state s, uint64_t count);
 uint64_t *record_ptr(pq_state s, uint64_t id);
 int plain_gate(pq_state s, uint32_t code, uint64_t q0, uint64_t q1);
-int plain_flag_plane(pq_state s, uint32_t code, uint64_t q, const uint64_t *plane);
+int plain_flag_plane(pq_state s, uint32_t code, uint64_t q, uint64_t q1,
+                     const uint64_t *plane);
+int plain_random_flag(pq_state s, uint32_t code, uint64_t q0, uint64_t q1, uint32_t T);
 int plain_noise(pq_state s, uint64_t q, uint32_t t0, uint32_t t1, uint32_t t2);
 int plain_measure(pq_state s, uint64_t q, uint32_t T, uint64_t rec,
                   const uint64_t *coin_plane_dev);

@@ -59,7 +59,8 @@ PQ_HD void add_mod4(uint64_t &a0, uint64_t &a1, uint64_t b0, uint64_t b1) {
 }
 
 // Device-side instruction of a compiled program (32 bytes).
-//  gates: code, qa, qb; flagged ops: slot = record id;
+//  gates: code, qa, qb; flagged ops: slot = record id, or kSlotPlane | k
+//  when the flag is the X part of noise plane k (a random flag);
 //  na / nb: noise plane fused after the op on qa / qb (kNone if none);
 //  t0 (gate segments): local column indices of qa (low 16 bits) and qb
 //  (high 16 bits) in the segment's qubit list;
@@ -74,6 +75,7 @@ struct DInstr {
   uint32_t code, qa, qb, slot, na, nb, t0, pad;
 };
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr uint32_t kSlotPlane = 0x80000000u;
 constexpr uint32_t kLayerStart = 0x80000000u;
 constexpr uint32_t kBatchLenMask = 0xFFu;
 constexpr uint32_t kRunShift = 8;
@@ -89,10 +91,26 @@ struct DSegment {
   uint32_t begin, count, nq, qoff, pad0, pad1, pad2;
 };
 
-// Noise plane descriptor: stream and cumulative X/Y/Z thresholds.
+// Noise plane descriptor: stream, thresholds and kind.  A plane pair holds,
+// per 64-shot word, the shots whose draw v = draw_keyed(key, shot) fires:
+//  kNoisePauli: u = v >> 32; X part u < t1 (X or Y), Z part t0 <= u < t2
+//               (Y or Z) for cumulative thresholds t0 <= t1 <= t2;
+//  kNoiseFlag:  X part (v >> 40) < t1 (a coin threshold, 2^24 = always),
+//               Z part empty: the flag of a randomly flagged op.
+constexpr uint32_t kNoisePauli = 0, kNoiseFlag = 1;
 struct NoiseDesc {
   uint64_t key;  // stream_key(seed, 2k+1)
-  uint32_t t0, t1, t2, pad;
+  uint32_t t0, t1, t2, kind;
 };
+PQ_HD void noise_bits(const NoiseDesc &d, uint64_t v, bool &bx, bool &bz) {
+  if (d.kind == kNoiseFlag) {
+    bx = (uint32_t)(v >> 40) < d.t1;
+    bz = false;
+  } else {
+    const uint32_t u = (uint32_t)(v >> 32);
+    bx = u < d.t1;
+    bz = (u >= d.t0) & (u < d.t2);
+  }
+}
 
 }  // namespace pq

@@ -115,6 +115,26 @@ __global__ __launch_bounds__(256) void k_gate(uint64_t *__restrict__ tab,
         *pr ^= (fx & *pza) ^ (fz & *pxa);
         break;
       }
+      case PQ_OP_CX_FLAG: {  // the CX rule (:248-252) on flagged shots
+        const uint64_t f = a.fa[w];
+        uint64_t *pxb = T + (uint64_t)a.qb * R;
+        uint64_t *pzb = T + (uint64_t)(a.n + a.qb) * R;
+        const uint64_t xc = *pxa, zc = *pza, xt = *pxb, zt = *pzb;
+        *pr ^= f & xc & zt & ~(xt ^ zc);
+        *pxb = xt ^ (f & xc);
+        *pza = zc ^ (f & zt);
+        break;
+      }
+      case PQ_OP_CZ_FLAG: {  // the CZ rule (:30-35) on flagged shots
+        const uint64_t f = a.fa[w];
+        uint64_t *pxb = T + (uint64_t)a.qb * R;
+        uint64_t *pzb = T + (uint64_t)(a.n + a.qb) * R;
+        const uint64_t xc = *pxa, zc = *pza, xt = *pxb, zt = *pzb;
+        *pr ^= f & xc & xt & (zt ^ zc);
+        *pzb = zt ^ (f & xc);
+        *pza = zc ^ (f & xt);
+        break;
+      }
       default: break;
     }
   }
@@ -134,9 +154,10 @@ __global__ __launch_bounds__(256) void k_noise_planes(
     const uint64_t k = p / W;
     const uint64_t w = p - k * W;
     const NoiseDesc d = desc ? desc[k] : one;
-    const uint32_t u = (uint32_t)(draw_keyed(d.key, 64 * w + lane) >> 32);
-    const uint64_t mx = __ballot(u < d.t1);                 // X or Y
-    const uint64_t mz = __ballot((u >= d.t0) & (u < d.t2)); // Y or Z
+    bool bx, bz;
+    noise_bits(d, draw_keyed(d.key, 64 * w + lane), bx, bz);
+    const uint64_t mx = __ballot(bx);  // X or Y (or the random flag)
+    const uint64_t mz = __ballot(bz);  // Y or Z
     if (lane == 0) {
       planes[(2 * k) * W + w] = mx;
       planes[(2 * k + 1) * W + w] = mz;
@@ -621,11 +642,14 @@ __device__ __forceinline__ void stage_chunk(const pq::ProgramArgs &a, uint64_t w
       m2 = planes[(2ull * I.nb) * W + w];
       m3 = planes[(2ull * I.nb + 1) * W + w];
     } else if (I.code >= PQ_OP_H_FLAG && I.code != PQ_OP_NOISE) {
-      m2 = recs[(2ull * I.slot) * W + w];  // flag of a flagged op
+      // flag of a flagged op: a record's results or a random-flag plane
+      m2 = (I.slot & kSlotPlane) ? planes[(2ull * (I.slot & ~kSlotPlane)) * W + w]
+                                 : recs[(2ull * I.slot) * W + w];
     }
     // for the row-tiled path: whether the op's noise masks are non-zero on
     // this word group, and whether the op is the identity here (a noise op
-    // drawing nothing, a flagged op whose flag is clear in all 64 shots)
+    // drawing nothing, a flagged op whose flag is clear in all 64 shots);
+    // only unflagged CX/CZ carry noise on qb (m2 is a flagged op's flag)
     const bool two = I.code == PQ_OP_CX || I.code == PQ_OP_CZ;
     const bool flagged = I.code >= PQ_OP_H_FLAG && I.code != PQ_OP_NOISE;
     const bool noisy = (m0 | m1) != 0 || (two && (m2 | m3) != 0);
@@ -663,7 +687,8 @@ __device__ void run_gate_segment(uint64_t *__restrict__ T, uint64_t w,
             const DInstr &I = s_ins[k + u];
             xa[u] = Tr[(uint64_t)I.qa * Ru];
             za[u] = Tr[(uint64_t)(n + I.qa) * Ru];
-            if (I.code == PQ_OP_CX || I.code == PQ_OP_CZ) {
+            if (I.code == PQ_OP_CX || I.code == PQ_OP_CZ || I.code == PQ_OP_CX_FLAG ||
+                I.code == PQ_OP_CZ_FLAG) {
               xb[u] = Tr[(uint64_t)I.qb * Ru];
               zb[u] = Tr[(uint64_t)(n + I.qb) * Ru];
             }
@@ -692,24 +717,31 @@ __device__ void run_gate_segment(uint64_t *__restrict__ T, uint64_t w,
               case PQ_OP_X: r ^= z; break;
               case PQ_OP_Y: r ^= z ^ x; break;
               case PQ_OP_Z: r ^= x; break;
-              case PQ_OP_CX: {
+              case PQ_OP_CX:
+              case PQ_OP_CX_FLAG: {
+                // g: the flag of CX_FLAG (whose m2 is that flag, not noise)
+                const bool fl = I.code == PQ_OP_CX_FLAG;
+                const uint64_t g = fl ? f : ~0ull, n2 = fl ? 0ull : M[2];
                 uint64_t xt = xb[u], zt = zb[u];
-                r ^= x & zt & ~(xt ^ z);
-                xt ^= x;
-                z ^= zt;
+                r ^= g & x & zt & ~(xt ^ z);
+                xt ^= g & x;
+                z ^= g & zt;
                 Tr[(uint64_t)I.qb * Ru] = xt;
                 *pza = z;
-                r ^= (M[2] & zt) ^ (M[3] & xt);
+                r ^= (n2 & zt) ^ (M[3] & xt);
                 break;
               }
-              case PQ_OP_CZ: {
+              case PQ_OP_CZ:
+              case PQ_OP_CZ_FLAG: {
+                const bool fl = I.code == PQ_OP_CZ_FLAG;
+                const uint64_t g = fl ? f : ~0ull, n2 = fl ? 0ull : M[2];
                 uint64_t xt = xb[u], zt = zb[u];
-                r ^= x & xt & (zt ^ z);
-                zt ^= x;
-                z ^= xt;
+                r ^= g & x & xt & (zt ^ z);
+                zt ^= g & x;
+                z ^= g & xt;
                 Tr[(uint64_t)(n + I.qb) * Ru] = zt;
                 *pza = z;
-                r ^= (M[2] & zt) ^ (M[3] & xt);
+                r ^= (n2 & zt) ^ (M[3] & xt);
                 break;
               }
               case PQ_OP_H_FLAG: {
@@ -776,7 +808,7 @@ __device__ __forceinline__ uint32_t tile_rows(uint64_t lds_words, uint32_t nq) {
 //   r ^= (Cxz & x & z) ^ (Cz & z) ^ (Cx & x)
 // where each coefficient is the shot flag for a flagged op and all ones
 // otherwise: H = A Cxz, S = Bs Cxz, X = Cz, Y = Cz Cx, Z = Cx; NOISE has
-// none (only its fused planes).
+// none (only its fused planes).  CX and CZ are masked by the flag too.
 enum : uint32_t {
   kCoefA = 1, kCoefBs = 2, kCoefCxz = 4, kCoefCz = 8, kCoefCx = 16,
   kCoefF = 32, kCoefCX = 64, kCoefCZ = 128
@@ -794,6 +826,8 @@ constexpr uint64_t coef_byte(uint32_t code) {
        : code == PQ_OP_X_FLAG ? (kCoefCz | kCoefF)
        : code == PQ_OP_Y_FLAG ? (kCoefCz | kCoefCx | kCoefF)
        : code == PQ_OP_Z_FLAG ? (kCoefCx | kCoefF)
+       : code == PQ_OP_CX_FLAG ? (kCoefCX | kCoefF)
+       : code == PQ_OP_CZ_FLAG ? (kCoefCZ | kCoefF)
        : 0;
 }
 constexpr uint64_t coef_table(uint32_t c0) {
@@ -801,7 +835,7 @@ constexpr uint64_t coef_table(uint32_t c0) {
   for (uint32_t c = 0; c < 8; ++c) t |= coef_byte(c0 + c) << (8 * c);
   return t;
 }
-constexpr uint64_t kCoefLo = coef_table(0), kCoefHi = coef_table(8);
+constexpr uint64_t kCoefLo = coef_table(0), kCoefHi = coef_table(8), kCoefTop = coef_table(16);
 __device__ __forceinline__ uint64_t bitmask(uint32_t bits, uint32_t bit) {
   return 0ull - (uint64_t)((bits >> bit) & 1u);
 }
@@ -817,10 +851,12 @@ __device__ __forceinline__ uint64_t tile_op_any(uint32_t code, Ptr pxa, Ptr pza,
                                                 uint64_t z, uint64_t xt, uint64_t zt,
                                                 uint64_t m0, uint64_t m1, uint64_t m2,
                                                 uint64_t m3) {
-  const uint32_t cf =
-      (uint32_t)(((code < 8 ? kCoefLo : kCoefHi) >> (8 * (code & 7))) & 0xFFu);
+  const uint64_t coefs = code < 8 ? kCoefLo : (code < 16 ? kCoefHi : kCoefTop);
+  const uint32_t cf = (uint32_t)((coefs >> (8 * (code & 7))) & 0xFFu);
   const uint64_t F = m2 | ~bitmask(cf, 5);  // the flag, or all ones
-  const uint64_t CX = bitmask(cf, 6), CZ = bitmask(cf, 7), two = CX | CZ;
+  const uint64_t CX = bitmask(cf, 6) & F, CZ = bitmask(cf, 7) & F;
+  // noise planes on qb: unflagged CX/CZ only
+  const uint64_t two = (bitmask(cf, 6) | bitmask(cf, 7)) & ~bitmask(cf, 5);
   const uint64_t d = bitmask(cf, 0) & F & (x ^ z);
   const uint64_t x2 = x ^ d;
   const uint64_t z2 = z ^ d ^ (bitmask(cf, 1) & F & x) ^ (CX & zt) ^ (CZ & xt);
@@ -891,6 +927,20 @@ __device__ __forceinline__ uint64_t tile_op_uniform(uint32_t code, Ptr pxa, Ptr 
     case PQ_OP_X_FLAG: rp = f & z; break;
     case PQ_OP_Y_FLAG: rp = f & (z ^ x); break;
     case PQ_OP_Z_FLAG: rp = f & x; break;
+    case PQ_OP_CX_FLAG:
+      rp = f & x & zt & ~(xt ^ z);
+      xt ^= f & x;
+      z ^= f & zt;
+      *pxb = xt;
+      *pza = z;
+      break;
+    case PQ_OP_CZ_FLAG:
+      rp = f & x & xt & (zt ^ z);
+      zt ^= f & x;
+      z ^= f & xt;
+      *pzb = zt;
+      *pza = z;
+      break;
     default: break;  // PQ_OP_NOISE: only the fused plane below
   }
   return kNoise ? rp ^ (m0 & z) ^ (m1 & x) : rp;
@@ -918,7 +968,8 @@ __device__ __forceinline__ uint64_t tile_run_ops(lds_u64 *tl, uint32_t TRp, uint
     lds_u64 *pb = tl + (2 * (t0 >> 16) * TRp + lr);
     const uint64_t vx = pa[0], vz = pa[TRp];
     uint64_t vxt = 0, vzt = 0;
-    if (code == PQ_OP_CX || code == PQ_OP_CZ) {
+    if (code == PQ_OP_CX || code == PQ_OP_CZ || code == PQ_OP_CX_FLAG ||
+        code == PQ_OP_CZ_FLAG) {
       vxt = pb[0];
       vzt = pb[TRp];
     }
@@ -1476,9 +1527,10 @@ __global__ __launch_bounds__(256) void k_batch_noise(const pq::BatchCircuit *__r
     const NoiseDesc d = bc.noise[k];
     const uint64_t key = stream_key(bc.seed, 2 * (bc.nseq0 + k) + 1);
     for (uint64_t w = wave % ww; w < W; w += ww) {
-      const uint32_t u = (uint32_t)(draw_keyed(key, 64 * w + lane) >> 32);
-      const uint64_t mx = __ballot(u < d.t1);
-      const uint64_t mz = __ballot((u >= d.t0) & (u < d.t2));
+      bool bx, bz;
+      noise_bits(d, draw_keyed(key, 64 * w + lane), bx, bz);
+      const uint64_t mx = __ballot(bx);
+      const uint64_t mz = __ballot(bz);
       if (lane == 0) {
         planes[(2 * k) * W + w] = mx;
         planes[(2 * k + 1) * W + w] = mz;
@@ -1506,9 +1558,10 @@ __global__ __launch_bounds__(256) void k_program_noise(
     const uint64_t key = stream_key(seed, 2 * (nseq0 + k) + 1);
     uint64_t *px = planes + (2ull * k) * W, *pz = px + W;
     for (uint64_t w = wv; w < W; w += nwv) {
-      const uint32_t u = (uint32_t)(draw_keyed(key, 64 * w + lane) >> 32);
-      const uint64_t mx = __ballot(u < d.t1);
-      const uint64_t mz = __ballot((u >= d.t0) & (u < d.t2));
+      bool bx, bz;
+      noise_bits(d, draw_keyed(key, 64 * w + lane), bx, bz);
+      const uint64_t mx = __ballot(bx);
+      const uint64_t mz = __ballot(bz);
       if (lane == 0) {
         px[w] = mx;
         pz[w] = mz;

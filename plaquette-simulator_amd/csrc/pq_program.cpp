@@ -37,16 +37,20 @@ using namespace pq;
 
 namespace {
 
-bool is_two(uint32_t c) { return c == PQ_OP_CX || c == PQ_OP_CZ; }
+bool is_two(uint32_t c) {
+  return c == PQ_OP_CX || c == PQ_OP_CZ || c == PQ_OP_CX_FLAG || c == PQ_OP_CZ_FLAG;
+}
 bool is_flag(uint32_t c) {
   return c == PQ_OP_H_FLAG || c == PQ_OP_S_FLAG || c == PQ_OP_X_FLAG ||
-         c == PQ_OP_Y_FLAG || c == PQ_OP_Z_FLAG;
+         c == PQ_OP_Y_FLAG || c == PQ_OP_Z_FLAG || c == PQ_OP_CX_FLAG || c == PQ_OP_CZ_FLAG;
 }
+// A flagged op whose flag is a Bernoulli draw (a noise stream of its own).
+bool is_random_flag(const pq_op &o) { return is_flag(o.code) && o.slot == PQ_FLAG_RANDOM; }
 
 int validate(uint64_t n, const pq_op *ops, uint64_t nops) {
   for (uint64_t i = 0; i < nops; ++i) {
     const pq_op &o = ops[i];
-    if (o.code < PQ_OP_H || o.code > PQ_OP_M)
+    if (o.code < PQ_OP_H || o.code > PQ_OP_CZ_FLAG)
       return pq_set_error(PQ_ERR_ARG, ("op " + std::to_string(i) + ": unknown code").c_str());
     if (o.q0 >= n)
       return pq_set_error(PQ_ERR_ARG, ("op " + std::to_string(i) + ": qubit out of range").c_str());
@@ -55,6 +59,10 @@ int validate(uint64_t n, const pq_op *ops, uint64_t nops) {
                           ("op " + std::to_string(i) + ": two-qubit op needs distinct qubits in range").c_str());
     if (o.code == PQ_OP_NOISE && !(o.t0 <= o.t1 && o.t1 <= o.t2))
       return pq_set_error(PQ_ERR_ARG, ("op " + std::to_string(i) + ": noise thresholds not cumulative").c_str());
+    if (is_random_flag(o) && o.t0 > (1u << 24))
+      return pq_set_error(PQ_ERR_ARG, ("op " + std::to_string(i) + ": flag threshold above 2^24").c_str());
+    if (is_flag(o.code) && !is_random_flag(o) && (o.slot & kSlotPlane))
+      return pq_set_error(PQ_ERR_RANGE, ("op " + std::to_string(i) + ": flag record out of range").c_str());
   }
   return PQ_OK;
 }
@@ -108,7 +116,7 @@ static void plan_tiling(pq_program_s *p, DSegment &sg) {
     switch (I.code) {
       case PQ_OP_H: case PQ_OP_H_FLAG: direct += 4; break;
       case PQ_OP_S: case PQ_OP_S_FLAG: direct += 3; break;
-      case PQ_OP_CX: case PQ_OP_CZ: direct += 6; break;
+      case PQ_OP_CX: case PQ_OP_CZ: case PQ_OP_CX_FLAG: case PQ_OP_CZ_FLAG: direct += 6; break;
       default: direct += 2; break;
     }
   }
@@ -148,12 +156,17 @@ int compile_program(pq_program_s *p) {
         const pq_op &o = ops[k];
         if (o.code == PQ_OP_NOISE) {
           const uint32_t kn = nnoise++;
-          p->noise.push_back(NoiseDesc{0, o.t0, o.t1, o.t2, 0});
+          p->noise.push_back(NoiseDesc{0, o.t0, o.t1, o.t2, kNoisePauli});
           const int64_t l = last[o.q0];
           if (l >= 0 && !noise_only[l]) {
             DInstr &I = seg[l];
             if (I.qa == o.q0 && I.na == kNone) { I.na = kn; continue; }
-            if (is_two(I.code) && I.qb == o.q0 && I.nb == kNone) { I.nb = kn; continue; }
+            // noise on qb fuses into unflagged CX/CZ only (a flagged op's
+            // second mask is its flag)
+            if ((I.code == PQ_OP_CX || I.code == PQ_OP_CZ) && I.qb == o.q0 && I.nb == kNone) {
+              I.nb = kn;
+              continue;
+            }
           }
           const int32_t L = qlayer[o.q0] + 1;
           qlayer[o.q0] = L;
@@ -171,7 +184,13 @@ int compile_program(pq_program_s *p) {
           qlayer[o.q1] = L;
           last[o.q1] = (int64_t)seg.size();
         }
-        seg.push_back(DInstr{o.code, o.q0, is_two(o.code) ? o.q1 : 0, o.slot, kNone, kNone, 0, 0});
+        uint32_t slot = o.slot;
+        if (is_random_flag(o)) {  // the flag is the X part of a noise plane
+          const uint32_t kn = nnoise++;
+          p->noise.push_back(NoiseDesc{0, 0, o.t0, 0, kNoiseFlag});
+          slot = kSlotPlane | kn;
+        }
+        seg.push_back(DInstr{o.code, o.q0, is_two(o.code) ? o.q1 : 0, slot, kNone, kNone, 0, 0});
         layer.push_back((uint32_t)L);
         noise_only.push_back(0);
         if (o.code >= PQ_OP_H && o.code <= PQ_OP_CZ) ++p->ngates;
@@ -328,7 +347,7 @@ int prepare_records(pq_state s, pq_program p) {
       if (o.slot > cur)
         return pq_set_error(PQ_ERR_RANGE, ("op " + std::to_string(i) + ": record id out of range").c_str());
       if (o.slot == cur) ++cur;
-    } else if (is_flag(o.code) && o.slot >= cur) {
+    } else if (is_flag(o.code) && !is_random_flag(o) && o.slot >= cur) {
       return pq_set_error(PQ_ERR_RANGE, ("op " + std::to_string(i) + ": flag record out of range").c_str());
     }
   }
@@ -354,7 +373,8 @@ static int run_plain(pq_state s, pq_program p) {
         rc = plain_measure(s, o.q0, o.t0, o.slot, nullptr);
         break;
       default:
-        rc = plain_flag_plane(s, o.code, o.q0, record_ptr(s, o.slot));
+        rc = is_random_flag(o) ? plain_random_flag(s, o.code, o.q0, o.q1, o.t0)
+                               : plain_flag_plane(s, o.code, o.q0, o.q1, record_ptr(s, o.slot));
         break;
     }
     if (rc) return rc;

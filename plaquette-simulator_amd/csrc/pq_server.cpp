@@ -82,7 +82,9 @@ static uint64_t request_bytes(uint64_t n, uint64_t B, const pq_op *ops, uint64_t
   uint64_t nrec = 0, nnoise = 0;
   for (uint64_t i = 0; i < nops; ++i) {
     if (ops[i].code == PQ_OP_M && ops[i].slot + 1 > nrec) nrec = ops[i].slot + 1;
-    if (ops[i].code == PQ_OP_NOISE) ++nnoise;
+    if (ops[i].code == PQ_OP_NOISE || (ops[i].code != PQ_OP_M && ops[i].code >= PQ_OP_H_FLAG &&
+                                       ops[i].slot == PQ_FLAG_RANDOM))
+      ++nnoise;
   }
   *nrec_out = nrec;
   return W * C * C * 8 + nrec * 2 * W * 8 + nnoise * (2 * W * 8 + sizeof(pq::NoiseDesc)) +
@@ -230,8 +232,9 @@ extern "C" int pq_server_submit(pq_server sv, uint64_t n, uint64_t B, int64_t se
     return pq_set_error(PQ_ERR_UNSUPPORTED, "requests support at most 4095 qubits");
   for (uint64_t i = 0; i < nops; ++i) {
     const pq_op &o = ops[i];
-    const bool two = o.code == PQ_OP_CX || o.code == PQ_OP_CZ;
-    if (o.code < PQ_OP_H || o.code > PQ_OP_M || o.q0 >= n ||
+    const bool two = o.code == PQ_OP_CX || o.code == PQ_OP_CZ || o.code == PQ_OP_CX_FLAG ||
+                     o.code == PQ_OP_CZ_FLAG;
+    if (o.code < PQ_OP_H || o.code > PQ_OP_CZ_FLAG || o.q0 >= n ||
         (two && (o.q1 >= n || o.q1 == o.q0)))
       return pq_set_error(PQ_ERR_ARG, ("request op " + std::to_string(i) + " is invalid").c_str());
   }

@@ -33,6 +33,10 @@ import numpy as np
 OP_H, OP_S, OP_X, OP_Y, OP_Z, OP_CX, OP_CZ = 1, 2, 3, 4, 5, 6, 7
 OP_H_FLAG, OP_S_FLAG, OP_NOISE = 8, 9, 10
 OP_X_FLAG, OP_Y_FLAG, OP_Z_FLAG, OP_M = 11, 12, 13, 14
+OP_CX_FLAG, OP_CZ_FLAG = 15, 16
+FLAG_RANDOM = 0xFFFFFFFF  # slot of a flagged op whose flag is a Bernoulli draw
+FLAGGED = {OP_H: OP_H_FLAG, OP_S: OP_S_FLAG, OP_X: OP_X_FLAG, OP_Y: OP_Y_FLAG,
+           OP_Z: OP_Z_FLAG, OP_CX: OP_CX_FLAG, OP_CZ: OP_CZ_FLAG}
 GATE_CODES = (OP_H, OP_S, OP_X, OP_Y, OP_Z, OP_CX, OP_CZ)
 
 
@@ -69,8 +73,13 @@ class OpList:
     def cx(self, c, t): self.gate(OP_CX, c, t)
     def cz(self, c, t): self.gate(OP_CZ, c, t)
 
-    def flagged(self, code, q, slot):
-        self.rows.append((code, q, 0, slot, 0, 0, 0, 0))
+    def flagged(self, code, q, slot, q1=0):
+        """Flagged op `code` (H_FLAG..CZ_FLAG) with record `slot` as its flag."""
+        self.rows.append((code, q, q1, slot, 0, 0, 0, 0))
+
+    def with_prob(self, gate, q, prob, q1=0):
+        """Gate `gate` (H..CZ) on each shot with probability `prob`."""
+        self.rows.append((FLAGGED[gate], q, q1, FLAG_RANDOM, coin_threshold(prob), 0, 0, 0))
 
     def noise(self, q, px, py, pz):
         t0, t1, t2 = noise_thresholds(px, py, pz)

@@ -187,16 +187,28 @@ def test_record_reuse_and_errors(ps):
 
 # ---------------- random circuits, all forms, ragged batches ----------------
 
-def random_program(W, rng, n, nops, pflag=0.08, pmeas=0.08, pnoise=0.05):
+def random_program(W, rng, n, nops, pflag=0.08, pmeas=0.08, pnoise=0.05, flag2=False,
+                   prand=0.0):
+    """Random op list.  flag2 adds flagged CX/CZ to the flagged ops; prand is
+    the share of gates applied with a probability (random flags)."""
     b = W.OpList()
+    codes = [W.OP_H_FLAG, W.OP_S_FLAG, W.OP_X_FLAG, W.OP_Y_FLAG, W.OP_Z_FLAG]
+    if flag2 and n > 1:
+        codes += [W.OP_CX_FLAG, W.OP_CZ_FLAG]
     for _ in range(nops):
         u = rng.random()
         if u < pmeas:
             b.measure(int(rng.integers(0, n)), float(rng.choice([0.5, 0.5, 0.0, 1.0, 0.3])))
         elif u < pmeas + pflag and b.nrec:
-            code = int(rng.choice([W.OP_H_FLAG, W.OP_S_FLAG, W.OP_X_FLAG, W.OP_Y_FLAG, W.OP_Z_FLAG]))
-            b.flagged(code, int(rng.integers(0, n)), int(rng.integers(0, b.nrec)))
-        elif u < pmeas + pflag + pnoise:
+            code = int(rng.choice(codes))
+            c, t = rng.choice(n, 2, replace=False) if n > 1 else (0, 0)
+            b.flagged(code, int(c), int(rng.integers(0, b.nrec)), int(t))
+        elif u < pmeas + pflag + prand:
+            gate = int(rng.choice([W.OP_H, W.OP_S, W.OP_X, W.OP_Y, W.OP_Z] +
+                                  ([W.OP_CX, W.OP_CZ] if n > 1 else [])))
+            c, t = rng.choice(n, 2, replace=False) if n > 1 else (0, 0)
+            b.with_prob(gate, int(c), float(rng.choice([0.0, 0.1, 0.5, 1.0])), int(t))
+        elif u < pmeas + pflag + prand + pnoise:
             b.noise(int(rng.integers(0, n)), 0.1, 0.05, 0.2)
         else:
             code = int(rng.choice([W.OP_H, W.OP_S, W.OP_CX, W.OP_CZ, W.OP_X, W.OP_Y, W.OP_Z]))
@@ -237,6 +249,81 @@ def test_long_gate_segments(ps, W, form, seed, n, B, nops, pmeas):
     say(f"[layout] {info}")
     assert info["tiled_segments"] >= 1
     st, orc = run_both(ps, n, B, seed, ops, form)
+    assert_same(st, orc)
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("seed,n,B", [(21, 2, 64), (22, 6, 100), (23, 17, 200), (24, 40, 65)])
+def test_flagged_two_qubit_and_random_flags(ps, W, form, seed, n, B):
+    """Flagged CX/CZ and gates applied with a probability (random flags that
+    share the noise sequence with NOISE ops), mixed with everything else."""
+    rng = np.random.default_rng(seed)
+    ops = random_program(W, rng, n, 400, pflag=0.1, flag2=True, prand=0.1)
+    st, orc = run_both(ps, n, B, seed, ops, form)
+    assert_same(st, orc)
+
+
+@pytest.mark.parametrize("form", ["persistent", "graph"])
+@pytest.mark.parametrize("seed,n,B", [(31, 64, 130), (32, 300, 70)])
+def test_long_gate_segments_flagged(ps, W, form, seed, n, B):
+    """Row-tiled segments carrying flagged CX/CZ and random flags, with one op
+    per wave (n=64) and with several ops per wave (n=300)."""
+    rng = np.random.default_rng(seed)
+    ops = random_program(W, rng, n, 2000, pflag=0.05, pmeas=0.002, pnoise=0.1, flag2=True,
+                         prand=0.05)
+    ops = np.vstack([np.array([[W.OP_M, 0, 0, 0, O.coin_threshold(0.5), 0, 0, 0]], np.uint32), ops])
+    st0 = ps.CliffordState(n, B, seed, True)
+    info = ps.Program(st0, ops).info()
+    say(f"[layout] {info}")
+    assert info["tiled_segments"] >= 1
+    st, orc = run_both(ps, n, B, seed, ops, form)
+    assert_same(st, orc)
+
+
+def test_flag_and_prob_methods(ps, W):
+    """The per-gate API: flagged CX/CZ/Paulis (record id or host flags) and the
+    *_with_prob methods, against the oracle's program interpreter."""
+    n, B = 5, 300
+    st = ps.CliffordState(n, B, 23, True)
+    orc = O.OracleState(n, B, 23)
+    for q in range(n):
+        st.apply_hadamard_gate(q); orc.apply_hadamard_gate(q)
+    st.apply_control_not_gate(0, 1); orc.apply_control_not_gate(0, 1)
+    m = st.measure_qubit(2)
+    orc.measure_qubit(2)
+    host = (np.arange(B) % 5 < 2).astype(np.int32)
+    st.apply_control_not_gate_with_apply_flag(3, 4, m)
+    orc.apply_control_not_gate_with_apply_flag(3, 4, orc.records[m][0])
+    st.apply_control_phase_gate_with_apply_flag(1, 3, host.tolist())
+    orc.apply_control_phase_gate_with_apply_flag(1, 3, host)
+    st.apply_control_not_gate_with_apply_flag(4, 0, host.tolist())
+    orc.apply_control_not_gate_with_apply_flag(4, 0, host)
+    st.apply_control_phase_gate_with_apply_flag(2, 0, m)
+    orc.apply_control_phase_gate_with_apply_flag(2, 0, orc.records[m][0])
+    st.apply_pauli_x_gate_with_apply_flag(0, m)
+    st.apply_pauli_y_gate_with_apply_flag(1, host.tolist())
+    st.apply_pauli_z_gate_with_apply_flag(2, m)
+    orc.apply_pauli_flag(0, orc.records[m][0], None)
+    orc.apply_pauli_flag(1, host, host)
+    orc.apply_pauli_flag(2, None, orc.records[m][0])
+    rows = []
+    for k, (name, gate, q0, q1) in enumerate([
+            ("apply_hadamard_gate_with_prob", W.OP_H, 0, 0),
+            ("apply_phase_gate_with_prob", W.OP_S, 1, 0),
+            ("apply_pauli_x_gate_with_prob", W.OP_X, 2, 0),
+            ("apply_pauli_y_gate_with_prob", W.OP_Y, 3, 0),
+            ("apply_pauli_z_gate_with_prob", W.OP_Z, 4, 0),
+            ("apply_control_not_gate_with_prob", W.OP_CX, 0, 3),
+            ("apply_control_phase_gate_with_prob", W.OP_CZ, 4, 1)]):
+        p = (0.25, 0.5, 1.0, 0.0, 0.75, 0.4, 0.6)[k]
+        if q1 or gate in (W.OP_CX, W.OP_CZ):
+            getattr(st, name)(q0, q1, p)
+        else:
+            getattr(st, name)(q0, p)
+        rows.append([W.FLAGGED[gate], q0, q1, W.FLAG_RANDOM, W.coin_threshold(p), 0, 0, 0])
+    orc.run_program(np.array(rows, np.int64))
+    for q in range(n):
+        st.measure_qubit(q); orc.measure_qubit(q)
     assert_same(st, orc)
 
 

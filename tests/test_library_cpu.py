@@ -124,7 +124,9 @@ def test_op_codes_match_header():
     from plaquette_simulator import workloads as W
     src = open(HDR).read()
     for name in ("H", "S", "X", "Y", "Z", "CX", "CZ", "H_FLAG", "S_FLAG", "NOISE",
-                 "X_FLAG", "Y_FLAG", "Z_FLAG", "M"):
+                 "X_FLAG", "Y_FLAG", "Z_FLAG", "M", "CX_FLAG", "CZ_FLAG"):
         m = re.search(r"#define PQ_OP_%s (\d+)" % name, src)
         assert int(m.group(1)) == getattr(W, "OP_" + name)
         assert int(m.group(1)) == getattr(O, "ORC_" + name)
+    m = re.search(r"#define PQ_FLAG_RANDOM (0x[0-9A-Fa-f]+)u", src)
+    assert int(m.group(1), 16) == W.FLAG_RANDOM == O.ORC_FLAG_RANDOM

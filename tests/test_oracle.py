@@ -426,3 +426,112 @@ def test_noise_draw_rates():
     # qubit 0 flips on X or Y (rate 2p); qubit 1 flips on Z or Y (rate 2p)
     assert abs(r0.mean() - 2 * p) < 0.01
     assert abs(r1.mean() - 2 * p) < 0.01
+
+
+# ---------------- flagged CX/CZ and random flags ----------------
+
+def _random_state(n, B, seed, depth=60):
+    """A state scrambled by random H/S/CX, identical in every shot."""
+    rng = np.random.default_rng(seed)
+    s = O.OracleState(n, B, seed=seed)
+    for _ in range(depth):
+        code = int(rng.choice([O.ORC_H, O.ORC_S, O.ORC_CX]))
+        c, t = rng.choice(n, 2, replace=False)
+        s._gate(code, int(c), int(t))
+    return s
+
+
+@pytest.mark.parametrize("code,gate", [(O.ORC_CX_FLAG, O.ORC_CX), (O.ORC_CZ_FLAG, O.ORC_CZ)])
+def test_flagged_two_qubit_gate_is_the_gate_on_flagged_shots(code, gate):
+    """A flagged shot ends exactly as under the unflagged gate (itself pinned
+    by the statevector and identities above); an unflagged shot is untouched."""
+    n, B = 5, 40
+    flag = (np.arange(B) % 3 == 1).astype(np.int32)
+    a = _random_state(n, B, 3)
+    ref = _random_state(n, B, 3)
+    ref._gate(gate, 1, 3)
+    before = _random_state(n, B, 3)
+    if code == O.ORC_CX_FLAG:
+        a.apply_control_not_gate_with_apply_flag(1, 3, flag)
+    else:
+        a.apply_control_phase_gate_with_apply_flag(1, 3, flag)
+    for b in range(B):
+        want = ref if flag[b] else before
+        assert np.array_equal(a.x[b], want.x[b])
+        assert np.array_equal(a.z[b], want.z[b])
+        assert np.array_equal(a.r[b], want.r[b])
+
+
+def test_flagged_cx_makes_bell_pairs_only_on_flagged_shots():
+    """Physics: |+>|0> then CX on flagged shots gives correlated outcomes there
+    and a deterministic 0 on the target elsewhere."""
+    n, B = 2, 256
+    flag = (np.arange(B) % 2).astype(np.int32)
+    s = O.OracleState(n, B, seed=11)
+    s.apply_hadamard_gate(0)
+    s.apply_control_not_gate_with_apply_flag(0, 1, flag)
+    m0 = s.measure_qubit(0)
+    m1 = s.measure_qubit(1)
+    r0, _ = s.records[m0]
+    r1, d1 = s.records[m1]
+    f = flag.astype(bool)
+    assert np.array_equal(r0[f], r1[f]) and d1[f].all()   # Bell pair: second is determined
+    assert (r1[~f] == 0).all() and d1[~f].all()            # target untouched
+    assert 0.3 < r0.mean() < 0.7
+
+
+def _prog(rows):
+    return np.array(rows, np.int64)
+
+
+@pytest.mark.parametrize("code", [O.ORC_H_FLAG, O.ORC_S_FLAG, O.ORC_X_FLAG, O.ORC_Y_FLAG,
+                                  O.ORC_Z_FLAG, O.ORC_CX_FLAG, O.ORC_CZ_FLAG])
+def test_random_flag_extremes(code):
+    """Probability 0 is the identity, probability 1 the unflagged gate; either
+    way the op takes one noise number."""
+    gate = {O.ORC_H_FLAG: O.ORC_H, O.ORC_S_FLAG: O.ORC_S, O.ORC_X_FLAG: O.ORC_X,
+            O.ORC_Y_FLAG: O.ORC_Y, O.ORC_Z_FLAG: O.ORC_Z, O.ORC_CX_FLAG: O.ORC_CX,
+            O.ORC_CZ_FLAG: O.ORC_CZ}[code]
+    n, B = 4, 70
+    for T, applied in ((0, False), (1 << 24, True)):
+        a = _random_state(n, B, 8)
+        a.run_program(_prog([[code, 2, 0, O.ORC_FLAG_RANDOM, T, 0, 0, 0]]))
+        assert a.noise_seq == 1
+        b = _random_state(n, B, 8)
+        if applied:
+            b._gate(gate, 2, 0)
+        assert np.array_equal(a.x, b.x) and np.array_equal(a.z, b.z)
+        assert np.array_equal(a.r, b.r)
+
+
+def test_random_flag_rate_and_shot_independence():
+    """X applied with probability p flips a |0> qubit on a fraction p of the
+    shots, independently per shot; a shot's draw depends only on its global
+    index (a sub-batch at shot0 reproduces the slice)."""
+    n, B, p = 1, 20000, 0.3
+    s = O.OracleState(n, B, seed=5)
+    ops = _prog([[O.ORC_X_FLAG, 0, 0, O.ORC_FLAG_RANDOM, O.coin_threshold(p), 0, 0, 0],
+                 [O.ORC_M, 0, 0, 0, O.coin_threshold(0.5), 0, 0, 0]])
+    s.run_program(ops)
+    r, d = s.records[0]
+    assert d.all()
+    assert abs(r.mean() - p) < 5 * np.sqrt(p * (1 - p) / B)
+    sub = O.OracleState(n, 100, seed=5, shot0=700)
+    sub.run_program(ops)
+    assert np.array_equal(sub.records[0][0], r[700:800])
+
+
+def test_random_flag_and_noise_share_one_sequence():
+    """Random flags and NOISE ops draw from one sequence of noise numbers in
+    program order: a NOISE after a random-flag op is the NOISE a state whose
+    sequence already stands at 1 would draw."""
+    n, B = 3, 512
+    t = [int(0.2 * k * 2 ** 32) for k in (1, 2, 3)]
+    a = _random_state(n, B, 9)
+    a.run_program(_prog([[O.ORC_CX_FLAG, 0, 1, O.ORC_FLAG_RANDOM, 0, 0, 0, 0],
+                         [O.ORC_NOISE, 2, 0, 0, *t, 0]]))
+    b = _random_state(n, B, 9)
+    b.noise_seq = 1
+    b.run_program(_prog([[O.ORC_NOISE, 2, 0, 0, *t, 0]]))
+    assert a.noise_seq == b.noise_seq == 2
+    assert np.array_equal(a.r, b.r)
