@@ -105,6 +105,11 @@ int pq_circuit_parse(const char *text, uint64_t len, uint32_t first_record, pq_o
 #define PQ_RUN_PLAIN 0      /* one kernel launch per op (reference style) */
 #define PQ_RUN_PERSISTENT 1 /* one persistent kernel for the whole op list */
 #define PQ_RUN_GRAPH 2      /* the persistent form captured once into a hipGraph and replayed */
+/* X/Z simulated once on the host, only the per-shot phase column on the GPU.
+ * For programs without flagged H/S/CX/CZ (so X/Z stay the same in every
+ * shot), run from the |0...0> tableau (a fresh or reset state); results are
+ * bit-identical to the other forms.  PQ_ERR_UNSUPPORTED otherwise. */
+#define PQ_RUN_FRAME 3
 
 typedef struct pq_state_s *pq_state;
 
@@ -218,6 +223,19 @@ int pq_program_info(pq_program p, uint64_t *gates, uint64_t *measurements,
 /* Compiled layout: segments (runs of gates or of measurements) and the gate
  * segments that run row-tiled out of LDS when the launch's LDS allows. */
 int pq_program_layout(pq_program p, uint64_t *segments, uint64_t *tiled_segments);
+/* Frame form plan of the op list `ops` on n qubits, as PQ_RUN_FRAME compiles
+ * it (host only; PQ_ERR_UNSUPPORTED if the ops hold flagged H/S/CX/CZ).
+ * info[16] = {stream words, chunks,
+ * preload pairs, words per X/Z bit row, flag records, LDS word offsets of
+ * the records, the all-ones word and the noise window, noise-window words,
+ * stream-buffer words, LDS bytes per word group, random measurements,
+ * determined measurements, final X/Z is |0...0>, 0, 0}; non-NULL buffers
+ * receive the op stream, the chunk table ({begin, words, k0, nk} each), the
+ * {slot, index} preload pairs and the final shared X/Z bit rows
+ * ([2n+1][words per row] each).  No reference counterpart (tests, tools). */
+int pq_frame_plan(uint64_t n, const pq_op *ops, uint64_t nops, uint64_t info[16],
+                  uint32_t *stream, uint32_t *chunks, uint32_t *preload, uint64_t *x,
+                  uint64_t *z);
 /* Launch shape of the persistent kernel (0 keeps the current value). */
 int pq_program_set_launch(pq_program p, uint32_t threads, uint32_t blocks);
 /* Work census of persistent/graph runs (enable zeroes the counters):

@@ -106,6 +106,12 @@ class OracleState:
         if init_to_zero and self.n > 0 and self.B > 0:
             lib().orc_init_zero(self.n, self.B, _p32(self.x), _p32(self.z), _p32(self.r))
 
+    def reset_clifford_state(self):
+        """ResetCliffordState's intended semantics (CliffordState.hpp:309-316):
+        the |0...0> tableau again; records and sequence numbers are kept."""
+        if self.n > 0 and self.B > 0:
+            lib().orc_init_zero(self.n, self.B, _p32(self.x), _p32(self.z), _p32(self.r))
+
     @classmethod
     def from_flat(cls, x, z, r, num_qubits, batch_size, seed=-1):
         s = cls(num_qubits, batch_size, seed, init_to_zero=False)

@@ -22,6 +22,7 @@ int form_of(const py::object &f) {
   if (s == "plain") return PQ_RUN_PLAIN;
   if (s == "persistent") return PQ_RUN_PERSISTENT;
   if (s == "graph") return PQ_RUN_GRAPH;
+  if (s == "frame") return PQ_RUN_FRAME;
   throw std::invalid_argument("form must be 'plain', 'persistent' or 'graph'");
 }
 
@@ -320,6 +321,7 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
              d["tiled_segments"] = nt;
              return d;
            })
+
       .def("set_launch", [](Program &p, uint32_t threads, uint32_t blocks) {
         check(pq_program_set_launch(p.p, threads, blocks), "set_launch");
       })
@@ -481,6 +483,35 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
       },
       py::arg("text"), py::arg("first_record") = 0,
       "Parse CliffordCircuit text into (ops uint32[nops, 8], lines, qubits used).");
+  m.def(
+      "frame_plan",
+      [](uint64_t n, py::array_t<uint32_t, py::array::c_style | py::array::forcecast> ops) {
+        size_t nops = 0;
+        const pq_op *o = ops_ptr(ops, &nops);
+        uint64_t info[16];
+        check(pq_frame_plan(n, o, nops, info, nullptr, nullptr, nullptr, nullptr, nullptr),
+              "frame_plan");
+        py::array_t<uint32_t> stream(info[0]), chunks({(size_t)info[1], (size_t)4}),
+            preload({(size_t)info[2], (size_t)2});
+        const size_t R = 2 * n + 1;
+        py::array_t<uint64_t> x({R, (size_t)info[3]}), z({R, (size_t)info[3]});
+        check(pq_frame_plan(n, o, nops, info, stream.mutable_data(), chunks.mutable_data(),
+                            preload.mutable_data(), x.mutable_data(), z.mutable_data()),
+              "frame_plan");
+        py::dict d;
+        const char *names[14] = {"stream_words", "chunks_n", "preload_n", "nw", "nref",
+                                 "lds_rec", "lds_const", "lds_noise", "nb_words", "sb_words",
+                                 "lds_bytes", "random", "determined", "final_identity"};
+        for (int i = 0; i < 14; ++i) d[names[i]] = info[i];
+        d["stream"] = stream;
+        d["chunks"] = chunks;
+        d["preload"] = preload;
+        d["x"] = x;
+        d["z"] = z;
+        return d;
+      },
+      py::arg("num_qubits"), py::arg("ops"),
+      "Frame-form plan of an op list (host only): op stream, chunk table, final shared X/Z");
   m.def("coin_threshold", &pq_coin_threshold);
   m.def("draw", &pq_draw);
   m.def("backend_name", &pq_backend_name);

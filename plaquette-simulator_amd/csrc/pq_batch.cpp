@@ -240,6 +240,7 @@ extern "C" int pq_batch_run(pq_batch b, int form) {
     bc.nnoise = (uint32_t)p->nnoise;
     bc.pad = 0;
     s->meas_seq += p->nmeas;
+    s->xz_identity = false;
     s->noise_seq += p->nnoise;
     gens[c] = s->arena_gen;
   }

@@ -157,6 +157,7 @@ int create_state_on(uint64_t n, uint64_t B, int64_t seed, hipStream_t stream,
   if (rc) return rc;
   pq_state s = *out;
   if (s->tab) HIPCHK(launch_fill_identity(s->stream, s->tab, geom_of(s)));
+  s->xz_identity = true;
   return PQ_OK;
 }
 
@@ -168,6 +169,7 @@ extern "C" int pq_create(uint64_t n, uint64_t B, int64_t seed, int init_zero,
   if (s->tab) {
     if (init_zero) {
       HIPCHK(launch_fill_identity(s->stream, s->tab, geom_of(s)));
+      s->xz_identity = true;
     } else {
       HIPCHK(hipMemsetAsync(s->tab, 0, s->W * s->C * s->R * 8, s->stream));
     }
@@ -206,6 +208,7 @@ extern "C" int pq_info(pq_state s, uint64_t *n, uint64_t *B, int64_t *seed,
 extern "C" int pq_reset(pq_state s) {
   ARGCHK(s, "null state");
   if (s->tab) HIPCHK(launch_fill_identity(s->stream, s->tab, geom_of(s)));
+  s->xz_identity = true;
   return PQ_OK;
 }
 
@@ -225,6 +228,7 @@ extern "C" int pq_copy_from(pq_state dst, pq_state src) {
   ARGCHK(dst && src, "null state");
   ARGCHK(dst->n == src->n && dst->B == src->B, "DeviceToDevice needs equal n and B");
   if (!dst->tab) return PQ_OK;
+  dst->xz_identity = src->xz_identity;
   HIPCHK(hipStreamSynchronize(src->stream));
   HIPCHK(hipMemcpyAsync(dst->tab, src->tab, dst->W * dst->C * dst->R * 8,
                         hipMemcpyDeviceToDevice, dst->stream));
@@ -341,6 +345,8 @@ int plain_gate(pq_state s, uint32_t code, uint64_t q0, uint64_t q1) {
   if (is_two_qubit(code) && (q1 >= s->n || q1 == q0))
     return fail(PQ_ERR_ARG, "two-qubit gate needs distinct qubits in range");
   if (!s->tab) return PQ_OK;
+  if (code == PQ_OP_H || code == PQ_OP_S || code == PQ_OP_CX || code == PQ_OP_CZ)
+    s->xz_identity = false;
   HIPCHK(launch_gate(s->stream, s->tab, geom_of(s), code, (uint32_t)q0,
                      (uint32_t)q1, nullptr, nullptr));
   return PQ_OK;
@@ -359,6 +365,9 @@ int plain_flag_plane(pq_state s, uint32_t code, uint64_t q, uint64_t q1,
   int rc = zero_scratch_tail(s);
   if (rc) return rc;
   const Geom g = geom_of(s);
+  if (code == PQ_OP_H_FLAG || code == PQ_OP_S_FLAG || code == PQ_OP_CX_FLAG ||
+      code == PQ_OP_CZ_FLAG)
+    s->xz_identity = false;
   switch (code) {
     case PQ_OP_H_FLAG:
     case PQ_OP_S_FLAG:
@@ -496,6 +505,7 @@ int plain_measure(pq_state s, uint64_t q, uint32_t T, uint64_t rec,
   const uint64_t m = s->meas_seq++;
   s->rec_valid[rec] = 0;
   if (!s->tab) return PQ_OK;
+  s->xz_identity = false;
   if (s->n > measure_max_qubits())
     return fail(PQ_ERR_UNSUPPORTED, "measurement supports at most 4095 qubits");
   MeasParams mp{(uint32_t)q, T, stream_key((uint64_t)s->seed, 2 * m), coin_plane_dev,
@@ -570,6 +580,7 @@ extern "C" int pq_host_to_device(pq_state s, const int32_t *x, const int32_t *z,
   ARGCHK(s, "null state");
   if (!s->tab) return PQ_OK;
   ARGCHK(x && z && r, "input buffer is null");
+  s->xz_identity = false;
   const uint64_t chunk = std::min<uint64_t>(io_chunk_shots(s), 64 * s->W);
   const uint64_t per = s->R * s->n;
   int32_t *dx = nullptr, *dz = nullptr, *dr = nullptr;
@@ -657,6 +668,7 @@ extern "C" int pq_row_multiply(pq_state s, uint64_t i, uint64_t k) {
   ARGCHK(s, "null state");
   ARGCHK(i < s->R && k < s->R, "row out of range");
   if (!s->tab) return PQ_OK;
+  s->xz_identity = false;
   HIPCHK(launch_rowsum(s->stream, s->tab, geom_of(s), (uint32_t)i, (uint32_t)k, nullptr));
   return PQ_OK;
 }
@@ -678,6 +690,7 @@ extern "C" int pq_measure_determined(pq_state s, uint64_t q, int32_t *out) {
   if (q >= s->n) return fail(PQ_ERR_ARG, "qubit index out of range");
   if (!s->tab) return PQ_OK;
   uint64_t *rec = s->scratch;  // 2W words: results, determined (scratch is 3W)
+  s->xz_identity = false;
   MeasParams mp{(uint32_t)q, 0, 0, nullptr, rec, true};
   HIPCHK(launch_measure(s->stream, s->tab, geom_of(s), mp));
   std::vector<uint64_t> h(s->W);

@@ -31,6 +31,8 @@ struct pq_state_s {
   hipStream_t alloc_stream = nullptr;
   bool own_stream = true;
   hipEvent_t ev[8] = {};
+  // X/Z hold the |0...0> tableau in every shot (frame programs start there)
+  bool xz_identity = false;
   std::vector<std::vector<uint64_t>> rec_host;  // host copies of records
   std::vector<char> rec_valid;
   ~pq_state_s();

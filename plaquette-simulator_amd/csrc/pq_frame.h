@@ -1,0 +1,36 @@
+// Frame form of a program (shared-X/Z circuits): the compiled plan.
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "pq_common.h"
+#include "pq_kernels.h"
+
+struct pq_program_s;
+struct pq_state_s;
+
+// Per-wave LDS: noise-mask window (8-byte words) and stream buffer (uint32).
+constexpr uint32_t kFrameNoiseWords = 2048;
+constexpr uint32_t kFrameStreamWords = 8192;
+constexpr uint64_t kFrameLdsMax = 160 * 1024;
+
+struct FramePlan {
+  uint32_t R = 0, nw = 0;
+  uint32_t nref = 0;                 // records read by flag ops (rec_l words)
+  uint32_t lds_rec = 0, lds_const = 0, lds_noise = 0;  // LDS word offsets
+  uint32_t nb_words = 0, sb_words = 0, lds_bytes = 0;
+  std::vector<uint32_t> stream;      // ops
+  std::vector<uint32_t> chunks;      // {begin, words, k0, nk} per chunk
+  std::vector<uint32_t> preload;     // {slot, rec_l index} loaded at start
+  std::vector<uint64_t> X, Z;        // final shared X/Z, [R][nw]
+  bool final_identity = false;
+  uint64_t nrandom = 0, ndet = 0;
+  uint32_t *d_stream = nullptr, *d_chunks = nullptr, *d_preload = nullptr;
+  uint64_t *d_X = nullptr, *d_Z = nullptr;
+  ~FramePlan();
+};
+
+int build_frame_plan(pq_program_s *p);
+int run_frame(pq_state_s *s, pq_program_s *p);

@@ -1599,6 +1599,130 @@ __global__ void k_rowsum(uint64_t *__restrict__ tab, uint32_t n, uint32_t R,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Frame form (X/Z shared by all shots): one wave per 64-shot word group runs
+// the compiled op stream on the group's R column, held in LDS.  The stream,
+// its layout and the LDS map are described in pq_frame.cpp; the shared X/Z
+// themselves are written by k_frame_xz.
+// ---------------------------------------------------------------------------
+typedef __attribute__((address_space(3))) void lds_void;
+
+__device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__global__ __launch_bounds__(64) void k_frame(FrameArgs a, const ProgParams *__restrict__ pp) {
+  extern __shared__ uint64_t dyn[];
+  const uint64_t w = blockIdx.x;
+  const uint32_t lane = lane_id();
+  const uint32_t n = a.n, R = a.R;
+  uint64_t *Rl = dyn;
+  uint64_t *nbuf = dyn + a.lds_noise;
+  uint32_t *sbuf = reinterpret_cast<uint32_t *>(dyn + a.lds_noise + a.nb_words);
+  uint64_t *Tr = a.tab + (w * a.C + 2ull * n) * R;  // the group's R column
+  for (uint32_t i = lane; i < R; i += 64) Rl[i] = Tr[i];
+  for (uint32_t i = lane; i < a.npreload; i += 64)
+    dyn[a.lds_rec + a.preload[2 * i + 1]] = a.recs[(2ull * a.preload[2 * i]) * a.W + w];
+  if (lane == 0) dyn[a.lds_const] = ~0ull;
+  const uint64_t seed = pp->seed, mseq0 = pp->mseq0;
+  for (uint32_t c = 0; c < a.nchunk; ++c) {
+    const uint32_t cb = a.chunks[4 * c], cw = a.chunks[4 * c + 1];
+    const uint32_t k0 = a.chunks[4 * c + 2], nk = a.chunks[4 * c + 3];
+    // the last chunk's LDS reads are done before its buffers are refilled
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    // stream words and the chunk's noise masks (planes 2k0 .. 2(k0+nk)-1 of
+    // this group), by LDS DMA: one wave-wide 4-byte load per 64 dwords;
+    // lanes past the end reload the last dword into the buffer's padding
+    for (uint32_t i = 0; i < cw; i += 64) {
+      const uint32_t j = min(i + lane, cw - 1);
+      __builtin_amdgcn_global_load_lds(a.stream + cb + j, (lds_void *)(sbuf + i), 4, 0, 0);
+    }
+    const uint32_t dw = 4 * nk;
+    for (uint32_t i = 0; i < dw; i += 64) {
+      const uint32_t t = min(i + lane, dw - 1);
+      const uint32_t *src =
+          reinterpret_cast<const uint32_t *>(a.planes + (2ull * k0 + (t >> 1)) * a.W + w) +
+          (t & 1);
+      __builtin_amdgcn_global_load_lds(src, (lds_void *)(reinterpret_cast<uint32_t *>(nbuf) + i),
+                                       4, 0, 0);
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    uint32_t pos = 0;
+    while (pos < cw) {
+      const uint32_t kind = rfl(sbuf[pos]);
+      const uint32_t h1 = rfl(sbuf[pos + 1]);
+      if (kind == kFrameXor) {
+        // R[row] ^= XOR of the LDS mask words its entries name; rows distinct
+        const uint32_t T = h1, ne = rfl(sbuf[pos + 2]);
+        const uint32_t *rows = sbuf + pos + 8;
+        const uint32_t *offs = rows + T;
+        const uint32_t *ents = offs + T + 1;
+        for (uint32_t t = lane; t < T; t += 64) {
+          const uint32_t row = rows[t], e1 = offs[t + 1];
+          uint64_t acc = 0;
+          for (uint32_t e = offs[t]; e < e1; ++e) acc ^= dyn[ents[e]];
+          Rl[row] ^= acc;
+        }
+        pos += 8 + 2 * T + 1 + ne;
+      } else if (kind == kFrameRandom) {
+        // MeasureRandom (MeasurementFunctors.hpp:225-265) with pivot p
+        const uint32_t p = h1, slot = rfl(sbuf[pos + 2]), mi = rfl(sbuf[pos + 3]);
+        const uint32_t Tc = rfl(sbuf[pos + 4]), ridx = rfl(sbuf[pos + 5]);
+        const uint32_t ne = rfl(sbuf[pos + 6]);
+        const uint64_t Rs = Rl[n + p];
+        const uint64_t key = stream_key(seed, 2 * (mseq0 + mi));
+        const uint32_t u = (uint32_t)(draw_keyed(key, 64 * w + lane) >> 40);
+        const uint64_t coin = __ballot(u < Tc);
+        const uint32_t *ents = sbuf + pos + 8;
+        for (uint32_t e = lane; e < ne; e += 64) {
+          const uint32_t v = ents[e];
+          Rl[v & 0x7FFFFFFFu] ^= Rs ^ ((v >> 31) ? ~0ull : 0ull);
+        }
+        if (lane == 0) {
+          Rl[p] = Rs;
+          Rl[n + p] = coin;
+          a.recs[(2ull * slot) * a.W + w] = coin;
+          a.recs[(2ull * slot + 1) * a.W + w] = 0ull;
+          if (ridx != kNone) dyn[a.lds_rec + ridx] = coin;
+        }
+        pos += 8 + ne;
+      } else {
+        // G determined measurements (MeasureDetermined, :189-201), one per
+        // lane; their rows exclude the scratch row, which takes the last
+        const uint32_t G = h1;
+        const uint32_t *desc = sbuf + pos + 8;
+        const uint32_t *rows = desc + 4 * G + 1;
+        if (lane < G) {
+          const uint32_t slot = desc[4 * lane], ridx = desc[4 * lane + 1];
+          const uint32_t o0 = desc[4 * lane + 3];
+          const uint32_t o1 = lane + 1 < G ? desc[4 * lane + 7] : desc[4 * G];
+          uint64_t acc = desc[4 * lane + 2] ? ~0ull : 0ull;
+          for (uint32_t e = o0; e < o1; ++e) acc ^= Rl[rows[e]];
+          a.recs[(2ull * slot) * a.W + w] = acc;
+          a.recs[(2ull * slot + 1) * a.W + w] = ~0ull;
+          if (ridx != kNone) dyn[a.lds_rec + ridx] = acc;
+          if (lane == G - 1) Rl[2 * n] = acc;
+        }
+        pos += 8 + 4 * G + 1 + rfl(desc[4 * G]);
+      }
+    }
+  }
+  for (uint32_t i = lane; i < R; i += 64) Tr[i] = Rl[i];
+}
+
+// Shared X/Z (bit rows [R][nw]) into columns 0..2n-1 of every word group.
+__global__ __launch_bounds__(256) void k_frame_xz(uint64_t *__restrict__ tab,
+                                                  const uint64_t *__restrict__ X,
+                                                  const uint64_t *__restrict__ Z, uint32_t n,
+                                                  uint32_t R, uint32_t C, uint64_t W,
+                                                  uint32_t nw) {
+  const uint32_t row = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t c = blockIdx.y;
+  if (row >= R) return;
+  const uint32_t q = c < n ? c : c - n;
+  const uint64_t *M = c < n ? X : Z;
+  const uint64_t v = ((M[(uint64_t)row * nw + q / 64] >> (q % 64)) & 1) ? ~0ull : 0ull;
+  for (uint64_t w = blockIdx.z; w < W; w += gridDim.z) tab[(w * C + c) * R + row] = v;
+}
+
 inline uint32_t grid_for(uint64_t items, uint32_t threads, uint32_t cap = 1u << 20) {
   uint64_t g = (items + threads - 1) / threads;
   if (g < 1) g = 1;
@@ -1746,6 +1870,23 @@ hipError_t launch_program_noise(hipStream_t s, uint64_t *planes,
   const uint32_t gy = std::min<uint32_t>(nk, 65535);
   hipLaunchKernelGGL(k_program_noise, dim3(gx, gy), dim3(256), 0, s, planes, desc, nk, W,
                      (const ProgParams *)params);
+  return hipGetLastError();
+}
+
+hipError_t launch_frame(hipStream_t s, const FrameArgs &a, const void *params) {
+  if (!a.W) return hipSuccess;
+  hipError_t e = ensure_lds_attr((const void *)k_frame, a.lds_bytes);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_frame, dim3((uint32_t)a.W), dim3(64), a.lds_bytes, s, a,
+                     (const ProgParams *)params);
+  return hipGetLastError();
+}
+
+hipError_t launch_frame_xz(hipStream_t s, uint64_t *tab, const Geom &g, const uint64_t *X,
+                           const uint64_t *Z, uint32_t nw) {
+  if (!g.W || !g.n) return hipSuccess;
+  const dim3 grid((g.R + 255) / 256, 2 * g.n, (uint32_t)std::min<uint64_t>(g.W, 64));
+  hipLaunchKernelGGL(k_frame_xz, grid, dim3(256), 0, s, tab, X, Z, g.n, g.R, g.C, g.W, nw);
   return hipGetLastError();
 }
 

@@ -69,6 +69,8 @@ int validate(uint64_t n, const pq_op *ops, uint64_t nops) {
 
 }  // namespace
 
+int validate_ops(uint64_t n, const pq_op *ops, uint64_t nops) { return validate(n, ops, nops); }
+
 // Compile `ops` into segments/instructions (host side only).
 // Run lengths of a gate segment: a run starts at every layer start and at
 // every staging-chunk start and ends before the next one; its length goes
@@ -440,6 +442,16 @@ extern "C" int pq_program_run(pq_state s, pq_program p, int form) {
   if (!s || !p) return pq_set_error(PQ_ERR_ARG, "null argument");
   if (p->n != s->n || p->W != s->W)
     return pq_set_error(PQ_ERR_ARG, "program was compiled for a state of another shape");
+  if (form == PQ_RUN_FRAME && s->tab) {
+    // eligibility first, so a refused run leaves the state untouched
+    if (!s->xz_identity)
+      return pq_set_error(PQ_ERR_UNSUPPORTED,
+                          "frame form runs from the |0...0> tableau: reset the state first");
+    if (!p->frame) {
+      int rc0 = build_frame_plan(p);
+      if (rc0) return rc0;
+    }
+  }
   int rc = prepare_records(s, p);
   if (rc) return rc;
   if (!s->tab) {
@@ -447,6 +459,10 @@ extern "C" int pq_program_run(pq_state s, pq_program p, int form) {
     s->noise_seq += p->nnoise;
     return PQ_OK;
   }
+  if (form == PQ_RUN_FRAME && s->n > measure_max_qubits())
+    return pq_set_error(PQ_ERR_UNSUPPORTED, "frame form supports at most 4095 qubits");
+  if (form == PQ_RUN_FRAME) return run_frame(s, p);
+  s->xz_identity = false;
   if (form == PQ_RUN_PLAIN) return run_plain(s, p);
   if (s->n > measure_max_qubits())
     return pq_set_error(PQ_ERR_UNSUPPORTED, "persistent form supports at most 4095 qubits");

@@ -4,10 +4,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <memory>
 #include <vector>
 
 #include "plaquette_amd.h"
 #include "pq_common.h"
+#include "pq_frame.h"
 
 constexpr uint32_t kProgramBatch = 8;  // must equal kBatch in pq_kernels.hip
 constexpr uint32_t kProgramThreadsMax = 768;  // largest block of the persistent kernel builds
@@ -38,11 +40,13 @@ struct pq_program_s {
   bool timing_on = false;
   std::vector<hipEvent_t> ev;  // start/end pairs of timed kernel launches
   size_t ev_used = 0;
+  std::unique_ptr<FramePlan> frame;  // frame form plan, built on first use
   void drop_graph();
   ~pq_program_s();
 };
 
 int compile_program(pq_program_s *p);
+int validate_ops(uint64_t n, const pq_op *ops, uint64_t nops);
 
 constexpr uint64_t kProgramLdsBytes = 158 * 1024;  // dynamic-LDS cap (kProgramLdsBudget there)
 

@@ -1,0 +1,158 @@
+"""GPU parity of the frame form (X/Z shared by all shots, simulated once on
+the host; only R runs on the GPU) against the oracle and against the
+bit-plane persistent form: bit-exact tableaux and records."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+
+@pytest.fixture(scope="module")
+def ps():
+    import plaquette_simulator as ps
+    import plaquette_simulator_bindings as b
+    if b.device_count() < 1:
+        pytest.skip("no GPU")
+    return ps
+
+
+@pytest.fixture(scope="module")
+def W():
+    from plaquette_simulator import workloads as W
+    return W
+
+
+def say(msg):
+    sys.stderr.write(msg + "\n")
+    sys.stderr.flush()
+
+
+def assert_same(st, orc):
+    x, z, r = st.device_to_host_numpy()
+    assert np.array_equal(x, orc.x), "x differs"
+    assert np.array_equal(z, orc.z), "z differs"
+    assert np.array_equal(r, orc.r), "r differs"
+    for i, (res, det) in enumerate(orc.records):
+        gres, gdet = st.get_record(i)
+        assert np.array_equal(gres, res), f"record {i} results differ"
+        assert np.array_equal(gdet, det), f"record {i} determined differ"
+
+
+@pytest.mark.parametrize("seed,n,B", [(1, 1, 1), (2, 2, 63), (3, 5, 64), (4, 9, 65),
+                                      (5, 20, 300), (6, 64, 129), (7, 150, 100)])
+def test_random_frame_programs(ps, W, seed, n, B):
+    from test_frame_cpu import frame_program
+    rng = np.random.default_rng(seed)
+    ops = frame_program(W, rng, n, 400)
+    say(f"[frame] n={n} B={B} seed={seed} ops={len(ops)}")
+    st = ps.CliffordState(n, B, seed, True)
+    st.run_program(ops, "frame")
+    orc = O.OracleState(n, B, seed)
+    orc.run_program(ops)
+    assert_same(st, orc)
+
+
+@pytest.mark.parametrize("d,rounds,p,B", [(3, 3, 0.05, 1000), (5, 5, 0.01, 200)])
+def test_surface_code_frame(ps, W, d, rounds, p, B):
+    ops, n = W.surface_code_memory_z(d, rounds, p)
+    say(f"[frame] surface d={d} B={B}")
+    st = ps.CliffordState(n, B, 7, True)
+    st.run_program(ops, "frame")
+    orc = O.OracleState(n, B, 7)
+    orc.run_program(ops)
+    assert_same(st, orc)
+
+
+def test_long_program_many_chunks(ps, W):
+    """Several stream chunks and noise windows (the CPU test checks the plan)."""
+    from test_frame_cpu import frame_program
+    rng = np.random.default_rng(9)
+    n, B = 40, 200
+    ops = frame_program(W, rng, n, 6000, pflag=0.02, pmeas=0.01, pnoise=0.5, prand=0.05)
+    st = ps.CliffordState(n, B, 9, True)
+    st.run_program(ops, "frame")
+    orc = O.OracleState(n, B, 9)
+    orc.run_program(ops)
+    assert_same(st, orc)
+
+
+def test_d9_frame_equals_persistent(ps, W):
+    """The d=9, 16384-shot surface code: frame and persistent forms give the
+    same planes and records, run after run."""
+    wl = W.config_surface_d9()
+    a = ps.CliffordState(wl.num_qubits, wl.batch_size, wl.seed, True)
+    bst = ps.CliffordState(wl.num_qubits, wl.batch_size, wl.seed, True)
+    pa = ps.Program(a, wl.ops)
+    pb = ps.Program(bst, wl.ops)
+    for it in range(2):
+        say(f"[frame] d=9 run {it}")
+        a.reset_clifford_state()
+        bst.reset_clifford_state()
+        pa.run(a, "persistent")
+        pb.run(bst, "frame")
+        assert np.array_equal(a.read_planes(), bst.read_planes()), "planes differ"
+        for i in range(a.num_records):
+            ra, da = a.get_record(i)
+            rb, db = bst.get_record(i)
+            assert np.array_equal(ra, rb) and np.array_equal(da, db), f"record {i}"
+
+
+def test_frame_records_across_runs(ps, W):
+    """A flag reading a record written by an earlier run (preloaded by the
+    kernel), reset in between."""
+    n, B = 4, 130
+    a = W.OpList()
+    a.h(0)
+    m0 = a.measure(0)
+    b_ = W.OpList()
+    b_.nrec = 1
+    b_.flagged(W.OP_X_FLAG, 1, m0)
+    b_.flagged(W.OP_Z_FLAG, 2, m0)
+    b_.h(2)
+    b_.measure(1)
+    b_.measure(2)
+    st = ps.CliffordState(n, B, 5, True)
+    orc = O.OracleState(n, B, 5)
+    st.run_program(a.array(), "frame")
+    orc.run_program(a.array())
+    st.reset_clifford_state()
+    orc.reset_clifford_state()
+    st.run_program(b_.array(), "frame")
+    orc.run_program(b_.array())
+    assert_same(st, orc)
+
+
+def test_frame_refusals_leave_state_untouched(ps, W):
+    n, B = 3, 64
+    st = ps.CliffordState(n, B, 1, True)
+    st.apply_hadamard_gate(0)
+    ob = W.OpList()
+    ob.measure(0)
+    with pytest.raises(RuntimeError, match="reset"):
+        st.run_program(ob.array(), "frame")
+    assert st.num_records == 0
+    st.reset_clifford_state()
+    bad = W.OpList()
+    m = bad.measure(0)
+    bad.flagged(W.OP_H_FLAG, 1, m)
+    with pytest.raises(RuntimeError, match="frame form"):
+        st.run_program(bad.array(), "frame")
+    # Pauli-only programs leave X/Z at |0...0>, so runs can follow each other
+    p = W.OpList()
+    p.gate(W.OP_X, 1)
+    p.noise(2, 0.3, 0.1, 0.1)
+    st.run_program(p.array(), "frame")
+    st.run_program(p.array(), "frame")
+    orc = O.OracleState(n, B, 1)
+    orc.run_program(p.array())
+    orc.run_program(p.array())
+    x, z, r = st.device_to_host_numpy()
+    assert np.array_equal(r, orc.r) and np.array_equal(x, orc.x) and np.array_equal(z, orc.z)
