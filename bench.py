@@ -373,6 +373,32 @@ def bench_random_n1024(args):
     kernel_ms = kms / max(1, launches)
     ms_step = 1e3 * wall / steps
     achieved = abytes["total"] / (kernel_ms / 1e3) / 1e9
+    progress(f"n1024: persistent {ms_step:.1f} ms/step")
+    # frame form (no flagged ops here, so X/Z are the same in every shot)
+    frame = {}
+    try:
+        t0 = time.perf_counter()
+        st.reset_clifford_state()
+        prog.run(st, "frame")  # compiles the plan on the host, then runs
+        st.synchronize()
+        frame["first_run_s"] = time.perf_counter() - t0
+        prog.kernel_ms()
+        prog.set_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st.reset_clifford_state()
+            prog.run(st, "frame")
+        st.synchronize()
+        fms = 1e3 * (time.perf_counter() - t0) / steps
+        fk, fl = prog.kernel_ms()
+        prog.set_timing(False)
+        frame.update({"ms_per_step": fms, "gate_shots_per_s": w.gates * B / (fms / 1e3),
+                      "meas_shots_per_s": w.measurements * B / (fms / 1e3),
+                      "kernel": "k_frame", "kernel_ms": fk / max(1, fl),
+                      "speedup_vs_persistent": ms_step / fms})
+        progress(f"n1024: frame {fms:.2f} ms/step")
+    except RuntimeError as e:
+        frame = {"error": str(e)}
     out = {
         "metric": "gate-shots/s and measurement-shots/s, random Clifford circuit n=1024",
         "value": w.gates * B / (ms_step / 1e3),
@@ -388,6 +414,7 @@ def bench_random_n1024(args):
                      "kernel_ms": kernel_ms, "algorithmic_bytes": abytes["total"]},
         "algorithmic_bytes": abytes,
         "census": census,
+        "frame_form": frame,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_prefix(w, args.cpu_seconds)

@@ -447,13 +447,15 @@ int build_frame_plan(pq_program_s *p) {
   F->lds_rec = R;
   F->lds_const = R + F->nref;
   F->lds_noise = F->lds_const + 1;
-  F->nb_words = kFrameNoiseWords;
+  // a window of up to kFrameNoiseWords / 2 noise ops (none without noise)
+  F->nb_words = (uint32_t)std::min<uint64_t>(kFrameNoiseWords, (2 * p->nnoise + 63) / 64 * 64);
   // the stream buffer holds the largest single measurement op
   uint32_t need = 64;
   for (const Logical &m : ops)
     if (m.kind != kFrameXor) need = std::max<uint32_t>(need, 8 + 4 + 1 + (uint32_t)m.rows.size());
   F->sb_words = std::max<uint32_t>(kFrameStreamWords, (need + 63) / 64 * 64);
-  const uint64_t lds = 8ull * (F->lds_noise + F->nb_words) + 4ull * F->sb_words;
+  // ... and 64 words of partial results per wave (up to 16 waves)
+  const uint64_t lds = 8ull * (F->lds_noise + F->nb_words) + 4ull * F->sb_words + 8ull * 64 * 16;
   if (lds > kFrameLdsMax)
     return pq_set_error(PQ_ERR_UNSUPPORTED,
                         ("frame form needs " + std::to_string(lds) +
@@ -555,6 +557,7 @@ int run_frame(pq_state s, pq_program_s *p) {
   a.lds_const = F.lds_const;
   a.lds_noise = F.lds_noise;
   a.nb_words = F.nb_words;
+  a.sb_words = F.sb_words;
   a.lds_bytes = F.lds_bytes;
   if (p->timing_on) {
     if (p->ev_used + 2 > p->ev.size()) {

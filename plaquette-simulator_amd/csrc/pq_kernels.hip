@@ -19,6 +19,15 @@ constexpr uint32_t kMeasMaxQubits = 4095;
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
+// draw_keyed(key, 64 w + lane) with the product split as
+// (64 w + lane + 1) G = 64 w G + (lane + 1) G: the first term is wave-uniform
+// (scalar) and lane_g = (lane + 1) G is computed once per thread, so a draw
+// costs one 64-bit add and the SplitMix64 output function.
+__device__ __forceinline__ uint64_t lane_golden() { return (uint64_t)(lane_id() + 1) * kGolden; }
+__device__ __forceinline__ uint64_t draw_word_lane(uint64_t key, uint64_t w, uint64_t lane_g) {
+  return mix64(key + (64 * w) * kGolden + lane_g);
+}
+
 // ---------------------------------------------------------------------------
 // Init: word (c, row) of every group is all-ones iff row == c, which is the
 // identity tableau with r = 1 on the scratch row (MiscFunctors.hpp:22-27).
@@ -155,7 +164,7 @@ __global__ __launch_bounds__(256) void k_noise_planes(
     const uint64_t w = p - k * W;
     const NoiseDesc d = desc ? desc[k] : one;
     bool bx, bz;
-    noise_bits(d, draw_keyed(d.key, 64 * w + lane), bx, bz);
+    noise_bits(d, draw_word_lane(d.key, w, lane_golden()), bx, bz);
     const uint64_t mx = __ballot(bx);  // X or Y (or the random flag)
     const uint64_t mz = __ballot(bz);  // Y or Z
     if (lane == 0) {
@@ -1522,13 +1531,14 @@ __global__ __launch_bounds__(256) void k_batch_noise(const pq::BatchCircuit *__r
   const uint64_t kstride = nwv / ww;
   if (!W || wave >= kstride * ww) return;
   const uint32_t lane = lane_id();
+  const uint64_t lane_g = lane_golden();
   uint64_t *planes = const_cast<uint64_t *>(bc.a.planes);
   for (uint64_t k = wave / ww; k < bc.nnoise; k += kstride) {
     const NoiseDesc d = bc.noise[k];
     const uint64_t key = stream_key(bc.seed, 2 * (bc.nseq0 + k) + 1);
     for (uint64_t w = wave % ww; w < W; w += ww) {
       bool bx, bz;
-      noise_bits(d, draw_keyed(key, 64 * w + lane), bx, bz);
+      noise_bits(d, draw_word_lane(key, w, lane_g), bx, bz);
       const uint64_t mx = __ballot(bx);
       const uint64_t mz = __ballot(bz);
       if (lane == 0) {
@@ -1550,6 +1560,7 @@ __global__ __launch_bounds__(256) void k_program_noise(
   // blockIdx.y strides over noise ops (one stream key per op), the waves of
   // blockIdx.x over word groups: one draw per lane per (op, word group)
   const uint32_t lane = lane_id();
+  const uint64_t lane_g = lane_golden();
   const uint64_t seed = pp->seed, nseq0 = pp->nseq0;
   const uint64_t wv = blockIdx.x * (uint64_t)(blockDim.x / 64) + (threadIdx.x >> 6);
   const uint64_t nwv = (uint64_t)gridDim.x * (blockDim.x / 64);
@@ -1559,7 +1570,7 @@ __global__ __launch_bounds__(256) void k_program_noise(
     uint64_t *px = planes + (2ull * k) * W, *pz = px + W;
     for (uint64_t w = wv; w < W; w += nwv) {
       bool bx, bz;
-      noise_bits(d, draw_keyed(key, 64 * w + lane), bx, bz);
+      noise_bits(d, draw_word_lane(key, w, lane_g), bx, bz);
       const uint64_t mx = __ballot(bx);
       const uint64_t mz = __ballot(bz);
       if (lane == 0) {
@@ -1609,34 +1620,48 @@ typedef __attribute__((address_space(3))) void lds_void;
 
 __device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
-__global__ __launch_bounds__(64) void k_frame(FrameArgs a, const ProgParams *__restrict__ pp) {
+// The kFrameThreads threads of a block share one word group: its R column,
+// record words, the noise-mask window and the stream chunk live in LDS, the
+// threads split each op's rows (ops touch distinct rows, so within an op no
+// two threads write one word) and a barrier separates ops.
+#ifndef PQ_FRAME_THREADS
+#define PQ_FRAME_THREADS 256
+#endif
+constexpr uint32_t kFrameThreads = PQ_FRAME_THREADS;
+constexpr uint32_t kFrameWaves = kFrameThreads / 64;
+
+__global__ __launch_bounds__(kFrameThreads) void k_frame(FrameArgs a,
+                                                         const ProgParams *__restrict__ pp) {
   extern __shared__ uint64_t dyn[];
   const uint64_t w = blockIdx.x;
-  const uint32_t lane = lane_id();
+  const uint32_t tid = threadIdx.x, lane = lane_id();
+  const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const uint32_t n = a.n, R = a.R;
   uint64_t *Rl = dyn;
   uint64_t *nbuf = dyn + a.lds_noise;
   uint32_t *sbuf = reinterpret_cast<uint32_t *>(dyn + a.lds_noise + a.nb_words);
+  uint64_t *part = reinterpret_cast<uint64_t *>(sbuf + a.sb_words);  // [waves][64]
   uint64_t *Tr = a.tab + (w * a.C + 2ull * n) * R;  // the group's R column
-  for (uint32_t i = lane; i < R; i += 64) Rl[i] = Tr[i];
-  for (uint32_t i = lane; i < a.npreload; i += 64)
+  for (uint32_t i = tid; i < R; i += kFrameThreads) Rl[i] = Tr[i];
+  for (uint32_t i = tid; i < a.npreload; i += kFrameThreads)
     dyn[a.lds_rec + a.preload[2 * i + 1]] = a.recs[(2ull * a.preload[2 * i]) * a.W + w];
-  if (lane == 0) dyn[a.lds_const] = ~0ull;
+  if (tid == 0) dyn[a.lds_const] = ~0ull;
   const uint64_t seed = pp->seed, mseq0 = pp->mseq0;
   for (uint32_t c = 0; c < a.nchunk; ++c) {
     const uint32_t cb = a.chunks[4 * c], cw = a.chunks[4 * c + 1];
     const uint32_t k0 = a.chunks[4 * c + 2], nk = a.chunks[4 * c + 3];
-    // the last chunk's LDS reads are done before its buffers are refilled
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    // every thread is done with the last chunk's buffers
+    __syncthreads();
     // stream words and the chunk's noise masks (planes 2k0 .. 2(k0+nk)-1 of
-    // this group), by LDS DMA: one wave-wide 4-byte load per 64 dwords;
-    // lanes past the end reload the last dword into the buffer's padding
-    for (uint32_t i = 0; i < cw; i += 64) {
+    // this group) by LDS DMA, one wave-wide 4-byte load per 64 dwords, the
+    // waves taking turns; lanes past the end reload the last dword into the
+    // buffer's padding
+    for (uint32_t i = wv * 64; i < cw; i += kFrameThreads) {
       const uint32_t j = min(i + lane, cw - 1);
       __builtin_amdgcn_global_load_lds(a.stream + cb + j, (lds_void *)(sbuf + i), 4, 0, 0);
     }
     const uint32_t dw = 4 * nk;
-    for (uint32_t i = 0; i < dw; i += 64) {
+    for (uint32_t i = wv * 64; i < dw; i += kFrameThreads) {
       const uint32_t t = min(i + lane, dw - 1);
       const uint32_t *src =
           reinterpret_cast<const uint32_t *>(a.planes + (2ull * k0 + (t >> 1)) * a.W + w) +
@@ -1645,67 +1670,90 @@ __global__ __launch_bounds__(64) void k_frame(FrameArgs a, const ProgParams *__r
                                        4, 0, 0);
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
     uint32_t pos = 0;
     while (pos < cw) {
       const uint32_t kind = rfl(sbuf[pos]);
       const uint32_t h1 = rfl(sbuf[pos + 1]);
       if (kind == kFrameXor) {
-        // R[row] ^= XOR of the LDS mask words its entries name; rows distinct
+        // R[row] ^= XOR of the LDS mask words its entries name
         const uint32_t T = h1, ne = rfl(sbuf[pos + 2]);
         const uint32_t *rows = sbuf + pos + 8;
         const uint32_t *offs = rows + T;
         const uint32_t *ents = offs + T + 1;
-        for (uint32_t t = lane; t < T; t += 64) {
+        for (uint32_t t = tid; t < T; t += kFrameThreads) {
           const uint32_t row = rows[t], e1 = offs[t + 1];
           uint64_t acc = 0;
-          for (uint32_t e = offs[t]; e < e1; ++e) acc ^= dyn[ents[e]];
+          uint32_t e = offs[t];
+          for (; e + 4 <= e1; e += 4) {
+            const uint32_t i0 = ents[e], i1 = ents[e + 1], i2 = ents[e + 2], i3 = ents[e + 3];
+            acc ^= (dyn[i0] ^ dyn[i1]) ^ (dyn[i2] ^ dyn[i3]);
+          }
+          for (; e < e1; ++e) acc ^= dyn[ents[e]];
           Rl[row] ^= acc;
         }
         pos += 8 + 2 * T + 1 + ne;
+        __syncthreads();
       } else if (kind == kFrameRandom) {
-        // MeasureRandom (MeasurementFunctors.hpp:225-265) with pivot p
+        // MeasureRandom (MeasurementFunctors.hpp:225-265) with pivot p: the
+        // rowsum rows first (all threads read R[n+p] before it changes),
+        // then R[p], R[n+p] and the record
         const uint32_t p = h1, slot = rfl(sbuf[pos + 2]), mi = rfl(sbuf[pos + 3]);
         const uint32_t Tc = rfl(sbuf[pos + 4]), ridx = rfl(sbuf[pos + 5]);
         const uint32_t ne = rfl(sbuf[pos + 6]);
         const uint64_t Rs = Rl[n + p];
-        const uint64_t key = stream_key(seed, 2 * (mseq0 + mi));
-        const uint32_t u = (uint32_t)(draw_keyed(key, 64 * w + lane) >> 40);
-        const uint64_t coin = __ballot(u < Tc);
         const uint32_t *ents = sbuf + pos + 8;
-        for (uint32_t e = lane; e < ne; e += 64) {
+        for (uint32_t e = tid; e < ne; e += kFrameThreads) {
           const uint32_t v = ents[e];
           Rl[v & 0x7FFFFFFFu] ^= Rs ^ ((v >> 31) ? ~0ull : 0ull);
         }
-        if (lane == 0) {
-          Rl[p] = Rs;
-          Rl[n + p] = coin;
-          a.recs[(2ull * slot) * a.W + w] = coin;
-          a.recs[(2ull * slot + 1) * a.W + w] = 0ull;
-          if (ridx != kNone) dyn[a.lds_rec + ridx] = coin;
+        __syncthreads();
+        if (wv == 0) {
+          const uint64_t key = stream_key(seed, 2 * (mseq0 + mi));
+          const uint32_t u = (uint32_t)(draw_keyed(key, 64 * w + lane) >> 40);
+          const uint64_t coin = __ballot(u < Tc);
+          if (lane == 0) {
+            Rl[p] = Rs;
+            Rl[n + p] = coin;
+            a.recs[(2ull * slot) * a.W + w] = coin;
+            a.recs[(2ull * slot + 1) * a.W + w] = 0ull;
+            if (ridx != kNone) dyn[a.lds_rec + ridx] = coin;
+          }
         }
         pos += 8 + ne;
+        __syncthreads();
       } else {
-        // G determined measurements (MeasureDetermined, :189-201), one per
-        // lane; their rows exclude the scratch row, which takes the last
+        // G determined measurements (MeasureDetermined, :189-201): lane j of
+        // every wave XORs every kFrameWaves-th row of measurement j, wave 0
+        // combines; their rows exclude the scratch row, which takes the last
         const uint32_t G = h1;
         const uint32_t *desc = sbuf + pos + 8;
         const uint32_t *rows = desc + 4 * G + 1;
         if (lane < G) {
-          const uint32_t slot = desc[4 * lane], ridx = desc[4 * lane + 1];
-          const uint32_t o0 = desc[4 * lane + 3];
           const uint32_t o1 = lane + 1 < G ? desc[4 * lane + 7] : desc[4 * G];
+          uint64_t acc = 0;
+          for (uint32_t e = desc[4 * lane + 3] + wv; e < o1; e += kFrameWaves)
+            acc ^= Rl[rows[e]];
+          part[wv * 64 + lane] = acc;
+        }
+        __syncthreads();
+        if (wv == 0 && lane < G) {
           uint64_t acc = desc[4 * lane + 2] ? ~0ull : 0ull;
-          for (uint32_t e = o0; e < o1; ++e) acc ^= Rl[rows[e]];
+#pragma unroll
+          for (uint32_t v = 0; v < kFrameWaves; ++v) acc ^= part[v * 64 + lane];
+          const uint32_t slot = desc[4 * lane], ridx = desc[4 * lane + 1];
           a.recs[(2ull * slot) * a.W + w] = acc;
           a.recs[(2ull * slot + 1) * a.W + w] = ~0ull;
           if (ridx != kNone) dyn[a.lds_rec + ridx] = acc;
           if (lane == G - 1) Rl[2 * n] = acc;
         }
         pos += 8 + 4 * G + 1 + rfl(desc[4 * G]);
+        __syncthreads();
       }
     }
   }
-  for (uint32_t i = lane; i < R; i += 64) Tr[i] = Rl[i];
+  __syncthreads();
+  for (uint32_t i = tid; i < R; i += kFrameThreads) Tr[i] = Rl[i];
 }
 
 // Shared X/Z (bit rows [R][nw]) into columns 0..2n-1 of every word group.
@@ -1877,7 +1925,7 @@ hipError_t launch_frame(hipStream_t s, const FrameArgs &a, const void *params) {
   if (!a.W) return hipSuccess;
   hipError_t e = ensure_lds_attr((const void *)k_frame, a.lds_bytes);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_frame, dim3((uint32_t)a.W), dim3(64), a.lds_bytes, s, a,
+  hipLaunchKernelGGL(k_frame, dim3((uint32_t)a.W), dim3(kFrameThreads), a.lds_bytes, s, a,
                      (const ProgParams *)params);
   return hipGetLastError();
 }

@@ -73,13 +73,14 @@ def kernels(d, out):
         trace = list(csv.DictReader(open(_find(d, "*kernel_trace.csv"))))
     except (FileNotFoundError, StopIteration, IndexError):
         trace = []
-    durs = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
-            for r in sorted(trace, key=lambda r: int(r["Start_Timestamp"]))
-            if "k_program" in r["Kernel_Name"] and "noise" not in r["Kernel_Name"]]
-    if durs:
-        srt = sorted(durs)
-        lines += ["", "k_program launches in order (us): " + ", ".join("%.1f" % v for v in durs),
-                  "k_program median: %.1f us over %d launches" % (srt[len(srt) // 2], len(srt))]
+    for kname in ("k_program", "k_frame"):
+        durs = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+                for r in sorted(trace, key=lambda r: int(r["Start_Timestamp"]))
+                if kname + "_t" in r["Kernel_Name"] or kname + "(" in r["Kernel_Name"]]
+        if durs:
+            srt = sorted(durs)
+            lines += ["", "%s launches in order (us): " % kname + ", ".join("%.1f" % v for v in durs),
+                      "%s median: %.1f us over %d launches" % (kname, srt[len(srt) // 2], len(srt))]
     with open(out, "w") as f:
         f.write("\n".join(lines) + "\n")
     print("\n".join(lines))
