@@ -1,11 +1,12 @@
 // run_circuit: time one circuit file on the GPU, as the reference's
 // benchmarks/run_circuit.cpp:10-32 times CliffordCircuit::Execute.
 //
-//   run_circuit <file_name> <n_qubits> <batch_size> [plain|persistent|graph]
+//   run_circuit <file_name> <n_qubits> <batch_size> [plain|persistent|graph|frame]
 //
 // Prints "<file_name> <seconds>".  The timed region is Execute() plus the
 // wait for its kernels (the form defaults to one persistent kernel; `plain`
-// issues one launch per op like the reference).
+// issues one launch per op like the reference; `frame` runs X/Z once on the
+// host, which the ring circuits allow: they have no flagged gates).
 #include <chrono>
 #include <cstdlib>
 #include <iostream>
@@ -16,7 +17,7 @@
 int main(int argc, char *argv[]) {
   using namespace Plaquette;
   if (argc != 4 && argc != 5) {
-    std::cout << "Usage: " << argv[0] << " <file_name> <n_qubits> <batch_size> [plain|persistent|graph]"
+    std::cout << "Usage: " << argv[0] << " <file_name> <n_qubits> <batch_size> [plain|persistent|graph|frame]"
               << std::endl;
     return 1;
   }
@@ -28,6 +29,7 @@ int main(int argc, char *argv[]) {
     const std::string f = argv[4];
     if (f == "plain") form = PQ_RUN_PLAIN;
     else if (f == "graph") form = PQ_RUN_GRAPH;
+    else if (f == "frame") form = PQ_RUN_FRAME;
     else if (f != "persistent") {
       std::cerr << "unknown form " << f << std::endl;
       return 1;

@@ -5,7 +5,8 @@ The file holds one operation per line, ``NAME q...`` with NAME in
 H S X Y Z CX CZ M.  The whole file is parsed once by the engine's own parser
 into an op list, and ``execute()`` runs it as one program on the GPU (one
 persistent kernel by default, ``form="plain"`` for one launch per op,
-``form="graph"`` for a hipGraph replay).  Results and record ids are those of
+``form="graph"`` for a hipGraph replay, ``form="frame"`` for the frame form,
+which runs the shared X/Z once on the host).  Results and record ids are those of
 the same operations issued one by one: each ``M`` line appends one record per
 listed qubit (bias 1/2).
 

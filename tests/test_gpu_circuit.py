@@ -44,9 +44,9 @@ def assert_same(st, orc):
         assert np.array_equal(gres, res) and np.array_equal(gdet, det), f"record {i}"
 
 
-@pytest.mark.parametrize("form", ["persistent", "plain"])
+@pytest.mark.parametrize("form", ["persistent", "plain", "frame"])
 def test_ring_circuits_match_oracle(ps, form):
-    names = sorted(RING) if form == "persistent" else sorted(RING)[::6]
+    names = sorted(RING) if form in ("persistent", "frame") else sorted(RING)[::6]
     for name in names:
         n = int(name.split("_")[2])
         B, seed = 1000, zlib.crc32(name.encode()) & 0x7FFFFFFF
@@ -106,7 +106,7 @@ def test_run_circuit_tool(ps, tmp_path):
     assert os.path.exists(TOOL), "build the engine first"
     f = tmp_path / "width_depth_8_15.circuit"
     f.write_text(RING["width_depth_8_15.circuit"])
-    for form in ("persistent", "plain"):
+    for form in ("persistent", "plain", "frame"):
         out = subprocess.run([TOOL, str(f), "8", "4096", form], capture_output=True, text=True,
                              timeout=120)
         say(f"[run_circuit {form}] {out.stdout.strip()} {out.stderr.strip()}")
