@@ -120,6 +120,65 @@ struct XZ {
   }
 };
 
+// LSB-first 64x64 bit-matrix transpose in place: bit b of a[k] <-> bit k of
+// a[b] (swapping ever smaller off-diagonal blocks).
+void transpose64(uint64_t a[64]) {
+  static const uint64_t kMask[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull,
+                                    0x00FF00FF00FF00FFull, 0x0F0F0F0F0F0F0F0Full,
+                                    0x3333333333333333ull, 0x5555555555555555ull};
+  for (int s = 0, j = 32; j; j >>= 1, ++s)
+    for (int k = 0; k < 64; ++k)
+      if (!(k & j)) {
+        const uint64_t t = ((a[k] >> j) ^ a[k | j]) & kMask[s];
+        a[k] ^= t << j;
+        a[k | j] ^= t;
+      }
+}
+
+// The same X/Z by columns: X[q] is the set of rows with x_q (bit i of word
+// i / 64), so a gate is a few word operations over rows.
+struct Cols {
+  uint32_t n = 0, R = 0, rw = 0;
+  std::vector<uint64_t> X, Z;  // [n][rw]
+  uint64_t *x(uint32_t q) { return &X[(size_t)q * rw]; }
+  uint64_t *z(uint32_t q) { return &Z[(size_t)q * rw]; }
+};
+
+// rows [R][nw] <-> columns [n][rw] by 64x64 blocks
+void rows_to_cols(const std::vector<uint64_t> &rows, uint32_t R, uint32_t n, uint32_t nw,
+                  std::vector<uint64_t> &cols, uint32_t rw) {
+  cols.assign((size_t)n * rw, 0);
+  uint64_t a[64];
+  for (uint32_t bi = 0; bi < rw; ++bi)
+    for (uint32_t bj = 0; bj < nw; ++bj) {
+      for (uint32_t k = 0; k < 64; ++k) {
+        const uint32_t row = bi * 64 + k;
+        a[k] = row < R ? rows[(size_t)row * nw + bj] : 0;
+      }
+      transpose64(a);
+      for (uint32_t k = 0; k < 64; ++k) {
+        const uint32_t q = bj * 64 + k;
+        if (q < n) cols[(size_t)q * rw + bi] = a[k];
+      }
+    }
+}
+void cols_to_rows(const std::vector<uint64_t> &cols, uint32_t R, uint32_t n, uint32_t rw,
+                  std::vector<uint64_t> &rows, uint32_t nw) {
+  uint64_t a[64];
+  for (uint32_t bj = 0; bj < nw; ++bj)
+    for (uint32_t bi = 0; bi < rw; ++bi) {
+      for (uint32_t k = 0; k < 64; ++k) {
+        const uint32_t q = bj * 64 + k;
+        a[k] = q < n ? cols[(size_t)q * rw + bi] : 0;
+      }
+      transpose64(a);
+      for (uint32_t k = 0; k < 64; ++k) {
+        const uint32_t row = bi * 64 + k;
+        if (row < R) rows[(size_t)row * nw + bj] = a[k];
+      }
+    }
+}
+
 // Mask source of an XOR entry: the constant all-ones word, a record's
 // result word (by rec_l index) or a noise mask part (2k + part: part 0 is
 // the X-or-Y mask, also a random flag; part 1 the Y-or-Z mask).
@@ -144,40 +203,147 @@ int simulate(pq_program_s *p, FramePlan &F, std::vector<Logical> &out) {
   const uint32_t n = (uint32_t)p->n, R = 2 * n + 1;
   XZ t;
   t.init_identity(n);
-  // records the flag ops read: rec_l index per slot
-  std::map<uint32_t, uint32_t> ridx_of;
+  // LDS words for record values that flag ops read.  A value lives from the
+  // M that writes it (or the program start) to its last reader; words are
+  // reused once a value is dead, so the count is the most values live at
+  // once, not the number of records.  op_ridx[i]: the word op i writes (M)
+  // or reads (flag), kNone if none.
+  const size_t nops = p->ops.size();
+  std::vector<uint32_t> op_ridx(nops, kNone);
   {
-    std::map<uint32_t, bool> written;  // slot -> written by an earlier M
-    for (const pq_op &o : p->ops) {
+    auto is_rec_flag = [](const pq_op &o) {
+      return o.code >= PQ_OP_X_FLAG && o.code <= PQ_OP_Z_FLAG && o.slot != PQ_FLAG_RANDOM;
+    };
+    // last reader of each value: producer -1 = the value from before the program
+    std::map<uint32_t, int64_t> producer;           // slot -> op index of its current value
+    std::map<std::pair<uint32_t, int64_t>, int64_t> last_read;  // (slot, producer) -> reader
+    for (size_t i = 0; i < nops; ++i) {
+      const pq_op &o = p->ops[i];
       if (o.code == PQ_OP_M) {
-        written[o.slot] = true;
-      } else if (o.code >= PQ_OP_X_FLAG && o.code <= PQ_OP_Z_FLAG && o.slot != PQ_FLAG_RANDOM) {
-        if (!ridx_of.count(o.slot)) {
-          const uint32_t idx = (uint32_t)ridx_of.size();
-          ridx_of[o.slot] = idx;
-          if (!written.count(o.slot)) {
-            F.preload.push_back(o.slot);
-            F.preload.push_back(idx);
-          }
+        producer[o.slot] = (int64_t)i;
+      } else if (is_rec_flag(o)) {
+        auto it = producer.find(o.slot);
+        last_read[{o.slot, it == producer.end() ? -1 : it->second}] = (int64_t)i;
+      }
+    }
+    std::vector<uint32_t> free_words;
+    uint32_t high = 0;
+    auto take = [&]() {
+      if (!free_words.empty()) {
+        const uint32_t w = free_words.back();
+        free_words.pop_back();
+        return w;
+      }
+      return high++;
+    };
+    std::map<uint32_t, uint32_t> word_of;  // slot -> word of its live value
+    for (const auto &kv : last_read)
+      if (kv.first.second < 0) {  // read before any M of this program writes it
+        const uint32_t w = take();
+        word_of[kv.first.first] = w;
+        F.preload.push_back(kv.first.first);
+        F.preload.push_back(w);
+      }
+    producer.clear();
+    for (size_t i = 0; i < nops; ++i) {
+      const pq_op &o = p->ops[i];
+      if (o.code == PQ_OP_M) {
+        producer[o.slot] = (int64_t)i;
+        word_of.erase(o.slot);
+        if (last_read.count({o.slot, (int64_t)i})) {
+          op_ridx[i] = take();
+          word_of[o.slot] = op_ridx[i];
+        }
+      } else if (is_rec_flag(o)) {
+        auto it = producer.find(o.slot);
+        const int64_t prod = it == producer.end() ? -1 : it->second;
+        op_ridx[i] = word_of.at(o.slot);
+        if (last_read.at({o.slot, prod}) == (int64_t)i) {  // the value's last reader
+          free_words.push_back(op_ridx[i]);
+          word_of.erase(o.slot);
         }
       }
     }
+    F.nref = high;
   }
-  F.nref = (uint32_t)ridx_of.size();
-  std::vector<uint8_t> flip(R, 0);
+  // Between measurements X/Z are held by columns (gates are word operations
+  // over rows); measurements work on rows.
+  Cols cl;
+  cl.n = n;
+  cl.R = R;
+  cl.rw = (R + 63) / 64;
+  const uint32_t rw = cl.rw;
+  bool by_cols = false;
+  auto need_cols = [&]() {
+    if (by_cols) return;
+    rows_to_cols(t.X, R, n, t.nw, cl.X, rw);
+    rows_to_cols(t.Z, R, n, t.nw, cl.Z, rw);
+    by_cols = true;
+  };
+  auto need_rows = [&]() {
+    if (!by_cols) return;
+    cols_to_rows(cl.X, R, n, rw, t.X, t.nw);
+    cols_to_rows(cl.Z, R, n, rw, t.Z, t.nw);
+    by_cols = false;
+  };
+  std::vector<uint64_t> flip(rw, 0);  // rows whose r the gates so far flip
+  const uint64_t last_mask = (R % 64) ? ((1ull << (R % 64)) - 1) : ~0ull;
   Logical cur;
   cur.kind = kFrameXor;
-  auto close_xor = [&]() {
-    for (uint32_t i = 0; i < R; ++i)
-      if (flip[i]) {
-        cur.ents.push_back(Entry{i, kSrcConst, 0});
-        flip[i] = 0;
+  // rows set in `bits` (rw words) each get mask source (src, arg)
+  auto add_rows = [&](const uint64_t *bits, uint32_t src, uint32_t arg) {
+    for (uint32_t j = 0; j < rw; ++j) {
+      uint64_t v = bits[j] & (j + 1 == rw ? last_mask : ~0ull);
+      while (v) {
+        const uint32_t b = (uint32_t)__builtin_ctzll(v);
+        v &= v - 1;
+        cur.ents.push_back(Entry{j * 64 + b, src, arg});
       }
+    }
+  };
+  auto close_xor = [&]() {
+    add_rows(flip.data(), kSrcConst, 0);
+    std::fill(flip.begin(), flip.end(), 0);
     if (!cur.ents.empty()) out.push_back(std::move(cur));
     cur = Logical();
     cur.kind = kFrameXor;
   };
+  std::vector<uint64_t> tmp(rw), tmp2(rw), sel(rw);
+  bool seg_rows = false;  // the current gate segment runs row by row
+  // Paulis, noise and flagged Paulis on qubit q, given the rows with z_q
+  // (zr) and with x_q (xr): X flips r where z_q, Z where x_q, Y where one of
+  // them (GateFunctors.hpp:304-307, 324-327, 373-376)
   uint32_t kk = 0, mi = 0;
+  auto apply_pauli_like = [&](const pq_op &o, size_t oi, const uint64_t *zr, const uint64_t *xr) {
+    switch (o.code) {
+      case PQ_OP_X:
+        for (uint32_t j = 0; j < rw; ++j) flip[j] ^= zr[j];
+        break;
+      case PQ_OP_Y:
+        for (uint32_t j = 0; j < rw; ++j) flip[j] ^= zr[j] ^ xr[j];
+        break;
+      case PQ_OP_Z:
+        for (uint32_t j = 0; j < rw; ++j) flip[j] ^= xr[j];
+        break;
+      case PQ_OP_NOISE: {
+        const uint32_t k = kk++;
+        add_rows(zr, kSrcNoise, 2 * k);      // X or Y part
+        add_rows(xr, kSrcNoise, 2 * k + 1);  // Y or Z part
+        break;
+      }
+      default: {  // X/Y/Z_FLAG
+        uint32_t src = kSrcRec, arg = op_ridx[oi];
+        if (o.slot == PQ_FLAG_RANDOM) {
+          src = kSrcNoise;
+          arg = 2 * kk++;
+        }
+        for (uint32_t j = 0; j < rw; ++j)
+          sel[j] = o.code == PQ_OP_X_FLAG ? zr[j] : o.code == PQ_OP_Z_FLAG ? xr[j] : (zr[j] ^ xr[j]);
+        add_rows(sel.data(), src, arg);
+        break;
+      }
+    }
+  };
   for (size_t oi = 0; oi < p->ops.size(); ++oi) {
     const pq_op &o = p->ops[oi];
     const uint32_t q = o.q0, q1 = o.q1;
@@ -185,83 +351,117 @@ int simulate(pq_program_s *p, FramePlan &F, std::vector<Logical> &out) {
       return pq_set_error(PQ_ERR_UNSUPPORTED,
                           ("frame form: op " + std::to_string(oi) +
                            " is a flagged H/S/CX/CZ (X/Z would differ between shots)").c_str());
-    switch (o.code) {
-      case PQ_OP_H:
-        for (uint32_t i = 0; i < R; ++i) {
-          const bool x = t.x(i, q), z = t.z(i, q);
-          flip[i] ^= (uint8_t)(x & z);
-          t.set(t.X, i, q, z);
-          t.set(t.Z, i, q, x);
-        }
-        break;
-      case PQ_OP_S:
-        for (uint32_t i = 0; i < R; ++i) {
-          const bool x = t.x(i, q), z = t.z(i, q);
-          flip[i] ^= (uint8_t)(x & z);
-          t.set(t.Z, i, q, z ^ x);
-        }
-        break;
-      case PQ_OP_CX:
-        for (uint32_t i = 0; i < R; ++i) {
-          const bool xc = t.x(i, q), zc = t.z(i, q), xt = t.x(i, q1), zt = t.z(i, q1);
-          flip[i] ^= (uint8_t)(xc & zt & !(xt ^ zc));
-          t.set(t.X, i, q1, xt ^ xc);
-          t.set(t.Z, i, q, zc ^ zt);
-        }
-        break;
-      case PQ_OP_CZ:
-        for (uint32_t i = 0; i < R; ++i) {
-          const bool xc = t.x(i, q), zc = t.z(i, q), xt = t.x(i, q1), zt = t.z(i, q1);
-          flip[i] ^= (uint8_t)(xc & xt & (zt ^ zc));
-          t.set(t.Z, i, q1, zt ^ xc);
-          t.set(t.Z, i, q, zc ^ xt);
-        }
-        break;
-      case PQ_OP_X:
-        for (uint32_t i = 0; i < R; ++i) flip[i] ^= (uint8_t)t.z(i, q);
-        break;
-      case PQ_OP_Y:
-        for (uint32_t i = 0; i < R; ++i) flip[i] ^= (uint8_t)(t.z(i, q) ^ t.x(i, q));
-        break;
-      case PQ_OP_Z:
-        for (uint32_t i = 0; i < R; ++i) flip[i] ^= (uint8_t)t.x(i, q);
-        break;
-      case PQ_OP_NOISE: {
-        const uint32_t k = kk++;
-        for (uint32_t i = 0; i < R; ++i) {
-          if (t.z(i, q)) cur.ents.push_back(Entry{i, kSrcNoise, 2 * k});
-          if (t.x(i, q)) cur.ents.push_back(Entry{i, kSrcNoise, 2 * k + 1});
-        }
-        break;
-      }
-      case PQ_OP_X_FLAG:
-      case PQ_OP_Y_FLAG:
-      case PQ_OP_Z_FLAG: {
-        Entry e{0, kSrcRec, 0};
-        if (o.slot == PQ_FLAG_RANDOM) {
-          e.src = kSrcNoise;
-          e.arg = 2 * kk++;
-        } else {
-          e.arg = ridx_of.at(o.slot);
-        }
-        for (uint32_t i = 0; i < R; ++i) {
-          const bool x = t.x(i, q), z = t.z(i, q);
-          const bool sel = o.code == PQ_OP_X_FLAG ? z : (o.code == PQ_OP_Z_FLAG ? x : (x ^ z));
-          if (sel) {
-            e.row = i;
-            cur.ents.push_back(e);
+    if (o.code != PQ_OP_M && !by_cols && !seg_rows) {
+      // a new gate segment: by columns when its gates outweigh the two
+      // transposes (64x64 blocks of X and Z, there and back)
+      uint64_t g = 0;
+      for (size_t k = oi; k < nops && p->ops[k].code != PQ_OP_M; ++k) ++g;
+      const uint64_t by_row = g * R * 8, by_col = g * rw * 8 + 4ull * rw * t.nw * 600;
+      if (by_col < by_row)
+        need_cols();
+      else
+        seg_rows = true;
+    }
+    if (o.code == PQ_OP_M) seg_rows = false;
+    if (seg_rows) {
+      // the same updates row by row
+      auto bit = [&](uint32_t i) { flip[i / 64] ^= 1ull << (i % 64); };
+      switch (o.code) {
+        case PQ_OP_H:
+          for (uint32_t i = 0; i < R; ++i) {
+            const bool x = t.x(i, q), z = t.z(i, q);
+            if (x & z) bit(i);
+            t.set(t.X, i, q, z);
+            t.set(t.Z, i, q, x);
           }
+          break;
+        case PQ_OP_S:
+          for (uint32_t i = 0; i < R; ++i) {
+            const bool x = t.x(i, q), z = t.z(i, q);
+            if (x & z) bit(i);
+            t.set(t.Z, i, q, z ^ x);
+          }
+          break;
+        case PQ_OP_CX:
+          for (uint32_t i = 0; i < R; ++i) {
+            const bool xc = t.x(i, q), zc = t.z(i, q), xt = t.x(i, q1), zt = t.z(i, q1);
+            if (xc & zt & !(xt ^ zc)) bit(i);
+            t.set(t.X, i, q1, xt ^ xc);
+            t.set(t.Z, i, q, zc ^ zt);
+          }
+          break;
+        case PQ_OP_CZ:
+          for (uint32_t i = 0; i < R; ++i) {
+            const bool xc = t.x(i, q), zc = t.z(i, q), xt = t.x(i, q1), zt = t.z(i, q1);
+            if (xc & xt & (zt ^ zc)) bit(i);
+            t.set(t.Z, i, q1, zt ^ xc);
+            t.set(t.Z, i, q, zc ^ xt);
+          }
+          break;
+        case PQ_OP_X: case PQ_OP_Y: case PQ_OP_Z: case PQ_OP_NOISE:
+        case PQ_OP_X_FLAG: case PQ_OP_Y_FLAG: case PQ_OP_Z_FLAG: {
+          // rows by z_q (X part) and x_q (Z part) as column bitsets
+          std::fill(tmp.begin(), tmp.end(), 0);
+          std::fill(tmp2.begin(), tmp2.end(), 0);
+          for (uint32_t i = 0; i < R; ++i) {
+            if (t.z(i, q)) tmp[i / 64] |= 1ull << (i % 64);
+            if (t.x(i, q)) tmp2[i / 64] |= 1ull << (i % 64);
+          }
+          apply_pauli_like(o, oi, tmp.data(), tmp2.data());
+          break;
+        }
+        default:
+          return pq_set_error(PQ_ERR_ARG, "frame form: unknown op code");
+      }
+      continue;
+    }
+    switch (o.code) {
+      case PQ_OP_H: {  // GateFunctors.hpp:90-93
+        uint64_t *x = cl.x(q), *z = cl.z(q);
+        for (uint32_t j = 0; j < rw; ++j) {
+          flip[j] ^= x[j] & z[j];
+          std::swap(x[j], z[j]);
         }
         break;
       }
+      case PQ_OP_S: {  // :151-170
+        uint64_t *x = cl.x(q), *z = cl.z(q);
+        for (uint32_t j = 0; j < rw; ++j) {
+          flip[j] ^= x[j] & z[j];
+          z[j] ^= x[j];
+        }
+        break;
+      }
+      case PQ_OP_CX: {  // :248-252
+        uint64_t *xc = cl.x(q), *zc = cl.z(q), *xt = cl.x(q1), *zt = cl.z(q1);
+        for (uint32_t j = 0; j < rw; ++j) {
+          flip[j] ^= xc[j] & zt[j] & ~(xt[j] ^ zc[j]);
+          xt[j] ^= xc[j];
+          zc[j] ^= zt[j];
+        }
+        break;
+      }
+      case PQ_OP_CZ: {  // :30-35
+        uint64_t *xc = cl.x(q), *zc = cl.z(q), *xt = cl.x(q1), *zt = cl.z(q1);
+        for (uint32_t j = 0; j < rw; ++j) {
+          flip[j] ^= xc[j] & xt[j] & (zt[j] ^ zc[j]);
+          zt[j] ^= xc[j];
+          zc[j] ^= xt[j];
+        }
+        break;
+      }
+      case PQ_OP_X: case PQ_OP_Y: case PQ_OP_Z: case PQ_OP_NOISE:
+      case PQ_OP_X_FLAG: case PQ_OP_Y_FLAG: case PQ_OP_Z_FLAG:
+        apply_pauli_like(o, oi, cl.z(q), cl.x(q));
+        break;
       case PQ_OP_M: {
         close_xor();
+        need_rows();
         Logical m;
         m.slot = o.slot;
         m.mi = mi++;
         m.T = o.t0;
-        auto it = ridx_of.find(o.slot);
-        m.ridx = it == ridx_of.end() ? kNone : it->second;
+        m.ridx = op_ridx[oi];
         uint32_t piv = kNone;
         for (uint32_t i = 0; i < n; ++i)
           if (t.x(n + i, q)) {
@@ -300,6 +500,7 @@ int simulate(pq_program_s *p, FramePlan &F, std::vector<Logical> &out) {
     }
   }
   close_xor();
+  need_rows();
   if (kk != p->nnoise || mi != p->nmeas)
     return pq_set_error(PQ_ERR_HIP, "frame form: noise/measurement numbering mismatch");
   F.final_identity = t.is_identity();
