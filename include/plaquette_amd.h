@@ -324,6 +324,15 @@ int pq_server_destroy(pq_server sv);
 #define PQ_SERVE_FCFS 0
 #define PQ_SERVE_SMALLEST_FIRST 1
 int pq_server_set_policy(pq_server sv, int policy, double age_limit_s);
+/* Execution form of admitted requests: PQ_RUN_PERSISTENT (default) or
+ * PQ_RUN_FRAME, which runs each request whose ops allow it (no flagged
+ * H/S/CX/CZ) in the frame form and the others persistent; frame plans are
+ * shared by requests of the same structure (n and ops, noise thresholds
+ * aside).  Results are the same either way. */
+int pq_server_set_form(pq_server sv, int form);
+/* Requests run in the frame form, frame plans built, plans reused. */
+int pq_server_frame_stats(pq_server sv, uint64_t *frame_runs, uint64_t *plan_builds,
+                          uint64_t *plan_hits);
 
 /* ---- diagnostics: the reference's test-only functors ---- */
 /* RowMultiply(i <- i*k) on every shot (BatchRowMultiplyFunctor,

@@ -450,6 +450,27 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
             check(pq_server_set_policy(sv.s, p, age_limit_s), "Server.set_policy");
           },
           py::arg("policy"), py::arg("age_limit_s") = 0.0)
+      .def(
+          "set_form",
+          [](Server &sv, const std::string &form) {
+            int f = -1;
+            if (form == "persistent") f = PQ_RUN_PERSISTENT;
+            if (form == "frame") f = PQ_RUN_FRAME;
+            if (f < 0) throw std::runtime_error("Server.set_form: 'persistent' or 'frame'");
+            check(pq_server_set_form(sv.s, f), "Server.set_form");
+          },
+          py::arg("form"),
+          "'frame': requests whose ops allow it run in the frame form (plans shared)")
+      .def("frame_stats",
+           [](Server &sv) {
+             uint64_t runs = 0, builds = 0, hits = 0;
+             check(pq_server_frame_stats(sv.s, &runs, &builds, &hits), "Server.frame_stats");
+             py::dict d;
+             d["frame_runs"] = runs;
+             d["plan_builds"] = builds;
+             d["plan_hits"] = hits;
+             return d;
+           })
       .def("drain",
            [](Server &sv) {
              py::gil_scoped_release nogil;

@@ -37,6 +37,7 @@
 #include <algorithm>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -637,7 +638,7 @@ struct Packer {
 }  // namespace
 
 int build_frame_plan(pq_program_s *p) {
-  auto F = std::make_unique<FramePlan>();
+  auto F = std::make_shared<FramePlan>();
   const uint32_t n = (uint32_t)p->n, R = 2 * n + 1;
   F->R = R;
   std::vector<Logical> ops;
@@ -699,11 +700,26 @@ int build_frame_plan(pq_program_s *p) {
   return PQ_OK;
 }
 
+std::string frame_plan_key(uint64_t n, const pq_op *ops, uint64_t nops) {
+  std::string key(reinterpret_cast<const char *>(&n), sizeof(n));
+  key.reserve(sizeof(n) + nops * sizeof(pq_op));
+  for (uint64_t i = 0; i < nops; ++i) {
+    pq_op o = ops[i];
+    if (o.code != PQ_OP_M) o.t0 = o.t1 = o.t2 = 0;  // noise and random-flag thresholds
+    o.reserved = 0;
+    key.append(reinterpret_cast<const char *>(&o), sizeof(o));
+  }
+  return key;
+}
+
 int upload_frame_plan(pq_program_s *p, hipStream_t stream) {
   FramePlan &F = *p->frame;
+  std::lock_guard<std::mutex> g(F.upload_mu);
   if (F.d_stream) return PQ_OK;
+  // plain allocations: a plan may be shared by programs on several streams
+  // and outlive the program that built it
   auto up = [&](void **dst, const void *src, size_t bytes) -> hipError_t {
-    hipError_t e = dev_alloc(dst, std::max<size_t>(bytes, 8), p->alloc_stream);
+    hipError_t e = hipMalloc(dst, std::max<size_t>(bytes, 8));
     if (e != hipSuccess) return e;
     if (!bytes) return hipSuccess;
     return hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, stream);

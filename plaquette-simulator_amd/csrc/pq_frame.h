@@ -3,6 +3,8 @@
 
 #include <stdint.h>
 
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "pq_common.h"
@@ -29,8 +31,13 @@ struct FramePlan {
   uint64_t nrandom = 0, ndet = 0;
   uint32_t *d_stream = nullptr, *d_chunks = nullptr, *d_preload = nullptr;
   uint64_t *d_X = nullptr, *d_Z = nullptr;
+  std::mutex upload_mu;
   ~FramePlan();
 };
 
 int build_frame_plan(pq_program_s *p);
+// What a frame plan depends on, as bytes: n and the op list with the noise
+// thresholds left out (they live in the noise descriptors, not the plan).
+// Programs with equal keys can share one plan.
+std::string frame_plan_key(uint64_t n, const pq_op *ops, uint64_t nops);
 int run_frame(pq_state_s *s, pq_program_s *p);

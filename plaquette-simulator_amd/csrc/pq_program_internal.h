@@ -40,7 +40,7 @@ struct pq_program_s {
   bool timing_on = false;
   std::vector<hipEvent_t> ev;  // start/end pairs of timed kernel launches
   size_t ev_used = 0;
-  std::unique_ptr<FramePlan> frame;  // frame form plan, built on first use
+  std::shared_ptr<FramePlan> frame;  // frame form plan, built on first use (may be shared)
   void drop_graph();
   ~pq_program_s();
 };

@@ -67,6 +67,13 @@ struct pq_server_s {
   uint32_t max_running = 0;
   int policy = PQ_SERVE_FCFS;
   double age_limit = 0;
+  int form = PQ_RUN_PERSISTENT;
+  // frame plans by structure (frame_plan_key), built once and shared;
+  // plans_mu guards them and the counters (a build can take a while, so it
+  // is not `mu`, which submit and status take)
+  std::mutex plans_mu;
+  std::unordered_map<std::string, std::shared_ptr<FramePlan>> plans;
+  uint64_t plan_hits = 0, plan_builds = 0, frame_runs = 0;
   std::atomic<bool> stop{false};
   std::thread worker;
   void loop();
@@ -116,7 +123,25 @@ int pq_server_s::admit(Req *r) {
   const hipStream_t s = streams[r->slot];
   int rc = create_state_on(r->n, r->B, r->seed, s, &r->st);
   if (rc == PQ_OK) rc = create_program_on(r->st, r->ops.data(), r->ops.size(), &r->prog);
-  if (rc == PQ_OK) rc = pq_program_run(r->st, r->prog, PQ_RUN_PERSISTENT);
+  int run_form = PQ_RUN_PERSISTENT;
+  if (rc == PQ_OK && form == PQ_RUN_FRAME) {
+    // the frame form where the request allows it (no flagged H/S/CX/CZ),
+    // with plans shared between requests of the same structure
+    const std::string key = frame_plan_key(r->n, r->ops.data(), r->ops.size());
+    std::lock_guard<std::mutex> g(plans_mu);
+    auto it = plans.find(key);
+    if (it != plans.end()) {
+      r->prog->frame = it->second;
+      ++plan_hits;
+      run_form = PQ_RUN_FRAME;
+    } else if (build_frame_plan(r->prog) == PQ_OK) {
+      plans.emplace(key, r->prog->frame);
+      ++plan_builds;
+      run_form = PQ_RUN_FRAME;
+    }
+    if (run_form == PQ_RUN_FRAME) ++frame_runs;
+  }
+  if (rc == PQ_OK) rc = pq_program_run(r->st, r->prog, run_form);
   if (rc == PQ_OK && hipEventRecord(r->ev, s) != hipSuccess) rc = PQ_ERR_HIP;
   return rc;
 }
@@ -331,6 +356,25 @@ extern "C" int pq_server_set_policy(pq_server sv, int policy, double age_limit_s
   std::lock_guard<std::mutex> g(sv->mu);
   sv->policy = policy;
   sv->age_limit = age_limit_s;
+  return PQ_OK;
+}
+
+extern "C" int pq_server_set_form(pq_server sv, int form) {
+  if (!sv) return pq_set_error(PQ_ERR_ARG, "null server");
+  if (form != PQ_RUN_PERSISTENT && form != PQ_RUN_FRAME)
+    return pq_set_error(PQ_ERR_ARG, "server form must be PQ_RUN_PERSISTENT or PQ_RUN_FRAME");
+  std::lock_guard<std::mutex> g(sv->mu);
+  sv->form = form;
+  return PQ_OK;
+}
+
+extern "C" int pq_server_frame_stats(pq_server sv, uint64_t *frame_runs, uint64_t *plan_builds,
+                                     uint64_t *plan_hits) {
+  if (!sv) return pq_set_error(PQ_ERR_ARG, "null server");
+  std::lock_guard<std::mutex> g(sv->plans_mu);
+  if (frame_runs) *frame_runs = sv->frame_runs;
+  if (plan_builds) *plan_builds = sv->plan_builds;
+  if (plan_hits) *plan_hits = sv->plan_hits;
   return PQ_OK;
 }
 

@@ -48,13 +48,25 @@ def alone(ps, n, B, seed, ops):
     return [st.get_record(i) for i in range(st.num_records)]
 
 
-@pytest.mark.parametrize("policy", ["fcfs", "smallest_first"])
-def test_server_results_equal_standalone(ps, policy):
+@pytest.mark.parametrize("policy,form", [("fcfs", "persistent"), ("smallest_first", "persistent"),
+                                         ("fcfs", "frame"), ("smallest_first", "frame")])
+def test_server_results_equal_standalone(ps, policy, form):
+    """Every request's records equal its standalone persistent run, in both
+    forms; in the frame form requests of one distance share a plan."""
     reqs = requests()
+    if form == "frame":  # and one request the frame form cannot take
+        from plaquette_simulator import workloads as W
+        ob = W.OpList()
+        ob.h(0)
+        m = ob.measure(0)
+        ob.flagged(W.OP_H_FLAG, 1, m)
+        ob.measure(1)
+        reqs.append((3, 100, 77, ob.array()))
     biggest = max(((B + 63) // 64) * (2 * n + 1) ** 2 * 8 for n, B, _, _ in reqs)
     sv = ps.Server(budget_bytes=3 * biggest + (64 << 20), slots=4)
     sv.set_policy(policy, 0.05)
-    say(f"[case] server {len(reqs)} requests, budget {3 * biggest}, {policy}")
+    sv.set_form(form)
+    say(f"[case] server {len(reqs)} requests, budget {3 * biggest}, {policy}, {form}")
     tickets = [sv.submit(n, B, seed, ops) for n, B, seed, ops in reqs]
     sv.drain()
     stats = sv.stats()
@@ -70,6 +82,14 @@ def test_server_results_equal_standalone(ps, policy):
         for i, (r, dd) in enumerate(ref):
             assert np.array_equal(res[i], r) and np.array_equal(det[i], dd), (t, i)
         sv.release(t)
+    if form == "frame":
+        fs = sv.frame_stats()
+        say(f"[frame] {fs}")
+        assert fs["frame_runs"] == len(reqs) - 1     # all but the flagged-H request
+        # one plan per structure: distance, and whether the circuit has noise ops
+        shapes = {(n, bool((ops[:, 0] == 10).any())) for n, B, seed, ops in reqs[:-1]}
+        assert fs["plan_builds"] == len(shapes)
+        assert fs["plan_hits"] == fs["frame_runs"] - fs["plan_builds"]
 
 
 def test_server_smallest_first_order(ps):
