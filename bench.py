@@ -158,6 +158,36 @@ def bench_mixed(args):
     ms_step = 1e3 * wall / args.steps
     kernel_ms = float(np.mean(kms))
     info = batch.info()
+    batch.set_timing(False)
+
+    # frame form of the same batch (all members allow it): one noise
+    # prologue, one frame kernel over the packed list, plans shared by the
+    # circuits of one distance
+    frame = {}
+    try:
+        t = time.perf_counter()
+        for st in states:
+            st.reset_clifford_state()
+        batch.run("frame")
+        batch.wait()
+        frame["first_run_s"] = time.perf_counter() - t
+        for _ in range(args.warmup):
+            for st in states:
+                st.reset_clifford_state()
+            batch.run("frame")
+        batch.wait()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            for st in states:
+                st.reset_clifford_state()
+            batch.run("frame")
+        batch.wait()
+        fms = 1e3 * (time.perf_counter() - t) / args.steps
+        frame.update({"ms_per_step": fms, "gate_shots_per_s": gate_shots / (fms / 1e3),
+                      "speedup_vs_batch": ms_step / fms})
+        progress(f"mixed: frame form {fms:.1f} ms/step")
+    except RuntimeError as e:
+        frame = {"error": str(e)}
     out = {
         "metric": "gate-shots/s, 256 mixed surface-code circuits in one batched launch",
         "value": gate_shots / (ms_step / 1e3),
@@ -185,6 +215,7 @@ def bench_mixed(args):
         "speedup_vs_sequential": seq_ms / ms_step,
         "autotune": choice,
         "launch": info,
+        "frame_form": frame,
         "setup_s": setup_s,
     }
     print(json.dumps(out))

@@ -15,6 +15,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "plaquette_amd.h"
@@ -38,6 +39,8 @@ struct pq_batch_s {
   hipStream_t stream = nullptr;
   BatchCircuit *d_circ = nullptr;
   BatchCircuit *h_circ = nullptr;  // pinned
+  FrameArgs *d_fargs = nullptr;    // frame form: per-member kernel arguments
+  FrameArgs *h_fargs = nullptr;    // pinned
   uint64_t *d_items = nullptr;
   unsigned long long *d_counter = nullptr;
   uint64_t nitems = 0;
@@ -65,6 +68,8 @@ pq_batch_s::~pq_batch_s() {
   drop_graph();
   if (d_circ) (void)hipFree(d_circ);
   if (h_circ) (void)hipHostFree(h_circ);
+  if (d_fargs) (void)hipFree(d_fargs);
+  if (h_fargs) (void)hipHostFree(h_fargs);
   if (d_items) (void)hipFree(d_items);
   if (d_counter) (void)hipFree(d_counter);
   for (hipEvent_t e : {ev_upload, ev_a, ev_b})
@@ -168,6 +173,10 @@ static int build(pq_batch b) {
   b->d_circ = nullptr;
   if (b->h_circ) HIPCHK(hipHostFree(b->h_circ));
   b->h_circ = nullptr;
+  if (b->d_fargs) HIPCHK(hipFree(b->d_fargs));
+  b->d_fargs = nullptr;
+  if (b->h_fargs) HIPCHK(hipHostFree(b->h_fargs));
+  b->h_fargs = nullptr;
   b->nitems = list.size();
   if (!list.empty()) {
     HIPCHK(hipMalloc(&b->d_items, list.size() * 8));
@@ -176,6 +185,8 @@ static int build(pq_batch b) {
   if (nc) {
     HIPCHK(hipMalloc(&b->d_circ, nc * sizeof(BatchCircuit)));
     HIPCHK(hipHostMalloc(&b->h_circ, nc * sizeof(BatchCircuit), hipHostMallocDefault));
+    HIPCHK(hipMalloc(&b->d_fargs, nc * sizeof(FrameArgs)));
+    HIPCHK(hipHostMalloc(&b->h_fargs, nc * sizeof(FrameArgs), hipHostMallocDefault));
   }
   b->built = true;
   return PQ_OK;
@@ -211,10 +222,86 @@ static int enqueue(pq_batch b) {
              : pq_set_error(PQ_ERR_HIP, "batch launch failed");
 }
 
+// The frame form of a batch: every member must allow it (no flagged
+// H/S/CX/CZ) and start from the |0...0> tableau; members with equal
+// structure share one plan.  One noise prologue for all members, one frame
+// kernel over the packed work list, then each member's shared X/Z.
+static int run_frame_batch(pq_batch b) {
+  const uint32_t nc = (uint32_t)b->members.size();
+  // everything that can refuse the run, before any member changes
+  std::unordered_map<std::string, std::shared_ptr<FramePlan>> plans;
+  for (uint32_t c = 0; c < nc; ++c) {
+    pq_state s = b->members[c].first;
+    pq_program p = b->members[c].second;
+    if (s->tab && !s->xz_identity)
+      return pq_set_error(PQ_ERR_UNSUPPORTED,
+                          "frame form runs from the |0...0> tableau: reset every member first");
+    if (p->frame) continue;
+    std::string key = frame_plan_key(p->n, p->ops.data(), p->ops.size());
+    auto it = plans.find(key);
+    if (it != plans.end()) {
+      p->frame = it->second;
+    } else {
+      int rc = build_frame_plan(p);
+      if (rc) return rc;
+      plans.emplace(std::move(key), p->frame);
+    }
+  }
+  HIPCHK(hipEventSynchronize(b->ev_upload));
+  uint32_t lds = 0, noise = 0;
+  for (uint32_t c = 0; c < nc; ++c) {
+    pq_state s = b->members[c].first;
+    pq_program p = b->members[c].second;
+    int rc = prepare_records(s, p);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if ((rc = upload_frame_plan(p, b->stream))) return rc;
+    BatchCircuit &bc = b->h_circ[c];
+    bc = BatchCircuit{};
+    bc.a.planes = p->d_planes;
+    bc.a.W = s->tab ? s->W : 0;
+    bc.seed = (uint64_t)s->seed;
+    bc.mseq0 = s->meas_seq;
+    bc.nseq0 = s->noise_seq;
+    bc.noise = p->d_noise;
+    bc.nnoise = (uint32_t)p->nnoise;
+    b->h_fargs[c] = frame_args(s, p);
+    if (!s->tab) b->h_fargs[c].W = 0;
+    lds = std::max(lds, p->frame->lds_bytes);
+    if (p->nnoise) noise = b->noise_blocks;
+    s->meas_seq += p->nmeas;
+    s->noise_seq += p->nnoise;
+    if (s->tab) s->xz_identity = p->frame->final_identity;
+  }
+  HIPCHK(hipMemcpyAsync(b->d_circ, b->h_circ, nc * sizeof(BatchCircuit), hipMemcpyHostToDevice,
+                        b->stream));
+  HIPCHK(hipMemcpyAsync(b->d_fargs, b->h_fargs, nc * sizeof(FrameArgs), hipMemcpyHostToDevice,
+                        b->stream));
+  HIPCHK(hipEventRecord(b->ev_upload, b->stream));
+  if (b->timed) HIPCHK(hipEventRecord(b->ev_a, b->stream));
+  HIPCHK(launch_batch_noise(b->stream, b->d_circ, nc, noise));
+  HIPCHK(launch_frame_batch(b->stream, b->d_fargs, b->d_items, b->nitems, lds));
+  for (uint32_t c = 0; c < nc; ++c) {
+    pq_state s = b->members[c].first;
+    const FramePlan &F = *b->members[c].second->frame;
+    if (s->tab) HIPCHK(launch_frame_xz(b->stream, s->tab, geom_of(s), F.d_X, F.d_Z, F.nw));
+  }
+  if (b->timed) HIPCHK(hipEventRecord(b->ev_b, b->stream));
+  return PQ_OK;
+}
+
 extern "C" int pq_batch_run(pq_batch b, int form) {
   if (!b) return pq_set_error(PQ_ERR_ARG, "null batch");
-  if (form != PQ_RUN_PERSISTENT && form != PQ_RUN_GRAPH)
-    return pq_set_error(PQ_ERR_ARG, "a batch runs in the persistent or graph form");
+  if (form != PQ_RUN_PERSISTENT && form != PQ_RUN_GRAPH && form != PQ_RUN_FRAME)
+    return pq_set_error(PQ_ERR_ARG, "a batch runs in the persistent, graph or frame form");
+  if (b->members.empty()) return PQ_OK;
+  if (form == PQ_RUN_FRAME) {
+    if (!b->built) {
+      int rc = build(b);
+      if (rc) return rc;
+    }
+    return run_frame_batch(b);
+  }
   if (b->members.empty()) return PQ_OK;
   if (!b->built) {
     int rc = build(b);

@@ -739,24 +739,8 @@ FramePlan::~FramePlan() {
     if (q) (void)hipFree(q);
 }
 
-int run_frame(pq_state s, pq_program_s *p) {
-  if (!s->xz_identity)
-    return pq_set_error(PQ_ERR_UNSUPPORTED,
-                        "frame form runs from the |0...0> tableau: reset the state first");
-  if (!p->frame) {
-    int rc = build_frame_plan(p);
-    if (rc) return rc;
-  }
-  int rc = upload_frame_plan(p, s->stream);
-  if (rc) return rc;
+FrameArgs frame_args(pq_state s, pq_program_s *p) {
   const FramePlan &F = *p->frame;
-  HIPCHK(launch_set_params(s->stream, s->params_dev, (uint64_t)s->seed, s->meas_seq,
-                           s->noise_seq));
-  s->meas_seq += p->nmeas;
-  s->noise_seq += p->nnoise;
-  if (p->nnoise)
-    HIPCHK(launch_program_noise(s->stream, p->d_planes, p->d_noise, (uint32_t)p->nnoise, s->W,
-                                s->params_dev));
   FrameArgs a{};
   a.tab = s->tab;
   a.recs = s->recs;
@@ -776,6 +760,30 @@ int run_frame(pq_state s, pq_program_s *p) {
   a.nb_words = F.nb_words;
   a.sb_words = F.sb_words;
   a.lds_bytes = F.lds_bytes;
+  a.seed = (uint64_t)s->seed;
+  a.mseq0 = s->meas_seq;
+  return a;
+}
+
+int run_frame(pq_state s, pq_program_s *p) {
+  if (!s->xz_identity)
+    return pq_set_error(PQ_ERR_UNSUPPORTED,
+                        "frame form runs from the |0...0> tableau: reset the state first");
+  if (!p->frame) {
+    int rc = build_frame_plan(p);
+    if (rc) return rc;
+  }
+  int rc = upload_frame_plan(p, s->stream);
+  if (rc) return rc;
+  const FramePlan &F = *p->frame;
+  HIPCHK(launch_set_params(s->stream, s->params_dev, (uint64_t)s->seed, s->meas_seq,
+                           s->noise_seq));
+  FrameArgs a = frame_args(s, p);
+  s->meas_seq += p->nmeas;
+  s->noise_seq += p->nnoise;
+  if (p->nnoise)
+    HIPCHK(launch_program_noise(s->stream, p->d_planes, p->d_noise, (uint32_t)p->nnoise, s->W,
+                                s->params_dev));
   if (p->timing_on) {
     if (p->ev_used + 2 > p->ev.size()) {
       hipEvent_t e0, e1;
@@ -786,7 +794,7 @@ int run_frame(pq_state s, pq_program_s *p) {
     }
     HIPCHK(hipEventRecord(p->ev[p->ev_used], s->stream));
   }
-  HIPCHK(launch_frame(s->stream, a, s->params_dev));
+  HIPCHK(launch_frame(s->stream, a));
   if (p->timing_on) {
     HIPCHK(hipEventRecord(p->ev[p->ev_used + 1], s->stream));
     p->ev_used += 2;

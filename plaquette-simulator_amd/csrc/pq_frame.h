@@ -36,6 +36,10 @@ struct FramePlan {
 };
 
 int build_frame_plan(pq_program_s *p);
+int upload_frame_plan(pq_program_s *p, hipStream_t stream);
+// Kernel arguments of program p's frame plan on state s (at s's current
+// measurement sequence number).
+pq::FrameArgs frame_args(pq_state_s *s, pq_program_s *p);
 // What a frame plan depends on, as bytes: n and the op list with the noise
 // thresholds left out (they live in the noise descriptors, not the plan).
 // Programs with equal keys can share one plan.

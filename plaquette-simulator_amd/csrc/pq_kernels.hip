@@ -1630,10 +1630,8 @@ __device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_re
 constexpr uint32_t kFrameThreads = PQ_FRAME_THREADS;
 constexpr uint32_t kFrameWaves = kFrameThreads / 64;
 
-__global__ __launch_bounds__(kFrameThreads) void k_frame(FrameArgs a,
-                                                         const ProgParams *__restrict__ pp) {
+__device__ __forceinline__ void frame_word_group(const FrameArgs &a, uint64_t w) {
   extern __shared__ uint64_t dyn[];
-  const uint64_t w = blockIdx.x;
   const uint32_t tid = threadIdx.x, lane = lane_id();
   const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const uint32_t n = a.n, R = a.R;
@@ -1646,7 +1644,7 @@ __global__ __launch_bounds__(kFrameThreads) void k_frame(FrameArgs a,
   for (uint32_t i = tid; i < a.npreload; i += kFrameThreads)
     dyn[a.lds_rec + a.preload[2 * i + 1]] = a.recs[(2ull * a.preload[2 * i]) * a.W + w];
   if (tid == 0) dyn[a.lds_const] = ~0ull;
-  const uint64_t seed = pp->seed, mseq0 = pp->mseq0;
+  const uint64_t seed = a.seed, mseq0 = a.mseq0;
   for (uint32_t c = 0; c < a.nchunk; ++c) {
     const uint32_t cb = a.chunks[4 * c], cw = a.chunks[4 * c + 1];
     const uint32_t k0 = a.chunks[4 * c + 2], nk = a.chunks[4 * c + 3];
@@ -1754,6 +1752,18 @@ __global__ __launch_bounds__(kFrameThreads) void k_frame(FrameArgs a,
   }
   __syncthreads();
   for (uint32_t i = tid; i < R; i += kFrameThreads) Tr[i] = Rl[i];
+}
+
+__global__ __launch_bounds__(kFrameThreads) void k_frame(FrameArgs a) {
+  frame_word_group(a, blockIdx.x);
+}
+
+// Frame form of a mixed batch: block b runs work item items[b] = (circuit
+// << 40 | word group) with that circuit's arguments.
+__global__ __launch_bounds__(kFrameThreads) void k_frame_batch(const FrameArgs *__restrict__ circ,
+                                                               const uint64_t *__restrict__ items) {
+  const uint64_t it = items[blockIdx.x];
+  frame_word_group(circ[it >> 40], it & ((1ull << 40) - 1));
 }
 
 // Shared X/Z (bit rows [R][nw]) into columns 0..2n-1 of every word group.
@@ -1921,12 +1931,28 @@ hipError_t launch_program_noise(hipStream_t s, uint64_t *planes,
   return hipGetLastError();
 }
 
-hipError_t launch_frame(hipStream_t s, const FrameArgs &a, const void *params) {
+hipError_t launch_frame(hipStream_t s, const FrameArgs &a) {
   if (!a.W) return hipSuccess;
   hipError_t e = ensure_lds_attr((const void *)k_frame, a.lds_bytes);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_frame, dim3((uint32_t)a.W), dim3(kFrameThreads), a.lds_bytes, s, a,
-                     (const ProgParams *)params);
+  hipLaunchKernelGGL(k_frame, dim3((uint32_t)a.W), dim3(kFrameThreads), a.lds_bytes, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_frame_batch(hipStream_t s, const FrameArgs *circ, const uint64_t *items,
+                              uint64_t nitems, uint32_t lds_bytes) {
+  if (!nitems) return hipSuccess;
+  hipError_t e = ensure_lds_attr((const void *)k_frame_batch, lds_bytes);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_frame_batch, dim3((uint32_t)nitems), dim3(kFrameThreads), lds_bytes, s,
+                     circ, items);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_noise(hipStream_t s, const BatchCircuit *circ, uint32_t ncirc,
+                              uint32_t noise_blocks) {
+  if (!ncirc || !noise_blocks) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_noise, dim3(noise_blocks, ncirc), dim3(256), 0, s, circ);
   return hipGetLastError();
 }
 

@@ -123,3 +123,76 @@ def test_batch_after_autotune(ps):
     got, batch = batched(ps, specs, "graph", autotune=True)
     for spec, g in zip(specs, got):
         same(standalone(ps, spec), g)
+
+
+def frame_members():
+    """Members the frame form takes: surface codes and random programs without
+    flagged H/S/CX/CZ (resets and other flagged Paulis included)."""
+    from plaquette_simulator import workloads as W
+    from test_frame_cpu import frame_program
+    out = [m for m in members() if m[0] in (17, 49, 97, 161)]  # the surface codes
+    rng = np.random.default_rng(6)
+    for i, (n, B) in enumerate([(5, 70), (12, 128), (33, 65), (64, 190)]):
+        out.append((n, B, 3000 + i, frame_program(W, rng, n, 250)))
+    return out
+
+
+def test_batch_frame_equals_standalone(ps):
+    """The frame form of a mixed batch (one noise prologue, one frame kernel
+    over the packed list, members of one structure sharing a plan) equals each
+    member's standalone persistent run; a reset and a second run too."""
+    specs = frame_members()
+    say(f"[case] frame batch of {len(specs)}")
+    batch = ps.Batch()
+    states = []
+    for n, B, seed, ops in specs:
+        st = ps.CliffordState(n, B, seed, True)
+        batch.add(st, ps.Program(st, ops))
+        states.append(st)
+    batch.run("frame")
+    for spec, st in zip(specs, states):
+        st.sync_records()
+        same(standalone(ps, spec), snapshot(st))
+    for st in states:
+        st.reset_clifford_state()
+    batch.run("frame")
+    for spec, st in zip(specs, states):
+        st.sync_records()
+        n, B, seed, ops = spec
+        ref = ps.CliffordState(n, B, seed, True)
+        prog = ps.Program(ref, ops)
+        prog.run(ref, "persistent")
+        ref.reset_clifford_state()
+        prog.run(ref, "persistent")
+        ref.synchronize()
+        ref.sync_records()
+        same(snapshot(ref), snapshot(st))
+
+
+def test_batch_frame_refusals(ps):
+    """A member the frame form cannot take, or one not at |0...0>, refuses the
+    whole run before any member changes."""
+    from plaquette_simulator import workloads as W
+    specs = frame_members()[:3]
+    ob = W.OpList()
+    m = ob.measure(0)
+    ob.flagged(W.OP_H_FLAG, 1, m)
+    batch = ps.Batch()
+    states = []
+    for n, B, seed, ops in specs + [(4, 64, 9, ob.array())]:
+        st = ps.CliffordState(n, B, seed, True)
+        batch.add(st, ps.Program(st, ops))
+        states.append(st)
+    with pytest.raises(RuntimeError, match="frame form"):
+        batch.run("frame")
+    assert all(st.num_records == 0 for st in states)
+    batch2 = ps.Batch()
+    st2 = []
+    for n, B, seed, ops in specs:
+        st = ps.CliffordState(n, B, seed, True)
+        batch2.add(st, ps.Program(st, ops))
+        st2.append(st)
+    st2[1].apply_hadamard_gate(0)
+    with pytest.raises(RuntimeError, match="reset"):
+        batch2.run("frame")
+    assert all(st.num_records == 0 for st in st2)
