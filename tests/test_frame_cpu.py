@@ -123,3 +123,14 @@ def test_flagged_h_is_not_frame_eligible(b, W):
     ob2.with_prob(W.OP_CX, 0, 0.5, q1=1)
     with pytest.raises(RuntimeError, match="frame form"):
         b.frame_plan(3, ob2.array())
+
+
+def test_gates_without_phase_flips(b, W):
+    """H, CX, S on |0...0> flip no phase, so the stream is empty, but X/Z
+    change (the plan's final X/Z, which the run writes into every group)."""
+    ob = W.OpList()
+    ob.h(0)
+    ob.cx(0, 1)
+    ob.s(2)
+    plan = check(b, 3, 10, 1, ob.array())
+    assert plan["stream_words"] == 0 and plan["chunks_n"] == 0

@@ -156,3 +156,16 @@ def test_frame_refusals_leave_state_untouched(ps, W):
     orc.run_program(p.array())
     x, z, r = st.device_to_host_numpy()
     assert np.array_equal(r, orc.r) and np.array_equal(x, orc.x) and np.array_equal(z, orc.z)
+
+
+def test_gates_without_phase_flips(ps, W):
+    """An op stream with nothing for R still writes the new X/Z."""
+    ob = W.OpList()
+    ob.h(0)
+    ob.cx(0, 1)
+    ob.s(2)
+    st = ps.CliffordState(3, 70, 1, True)
+    st.run_program(ob.array(), "frame")
+    orc = O.OracleState(3, 70, 1)
+    orc.run_program(ob.array())
+    assert_same(st, orc)
