@@ -30,6 +30,8 @@
 
 namespace {
 
+constexpr size_t kMaxCachedPlans = 64;  // frame plans kept between requests
+
 double now_s() {
   using namespace std::chrono;
   return duration<double>(steady_clock::now().time_since_epoch()).count();
@@ -135,6 +137,8 @@ int pq_server_s::admit(Req *r) {
       ++plan_hits;
       run_form = PQ_RUN_FRAME;
     } else if (build_frame_plan(r->prog) == PQ_OK) {
+      // bounded cache: plans in use stay alive through their programs
+      if (plans.size() >= kMaxCachedPlans) plans.clear();
       plans.emplace(key, r->prog->frame);
       ++plan_builds;
       run_form = PQ_RUN_FRAME;
