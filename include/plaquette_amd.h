@@ -229,7 +229,7 @@ int pq_program_layout(pq_program p, uint64_t *segments, uint64_t *tiled_segments
  * preload pairs, words per X/Z bit row, flag records, LDS word offsets of
  * the records, the all-ones word and the noise window, noise-window words,
  * stream-buffer words, LDS bytes per word group, random measurements,
- * determined measurements, final X/Z is |0...0>, 0, 0}; non-NULL buffers
+ * determined measurements, final X/Z is |0...0>, staging buffers, 0}; non-NULL buffers
  * receive the op stream, the chunk table ({begin, words, k0, nk} each), the
  * {slot, index} preload pairs and the final shared X/Z bit rows
  * ([2n+1][words per row] each).  No reference counterpart (tests, tools). */

@@ -32,6 +32,9 @@
 //           (rows n+i).  The last one's result is R[2n].
 // The stream is cut into chunks that fit the kernel's LDS staging buffer;
 // each chunk also names the noise numbers [k0, k0+nk) whose masks it reads.
+// With two staging buffers (when they fit the LDS) chunk c uses buffer c % 2
+// and the kernel loads chunk c+1 while it runs chunk c; a chunk's noise
+// entries name the mask words of its own buffer.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -538,7 +541,8 @@ struct Packer {
   uint32_t lds_index(const Entry &e) const {
     if (e.src == kSrcConst) return F.lds_const;
     if (e.src == kSrcRec) return F.lds_rec + e.arg;
-    return F.lds_noise + (e.arg - 2 * (uint32_t)k0);
+    const uint32_t buf = (uint32_t)(F.chunks.size() / 4) % F.nbufs;  // the open chunk's
+    return F.lds_noise + buf * F.nb_words + (e.arg - 2 * (uint32_t)k0);
   }
   // One XOR op of `es` (all noise numbers inside the open chunk's window).
   void emit_xor(std::vector<Entry> &es) {
@@ -656,8 +660,14 @@ int build_frame_plan(pq_program_s *p) {
   for (const Logical &m : ops)
     if (m.kind != kFrameXor) need = std::max<uint32_t>(need, 8 + 4 + 1 + (uint32_t)m.rows.size());
   F->sb_words = std::max<uint32_t>(kFrameStreamWords, (need + 63) / 64 * 64);
-  // ... and 64 words of partial results per wave (up to 16 waves)
-  const uint64_t lds = 8ull * (F->lds_noise + F->nb_words) + 4ull * F->sb_words + 8ull * 64 * 16;
+  // ... and 64 words of partial results per wave (up to 16 waves); two
+  // staging buffers (noise window and stream) when they fit
+  auto lds_for = [&](uint64_t nbufs) {
+    return 8ull * (F->lds_noise + nbufs * F->nb_words) + 4ull * nbufs * F->sb_words +
+           8ull * 64 * 16;
+  };
+  F->nbufs = lds_for(2) <= kFrameLdsMax ? 2 : 1;
+  const uint64_t lds = lds_for(F->nbufs);
   if (lds > kFrameLdsMax)
     return pq_set_error(PQ_ERR_UNSUPPORTED,
                         ("frame form needs " + std::to_string(lds) +
@@ -759,6 +769,7 @@ FrameArgs frame_args(pq_state s, pq_program_s *p) {
   a.lds_noise = F.lds_noise;
   a.nb_words = F.nb_words;
   a.sb_words = F.sb_words;
+  a.nbufs = F.nbufs;
   a.lds_bytes = F.lds_bytes;
   a.seed = (uint64_t)s->seed;
   a.mseq0 = s->meas_seq;
@@ -823,7 +834,8 @@ extern "C" int pq_frame_plan(uint64_t n, const pq_op *ops, uint64_t nops, uint64
   const FramePlan &F = *prog.frame;
   const uint64_t v[16] = {F.stream.size(), F.chunks.size() / 4, F.preload.size() / 2, F.nw,
                           F.nref, F.lds_rec, F.lds_const, F.lds_noise, F.nb_words, F.sb_words,
-                          F.lds_bytes, F.nrandom, F.ndet, F.final_identity ? 1u : 0u, 0, 0};
+                          F.lds_bytes, F.nrandom, F.ndet, F.final_identity ? 1u : 0u, F.nbufs,
+                          0};
   for (int i = 0; i < 16; ++i) info[i] = v[i];
   if (stream) std::copy(F.stream.begin(), F.stream.end(), stream);
   if (chunks) std::copy(F.chunks.begin(), F.chunks.end(), chunks);

@@ -23,6 +23,7 @@ struct FramePlan {
   uint32_t nref = 0;                 // records read by flag ops (rec_l words)
   uint32_t lds_rec = 0, lds_const = 0, lds_noise = 0;  // LDS word offsets
   uint32_t nb_words = 0, sb_words = 0, lds_bytes = 0;
+  uint32_t nbufs = 1;                // staging buffers (2: load ahead)
   std::vector<uint32_t> stream;      // ops
   std::vector<uint32_t> chunks;      // {begin, words, k0, nk} per chunk
   std::vector<uint32_t> preload;     // {slot, rec_l index} loaded at start

@@ -1636,27 +1636,23 @@ __device__ __forceinline__ void frame_word_group(const FrameArgs &a, uint64_t w)
   const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const uint32_t n = a.n, R = a.R;
   uint64_t *Rl = dyn;
-  uint64_t *nbuf = dyn + a.lds_noise;
-  uint32_t *sbuf = reinterpret_cast<uint32_t *>(dyn + a.lds_noise + a.nb_words);
-  uint64_t *part = reinterpret_cast<uint64_t *>(sbuf + a.sb_words);  // [waves][64]
+  uint64_t *nbuf0 = dyn + a.lds_noise;
+  uint32_t *sbuf0 = reinterpret_cast<uint32_t *>(dyn + a.lds_noise + a.nbufs * a.nb_words);
+  uint64_t *part = reinterpret_cast<uint64_t *>(sbuf0 + a.nbufs * a.sb_words);  // [waves][64]
   uint64_t *Tr = a.tab + (w * a.C + 2ull * n) * R;  // the group's R column
-  for (uint32_t i = tid; i < R; i += kFrameThreads) Rl[i] = Tr[i];
-  for (uint32_t i = tid; i < a.npreload; i += kFrameThreads)
-    dyn[a.lds_rec + a.preload[2 * i + 1]] = a.recs[(2ull * a.preload[2 * i]) * a.W + w];
-  if (tid == 0) dyn[a.lds_const] = ~0ull;
-  const uint64_t seed = a.seed, mseq0 = a.mseq0;
-  for (uint32_t c = 0; c < a.nchunk; ++c) {
+  // chunk c's stream words and noise masks (planes 2k0 .. 2(k0+nk)-1 of this
+  // group) into staging buffer c % nbufs by LDS DMA, one wave-wide 4-byte
+  // load per 64 dwords, the waves taking turns; lanes past the end reload
+  // the last dword into the buffer's padding
+  auto stage = [&](uint32_t c) {
     const uint32_t cb = a.chunks[4 * c], cw = a.chunks[4 * c + 1];
     const uint32_t k0 = a.chunks[4 * c + 2], nk = a.chunks[4 * c + 3];
-    // every thread is done with the last chunk's buffers
-    __syncthreads();
-    // stream words and the chunk's noise masks (planes 2k0 .. 2(k0+nk)-1 of
-    // this group) by LDS DMA, one wave-wide 4-byte load per 64 dwords, the
-    // waves taking turns; lanes past the end reload the last dword into the
-    // buffer's padding
+    const uint32_t buf = c % a.nbufs;
+    uint32_t *sb = sbuf0 + buf * a.sb_words;
+    uint32_t *nb = reinterpret_cast<uint32_t *>(nbuf0 + buf * a.nb_words);
     for (uint32_t i = wv * 64; i < cw; i += kFrameThreads) {
       const uint32_t j = min(i + lane, cw - 1);
-      __builtin_amdgcn_global_load_lds(a.stream + cb + j, (lds_void *)(sbuf + i), 4, 0, 0);
+      __builtin_amdgcn_global_load_lds(a.stream + cb + j, (lds_void *)(sb + i), 4, 0, 0);
     }
     const uint32_t dw = 4 * nk;
     for (uint32_t i = wv * 64; i < dw; i += kFrameThreads) {
@@ -1664,11 +1660,23 @@ __device__ __forceinline__ void frame_word_group(const FrameArgs &a, uint64_t w)
       const uint32_t *src =
           reinterpret_cast<const uint32_t *>(a.planes + (2ull * k0 + (t >> 1)) * a.W + w) +
           (t & 1);
-      __builtin_amdgcn_global_load_lds(src, (lds_void *)(reinterpret_cast<uint32_t *>(nbuf) + i),
-                                       4, 0, 0);
+      __builtin_amdgcn_global_load_lds(src, (lds_void *)(nb + i), 4, 0, 0);
     }
+  };
+  for (uint32_t i = tid; i < R; i += kFrameThreads) Rl[i] = Tr[i];
+  for (uint32_t i = tid; i < a.npreload; i += kFrameThreads)
+    dyn[a.lds_rec + a.preload[2 * i + 1]] = a.recs[(2ull * a.preload[2 * i]) * a.W + w];
+  if (tid == 0) dyn[a.lds_const] = ~0ull;
+  const uint64_t seed = a.seed, mseq0 = a.mseq0;
+  if (a.nchunk) stage(0);
+  for (uint32_t c = 0; c < a.nchunk; ++c) {
+    const uint32_t cw = a.chunks[4 * c + 1];
+    // chunk c has landed, and every thread is done with the buffer that
+    // chunk c+1 goes to (chunk c-1's, or chunk c's own with one buffer)
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
+    if (a.nbufs > 1 && c + 1 < a.nchunk) stage(c + 1);
+    const uint32_t *sbuf = sbuf0 + (c % a.nbufs) * a.sb_words;
     uint32_t pos = 0;
     while (pos < cw) {
       const uint32_t kind = rfl(sbuf[pos]);
@@ -1749,6 +1757,7 @@ __device__ __forceinline__ void frame_word_group(const FrameArgs &a, uint64_t w)
         __syncthreads();
       }
     }
+    if (a.nbufs == 1 && c + 1 < a.nchunk) stage(c + 1);  // after the ops' last barrier
   }
   __syncthreads();
   for (uint32_t i = tid; i < R; i += kFrameThreads) Tr[i] = Rl[i];

@@ -55,18 +55,20 @@ def run(plan, n, B, seed, ops, meas_seq0=0, noise_seq0=0):
     rec_res, rec_det = {}, {}
     for w in range(W):
         shots = [64 * w + i for i in range(64)]
-        size = plan["lds_noise"] + plan["nb_words"]
+        nbufs = plan.get("nbufs", 1)
+        size = plan["lds_noise"] + nbufs * plan["nb_words"]
         lds = [0] * size
         lds[2 * n] = M64                      # scratch row r = 1 (init)
         lds[plan["lds_const"]] = M64
         if plan["preload_n"]:
             raise NotImplementedError("preloaded records")
-        for cb, cw, k0, nk in plan["chunks"]:
+        for ci, (cb, cw, k0, nk) in enumerate(plan["chunks"]):
             cb, cw, k0, nk = int(cb), int(cw), int(k0), int(nk)
+            base = plan["lds_noise"] + (ci % nbufs) * plan["nb_words"]  # the chunk's window
             for j in range(nk):
                 mx, mz = _masks(descs[k0 + j], seed, noise_seq0 + k0 + j, shots)
-                lds[plan["lds_noise"] + 2 * j] = mx
-                lds[plan["lds_noise"] + 2 * j + 1] = mz
+                lds[base + 2 * j] = mx
+                lds[base + 2 * j + 1] = mz
             s = stream[cb:cb + cw]
             pos = 0
             while pos < cw:

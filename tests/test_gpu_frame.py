@@ -169,3 +169,22 @@ def test_gates_without_phase_flips(ps, W):
     orc = O.OracleState(3, 70, 1)
     orc.run_program(ob.array())
     assert_same(st, orc)
+
+
+@pytest.mark.parametrize("n,B,nops,seed", [(300, 128, 3000, 41), (1024, 64, 4000, 42)])
+def test_large_states_frame_equals_persistent(ps, W, n, B, nops, seed):
+    """Large random Clifford circuits (10% measurements, so most chunks hold
+    long random-measurement row lists): the frame form equals the persistent
+    form, planes and records."""
+    ops = W.random_clifford(n, nops, seed, meas_prob=0.1)
+    say(f"[frame] large n={n} B={B} ops={len(ops)}")
+    a = ps.CliffordState(n, B, seed, True)
+    b = ps.CliffordState(n, B, seed, True)
+    a.run_program(ops, "persistent")
+    b.run_program(ops, "frame")
+    assert np.array_equal(a.read_planes(), b.read_planes()), "planes differ"
+    assert a.num_records == b.num_records
+    for i in range(a.num_records):
+        ra, da = a.get_record(i)
+        rb, db = b.get_record(i)
+        assert np.array_equal(ra, rb) and np.array_equal(da, db), f"record {i}"
