@@ -1625,7 +1625,7 @@ __device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_re
 // threads split each op's rows (ops touch distinct rows, so within an op no
 // two threads write one word) and a barrier separates ops.
 #ifndef PQ_FRAME_THREADS
-#define PQ_FRAME_THREADS 256
+#define PQ_FRAME_THREADS 512
 #endif
 constexpr uint32_t kFrameThreads = PQ_FRAME_THREADS;
 constexpr uint32_t kFrameWaves = kFrameThreads / 64;
