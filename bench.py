@@ -484,9 +484,10 @@ def main():
                     help="execution forms to serve with (comma-separated)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build runs on a single MI355X "
-              f"(--gpus {args.gpus} is not supported; shots shard with no exchange, "
-              f"see the sharding notes in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU: this engine runs on a single MI355X "
+              f"(--gpus {args.gpus} is not supported; the path would shard on shots, each "
+              f"GPU a range of 64-shot word groups, with no exchange until the records "
+              f"are gathered)", file=sys.stderr)
         sys.exit(2)
 
     if args.mixed:

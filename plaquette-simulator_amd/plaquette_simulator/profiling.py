@@ -1,4 +1,4 @@
-"""Profiling helpers for the engine (run under rocprofv3; see DESIGN.md).
+"""Profiling helpers for the engine (run under rocprofv3).
 
   python profiling.py probe
       One persistent run of the d=9 surface-code workload (16384 shots),
