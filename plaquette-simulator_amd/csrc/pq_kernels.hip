@@ -32,15 +32,14 @@ __device__ __forceinline__ uint64_t draw_word_lane(uint64_t key, uint64_t w, uin
 // Init: word (c, row) of every group is all-ones iff row == c, which is the
 // identity tableau with r = 1 on the scratch row (MiscFunctors.hpp:22-27).
 // ---------------------------------------------------------------------------
-__global__ void k_fill_identity(uint64_t *__restrict__ tab, uint32_t R,
-                                uint32_t C, uint64_t total) {
-  const uint64_t blk = (uint64_t)C * R;
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t rem = i % blk;
-    const uint32_t c = (uint32_t)(rem / R);
-    const uint32_t row = (uint32_t)(rem - (uint64_t)c * R);
-    tab[i] = (row == c) ? ~0ull : 0ull;
+// One block per column (w, c) at a time, its threads over the column's R
+// contiguous rows: one index split per column, coalesced stores.
+__global__ __launch_bounds__(256) void k_fill_identity(uint64_t *__restrict__ tab, uint32_t R,
+                                                       uint32_t C, uint64_t ncols) {
+  for (uint64_t col = blockIdx.x; col < ncols; col += gridDim.x) {
+    const uint32_t c = (uint32_t)(col % C);
+    uint64_t *dst = tab + col * R;
+    for (uint32_t row = threadIdx.x; row < R; row += 256) dst[row] = (row == c) ? ~0ull : 0ull;
   }
 }
 
@@ -1804,10 +1803,10 @@ inline uint32_t grid_for(uint64_t items, uint32_t threads, uint32_t cap = 1u << 
 // ---------------------------------------------------------------------------
 
 hipError_t launch_fill_identity(hipStream_t s, uint64_t *tab, const Geom &g) {
-  const uint64_t total = g.W * (uint64_t)g.C * g.R;
-  if (!total) return hipSuccess;
-  hipLaunchKernelGGL(k_fill_identity, dim3(grid_for(total, 256, 65536)), dim3(256),
-                     0, s, tab, g.R, g.C, total);
+  const uint64_t ncols = g.W * (uint64_t)g.C;
+  if (!ncols || !g.R) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_identity, dim3((uint32_t)std::min<uint64_t>(ncols, 1u << 20)),
+                     dim3(256), 0, s, tab, g.R, g.C, ncols);
   return hipGetLastError();
 }
 
