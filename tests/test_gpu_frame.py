@@ -188,3 +188,25 @@ def test_large_states_frame_equals_persistent(ps, W, n, B, nops, seed):
         ra, da = a.get_record(i)
         rb, db = b.get_record(i)
         assert np.array_equal(ra, rb) and np.array_equal(da, db), f"record {i}"
+
+
+def test_three_forms_agree_at_n2000(ps, W):
+    """n=2000 (the measurement kernels keep the scratch row in LDS): the
+    plain, persistent and frame forms give the same planes and records."""
+    n, B, seed = 2000, 100, 43
+    ops = W.random_clifford(n, 2500, seed, meas_prob=0.04)
+    say(f"[forms] n={n} B={B} ops={len(ops)}")
+    states = {}
+    for form in ("plain", "persistent", "frame"):
+        st = ps.CliffordState(n, B, seed, True)
+        st.run_program(ops, form)
+        states[form] = st
+    ref = states["persistent"]
+    planes = ref.read_planes()
+    for form in ("plain", "frame"):
+        st = states[form]
+        assert np.array_equal(st.read_planes(), planes), f"{form}: planes differ"
+        for i in range(ref.num_records):
+            ra, da = ref.get_record(i)
+            rb, db = st.get_record(i)
+            assert np.array_equal(ra, rb) and np.array_equal(da, db), f"{form}: record {i}"
