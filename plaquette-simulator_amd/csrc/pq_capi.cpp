@@ -110,7 +110,7 @@ static int check_device() {
 }
 
 static int alloc_state(uint64_t n, uint64_t B, int64_t seed, hipStream_t borrowed,
-                       pq_state *out) {
+                       pq_state *out, bool r_only = false) {
   ARGCHK(out, "output handle is null");
   int rc = check_device();
   if (rc) return rc;
@@ -130,8 +130,9 @@ static int alloc_state(uint64_t n, uint64_t B, int64_t seed, hipStream_t borrowe
   } else {
     HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   }
+  s->r_only = r_only;
   if (n > 0 && B > 0) {
-    const uint64_t words = s->W * s->C * s->R;
+    const uint64_t words = s->W * (r_only ? 1 : s->C) * s->R;
     if (dev_alloc((void **)&s->tab, words * 8, s->alloc_stream) != hipSuccess) {
       (void)hipGetLastError();
       return fail(PQ_ERR_OOM, "tableau allocation failed (" +
@@ -152,11 +153,16 @@ Geom geom_of(const pq_state_s *s) {
 }
 
 int create_state_on(uint64_t n, uint64_t B, int64_t seed, hipStream_t stream,
-                    pq_state *out) {
-  int rc = alloc_state(n, B, seed, stream, out);
+                    pq_state *out, bool r_only) {
+  int rc = alloc_state(n, B, seed, stream, out, r_only);
   if (rc) return rc;
   pq_state s = *out;
-  if (s->tab) HIPCHK(launch_fill_identity(s->stream, s->tab, geom_of(s)));
+  if (s->tab) {
+    if (r_only)
+      HIPCHK(launch_fill_r_identity(s->stream, s->tab, s->W, (uint32_t)s->R));
+    else
+      HIPCHK(launch_fill_identity(s->stream, s->tab, geom_of(s)));
+  }
   s->xz_identity = true;
   return PQ_OK;
 }

@@ -33,6 +33,9 @@ struct pq_state_s {
   hipEvent_t ev[8] = {};
   // X/Z hold the |0...0> tableau in every shot (frame programs start there)
   bool xz_identity = false;
+  // only the R column is kept (tab is [W][R]): a server-internal state for
+  // frame-form requests, whose X/Z nobody reads; only frame runs take it
+  bool r_only = false;
   std::vector<std::vector<uint64_t>> rec_host;  // host copies of records
   std::vector<char> rec_valid;
   ~pq_state_s();
@@ -53,7 +56,8 @@ inline void dev_free(void *p, hipStream_t async_stream) {
     (void)hipFree(p);
 }
 // pq_create on a borrowed stream with stream-ordered allocation (server use).
-int create_state_on(uint64_t n, uint64_t B, int64_t seed, hipStream_t stream, pq_state *out);
+int create_state_on(uint64_t n, uint64_t B, int64_t seed, hipStream_t stream, pq_state *out,
+                    bool r_only = false);
 pq::Geom geom_of(const pq_state_s *s);
 int ensure_records(pq_state s, uint64_t count);
 uint64_t *record_ptr(pq_state s, uint64_t id);

@@ -773,6 +773,8 @@ FrameArgs frame_args(pq_state s, pq_program_s *p) {
   a.lds_bytes = F.lds_bytes;
   a.seed = (uint64_t)s->seed;
   a.mseq0 = s->meas_seq;
+  a.col_stride = s->r_only ? s->R : s->C * s->R;
+  a.r_off = s->r_only ? 0 : 2 * s->n * s->R;
   return a;
 }
 
@@ -810,7 +812,8 @@ int run_frame(pq_state s, pq_program_s *p) {
     HIPCHK(hipEventRecord(p->ev[p->ev_used + 1], s->stream));
     p->ev_used += 2;
   }
-  HIPCHK(launch_frame_xz(s->stream, s->tab, geom_of(s), F.d_X, F.d_Z, F.nw));
+  if (!s->r_only)  // an R-column-only state keeps no X/Z
+    HIPCHK(launch_frame_xz(s->stream, s->tab, geom_of(s), F.d_X, F.d_Z, F.nw));
   s->xz_identity = F.final_identity;
   return PQ_OK;
 }

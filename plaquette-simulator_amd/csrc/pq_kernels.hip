@@ -32,6 +32,14 @@ __device__ __forceinline__ uint64_t draw_word_lane(uint64_t key, uint64_t w, uin
 // Init: word (c, row) of every group is all-ones iff row == c, which is the
 // identity tableau with r = 1 on the scratch row (MiscFunctors.hpp:22-27).
 // ---------------------------------------------------------------------------
+// R-column-only states: word (w, row) = all ones on the scratch row R-1.
+__global__ __launch_bounds__(256) void k_fill_r_identity(uint64_t *__restrict__ rcol, uint32_t R,
+                                                         uint64_t W) {
+  for (uint64_t w = blockIdx.x; w < W; w += gridDim.x)
+    for (uint32_t row = threadIdx.x; row < R; row += 256)
+      rcol[w * R + row] = row == R - 1 ? ~0ull : 0ull;
+}
+
 // One block per column (w, c) at a time, its threads over the column's R
 // contiguous rows: one index split per column, coalesced stores.
 __global__ __launch_bounds__(256) void k_fill_identity(uint64_t *__restrict__ tab, uint32_t R,
@@ -1638,7 +1646,7 @@ __device__ __forceinline__ void frame_word_group(const FrameArgs &a, uint64_t w)
   uint64_t *nbuf0 = dyn + a.lds_noise;
   uint32_t *sbuf0 = reinterpret_cast<uint32_t *>(dyn + a.lds_noise + a.nbufs * a.nb_words);
   uint64_t *part = reinterpret_cast<uint64_t *>(sbuf0 + a.nbufs * a.sb_words);  // [waves][64]
-  uint64_t *Tr = a.tab + (w * a.C + 2ull * n) * R;  // the group's R column
+  uint64_t *Tr = a.tab + w * a.col_stride + a.r_off;  // the group's R column
   // chunk c's stream words and noise masks (planes 2k0 .. 2(k0+nk)-1 of this
   // group) into staging buffer c % nbufs by LDS DMA, one wave-wide 4-byte
   // load per 64 dwords, the waves taking turns; lanes past the end reload
@@ -1801,6 +1809,13 @@ inline uint32_t grid_for(uint64_t items, uint32_t threads, uint32_t cap = 1u << 
 // ---------------------------------------------------------------------------
 // Persistent program kernel (defined in pq_program.hip, shares measure_word).
 // ---------------------------------------------------------------------------
+
+hipError_t launch_fill_r_identity(hipStream_t s, uint64_t *rcol, uint64_t W, uint32_t R) {
+  if (!W || !R) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_r_identity, dim3((uint32_t)std::min<uint64_t>(W, 1u << 20)), dim3(256),
+                     0, s, rcol, R, W);
+  return hipGetLastError();
+}
 
 hipError_t launch_fill_identity(hipStream_t s, uint64_t *tab, const Geom &g) {
   const uint64_t ncols = g.W * (uint64_t)g.C;

@@ -442,6 +442,8 @@ extern "C" int pq_program_run(pq_state s, pq_program p, int form) {
   if (!s || !p) return pq_set_error(PQ_ERR_ARG, "null argument");
   if (p->n != s->n || p->W != s->W)
     return pq_set_error(PQ_ERR_ARG, "program was compiled for a state of another shape");
+  if (s->r_only && form != PQ_RUN_FRAME)
+    return pq_set_error(PQ_ERR_UNSUPPORTED, "this state keeps only its phase column: frame form only");
   if (form == PQ_RUN_FRAME && s->tab) {
     // eligibility first, so a refused run leaves the state untouched
     if (!s->xz_identity)

@@ -44,6 +44,9 @@ struct Req {
   std::vector<pq_op> ops;
   uint64_t nrec = 0, W = 0, bytes = 0;
   double cost = 0;  // estimated work: word groups x rows x ops
+  // frame form on a state that keeps only its R column: the server returns
+  // records, never the tableau, so X/Z need not exist per shot
+  bool lite = false;
   double t_submit = 0, t_start = 0, t_done = 0;
   int status = 0;  // 0 queued, 1 running, 2 done, < 0 failed (PQ_ERR_*)
   int slot = -1;
@@ -84,9 +87,21 @@ struct pq_server_s {
   void finish(Req *r);
 };
 
-// Device bytes a request reserves: tableau, records, noise planes, program.
+// Whether X/Z stay the same in every shot (no flagged H/S/CX/CZ), so the
+// frame form can run the request.
+static bool frame_eligible(const pq_op *ops, uint64_t nops) {
+  for (uint64_t i = 0; i < nops; ++i) {
+    const uint32_t c = ops[i].code;
+    if (c == PQ_OP_H_FLAG || c == PQ_OP_S_FLAG || c == PQ_OP_CX_FLAG || c == PQ_OP_CZ_FLAG)
+      return false;
+  }
+  return true;
+}
+
+// Device bytes a request reserves: tableau (its R column alone for a lite
+// request), records, noise planes, program.
 static uint64_t request_bytes(uint64_t n, uint64_t B, const pq_op *ops, uint64_t nops,
-                              uint64_t *nrec_out) {
+                              uint64_t *nrec_out, bool lite) {
   const uint64_t W = (B + 63) / 64, C = 2 * n + 1;
   uint64_t nrec = 0, nnoise = 0;
   for (uint64_t i = 0; i < nops; ++i) {
@@ -96,7 +111,8 @@ static uint64_t request_bytes(uint64_t n, uint64_t B, const pq_op *ops, uint64_t
       ++nnoise;
   }
   *nrec_out = nrec;
-  return W * C * C * 8 + nrec * 2 * W * 8 + nnoise * (2 * W * 8 + sizeof(pq::NoiseDesc)) +
+  return W * (lite ? 1 : C) * C * 8 + nrec * 2 * W * 8 +
+         nnoise * (2 * W * 8 + sizeof(pq::NoiseDesc)) +
          nops * sizeof(pq::DInstr) + 3 * W * 8 + 4096;
 }
 
@@ -123,10 +139,10 @@ Req *pq_server_s::next_fitting(double now) {
 
 int pq_server_s::admit(Req *r) {
   const hipStream_t s = streams[r->slot];
-  int rc = create_state_on(r->n, r->B, r->seed, s, &r->st);
+  int rc = create_state_on(r->n, r->B, r->seed, s, &r->st, r->lite);
   if (rc == PQ_OK) rc = create_program_on(r->st, r->ops.data(), r->ops.size(), &r->prog);
   int run_form = PQ_RUN_PERSISTENT;
-  if (rc == PQ_OK && form == PQ_RUN_FRAME) {
+  if (rc == PQ_OK && (r->lite || form == PQ_RUN_FRAME)) {
     // the frame form where the request allows it (no flagged H/S/CX/CZ),
     // with plans shared between requests of the same structure
     const std::string key = frame_plan_key(r->n, r->ops.data(), r->ops.size());
@@ -136,12 +152,14 @@ int pq_server_s::admit(Req *r) {
       r->prog->frame = it->second;
       ++plan_hits;
       run_form = PQ_RUN_FRAME;
-    } else if (build_frame_plan(r->prog) == PQ_OK) {
+    } else if ((rc = build_frame_plan(r->prog)) == PQ_OK) {
       // bounded cache: plans in use stay alive through their programs
       if (plans.size() >= kMaxCachedPlans) plans.clear();
       plans.emplace(key, r->prog->frame);
       ++plan_builds;
       run_form = PQ_RUN_FRAME;
+    } else if (!r->lite) {
+      rc = PQ_OK;  // a full state runs persistent instead
     }
     if (run_form == PQ_RUN_FRAME) ++frame_runs;
   }
@@ -273,7 +291,11 @@ extern "C" int pq_server_submit(pq_server sv, uint64_t n, uint64_t B, int64_t se
   r->W = (B + 63) / 64;
   r->seed = seed;
   r->ops.assign(ops, ops + nops);
-  r->bytes = request_bytes(n, B, ops, nops, &r->nrec);
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    r->lite = sv->form == PQ_RUN_FRAME && frame_eligible(ops, nops);
+  }
+  r->bytes = request_bytes(n, B, ops, nops, &r->nrec, r->lite);
   r->cost = (double)r->W * (double)(2 * n + 1) * (double)(nops + 1);
   if (r->bytes > sv->budget)
     return pq_set_error(PQ_ERR_OOM, "request can never fit the server's memory budget");
