@@ -346,6 +346,64 @@ def bench_serving(args):
     print(json.dumps(out))
 
 
+def bench_sampling(args):
+    """Many shots in one sampling state: a rotated surface-code memory-Z
+    experiment (distance d, d rounds, p=1e-3, seed 7; d=25 and 2^20 shots by
+    default) in a state that keeps only the phase column per 64-shot group,
+    run in the frame form.  One step = reset + whole circuit.  The full
+    bit-plane tableau of this job (W (2n+1)^2 words) would not fit one GPU."""
+    import numpy as np
+    import plaquette_simulator as ps
+    from plaquette_simulator import workloads as W
+
+    d, B = args.sampling_d, args.sampling_shots
+    ops, n = W.surface_code_memory_z(d, d, 1e-3)
+    words, R = (B + 63) // 64, 2 * n + 1
+    gates = int(np.isin(ops[:, 0], W.GATE_CODES).sum())
+    meas = int((ops[:, 0] == W.OP_M).sum())
+    nnoise = int((ops[:, 0] == W.OP_NOISE).sum())
+    st = ps.SamplingState(n, B, 7)
+    prog = ps.Program(st, ops)
+    t0 = time.perf_counter()
+    st.reset_clifford_state()
+    prog.run(st, "frame")  # builds the plan on the host, then runs
+    st.synchronize()
+    first = time.perf_counter() - t0
+    progress(f"sampling: d={d} {B} shots, first run {first:.2f} s")
+    steps = max(1, min(args.steps, 3))
+    prog.kernel_ms()
+    prog.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st.reset_clifford_state()
+        prog.run(st, "frame")
+    st.synchronize()
+    ms_step = 1e3 * (time.perf_counter() - t0) / steps
+    kms, launches = prog.kernel_ms()
+    prog.set_timing(False)
+    progress(f"sampling: {ms_step:.1f} ms/step")
+    out = {
+        "metric": f"gate-shots/s, d={d} surface-code memory, {B} shots in one sampling state",
+        "value": gates * B / (ms_step / 1e3),
+        "unit": "gate-shots/s",
+        "n_gpus": 1, "steps": steps, "warmup": 1, "ms_per_step": ms_step,
+        "higher_is_better": True, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": f"surface_code_memory_z_d{d}_sampling", "qubits": n,
+                   "rounds": d, "shots": B, "p": 1e-3, "seed": 7, "gates": gates,
+                   "measurements": meas, "noise_ops": nnoise, "form": "frame",
+                   "state": "phase column only"},
+        "shots_per_s": B / (ms_step / 1e3),
+        "meas_shots_per_s": meas * B / (ms_step / 1e3),
+        "kernel": "k_frame", "kernel_ms": kms / max(1, launches),
+        "first_run_s": first,
+        "device_bytes": {"phase_columns": words * R * 8,
+                         "records": meas * 2 * words * 8,
+                         "noise_planes": nnoise * 2 * words * 8},
+        "full_tableau_bytes": words * R * R * 8,
+    }
+    print(json.dumps(out))
+
+
 def cpu_baseline_prefix(w, target_s, shots=64):
     """The CPU oracle on `shots` shots and a prefix of the op list, the prefix
     scaled from a 1000-op calibration run to about `target_s` seconds (the
@@ -482,6 +540,10 @@ def main():
                     help="age limit (s) of the smallest-first admission policy")
     ap.add_argument("--serving-forms", default="persistent,frame",
                     help="execution forms to serve with (comma-separated)")
+    ap.add_argument("--sampling", action="store_true",
+                    help="measure many shots of one surface code in a sampling state instead")
+    ap.add_argument("--sampling-d", type=int, default=25)
+    ap.add_argument("--sampling-shots", type=int, default=1 << 20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this engine runs on a single MI355X "
@@ -498,6 +560,9 @@ def main():
         return
     if args.random_n1024:
         bench_random_n1024(args)
+        return
+    if args.sampling:
+        bench_sampling(args)
         return
 
     import plaquette_simulator as ps

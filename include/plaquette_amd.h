@@ -128,6 +128,14 @@ uint64_t pq_draw(uint64_t seed, uint64_t stream, uint64_t idx);
  * (CliffordState.hpp:97-139).  seed < 0 selects 213434232223 (:135). */
 int pq_create(uint64_t num_qubits, uint64_t batch_size, int64_t seed,
               int init_to_zero, pq_state *out);
+/* A sampling state: no reference counterpart.  It keeps only the phase
+ * column r per 64-shot group (W*(2n+1) words instead of W*(2n+1)^2) and runs
+ * frame-form programs (PQ_RUN_FRAME) from |0...0>, in single runs and in
+ * batches; records, pq_info, pq_reset and pq_destroy work as for any state,
+ * and every other entry point returns PQ_ERR_UNSUPPORTED.  For sampling many
+ * shots of circuits without flagged H/S/CX/CZ: d=25 surface-code memory with
+ * 1M shots needs 0.33 GB here against 820 GB as a full tableau. */
+int pq_create_sampling(uint64_t num_qubits, uint64_t batch_size, int64_t seed, pq_state *out);
 /* CliffordState(x, z, r, n, B, seed) from flat host vectors (:153-159). */
 int pq_create_from_host(const int32_t *x, const int32_t *z, const int32_t *r,
                         uint64_t num_qubits, uint64_t batch_size, int64_t seed,

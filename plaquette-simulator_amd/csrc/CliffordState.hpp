@@ -86,6 +86,14 @@ template <class Precision> class CliffordState {
                   "CliffordState");
   }
 
+  /// A sampling state (no reference counterpart): only the phase column per
+  /// 64-shot group is kept; it runs frame-form programs from |0...0> and
+  /// keeps their records (pq_create_sampling).
+  struct Sampling {};
+  CliffordState(Sampling, std::size_t num_qubits, std::size_t batch_size, int seed = -1) {
+    detail::check(pq_create_sampling(num_qubits, batch_size, seed, &h_), "CliffordState(sampling)");
+  }
+
   /// CliffordState.hpp:153-159.
   CliffordState(std::vector<Precision> &x, std::vector<Precision> &z,
                 std::vector<Precision> &r, std::size_t num_qubits,

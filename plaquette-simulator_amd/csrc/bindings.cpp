@@ -533,6 +533,13 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
       },
       py::arg("num_qubits"), py::arg("ops"),
       "Frame-form plan of an op list (host only): op stream, chunk table, final shared X/Z");
+  m.def(
+      "sampling_state",
+      [](size_t n, size_t B, int seed) { return new State(State::Sampling{}, n, B, seed); },
+      py::arg("num_qubits"), py::arg("batch_size"), py::arg("seed") = -1,
+      py::return_value_policy::take_ownership,
+      "A CliffordState that keeps only its phase column: frame-form programs from |0...0>, "
+      "records and reset only (W*(2n+1) words on the device instead of W*(2n+1)^2)");
   m.def("coin_threshold", &pq_coin_threshold);
   m.def("draw", &pq_draw);
   m.def("backend_name", &pq_backend_name);

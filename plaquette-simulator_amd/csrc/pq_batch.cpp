@@ -284,7 +284,8 @@ static int run_frame_batch(pq_batch b) {
   for (uint32_t c = 0; c < nc; ++c) {
     pq_state s = b->members[c].first;
     const FramePlan &F = *b->members[c].second->frame;
-    if (s->tab) HIPCHK(launch_frame_xz(b->stream, s->tab, geom_of(s), F.d_X, F.d_Z, F.nw));
+    if (s->tab && !s->r_only)
+      HIPCHK(launch_frame_xz(b->stream, s->tab, geom_of(s), F.d_X, F.d_Z, F.nw));
   }
   if (b->timed) HIPCHK(hipEventRecord(b->ev_b, b->stream));
   return PQ_OK;
@@ -295,6 +296,11 @@ extern "C" int pq_batch_run(pq_batch b, int form) {
   if (form != PQ_RUN_PERSISTENT && form != PQ_RUN_GRAPH && form != PQ_RUN_FRAME)
     return pq_set_error(PQ_ERR_ARG, "a batch runs in the persistent, graph or frame form");
   if (b->members.empty()) return PQ_OK;
+  if (form != PQ_RUN_FRAME)
+    for (auto &m : b->members)
+      if (m.first->r_only)
+        return pq_set_error(PQ_ERR_UNSUPPORTED,
+                            "a sampling state in the batch: the batch runs in the frame form only");
   if (form == PQ_RUN_FRAME) {
     if (!b->built) {
       int rc = build(b);

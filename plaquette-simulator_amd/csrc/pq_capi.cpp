@@ -102,6 +102,16 @@ pq_state_s::~pq_state_s() {
   if (stream && own_stream) (void)hipStreamDestroy(stream);
 }
 
+// Entry points that work on the full tableau refuse states that keep only
+// their phase column (sampling states: frame-form programs and records).
+#define FULL_ONLY(st)                                                                   \
+  do {                                                                                  \
+    if ((st) && (st)->r_only)                                                           \
+      return fail(PQ_ERR_UNSUPPORTED,                                                   \
+                  "a sampling state keeps only its phase column: frame-form programs, " \
+                  "records and reset only");                                            \
+  } while (0)
+
 static int check_device() {
   int c = 0;
   if (hipGetDeviceCount(&c) != hipSuccess || c < 1)
@@ -183,6 +193,15 @@ extern "C" int pq_create(uint64_t n, uint64_t B, int64_t seed, int init_zero,
   return PQ_OK;
 }
 
+extern "C" int pq_create_sampling(uint64_t n, uint64_t B, int64_t seed, pq_state *out) {
+  int rc = alloc_state(n, B, seed, nullptr, out, true);
+  if (rc) return rc;
+  pq_state s = *out;
+  if (s->tab) HIPCHK(launch_fill_r_identity(s->stream, s->tab, s->W, (uint32_t)s->R));
+  s->xz_identity = true;
+  return PQ_OK;
+}
+
 extern "C" int pq_create_from_host(const int32_t *x, const int32_t *z,
                                    const int32_t *r, uint64_t n, uint64_t B,
                                    int64_t seed, pq_state *out) {
@@ -213,12 +232,16 @@ extern "C" int pq_info(pq_state s, uint64_t *n, uint64_t *B, int64_t *seed,
 
 extern "C" int pq_reset(pq_state s) {
   ARGCHK(s, "null state");
-  if (s->tab) HIPCHK(launch_fill_identity(s->stream, s->tab, geom_of(s)));
+  if (s->tab && s->r_only)
+    HIPCHK(launch_fill_r_identity(s->stream, s->tab, s->W, (uint32_t)s->R));
+  else if (s->tab)
+    HIPCHK(launch_fill_identity(s->stream, s->tab, geom_of(s)));
   s->xz_identity = true;
   return PQ_OK;
 }
 
 extern "C" int pq_clone(pq_state src, pq_state *out) {
+  FULL_ONLY(src);
   ARGCHK(src, "null state");
   int rc = pq_create(src->n, src->B, src->seed, 0, out);
   if (rc) return rc;
@@ -231,6 +254,8 @@ extern "C" int pq_clone(pq_state src, pq_state *out) {
 }
 
 extern "C" int pq_copy_from(pq_state dst, pq_state src) {
+  FULL_ONLY(dst);
+  FULL_ONLY(src);
   ARGCHK(dst && src, "null state");
   ARGCHK(dst->n == src->n && dst->B == src->B, "DeviceToDevice needs equal n and B");
   if (!dst->tab) return PQ_OK;
@@ -359,6 +384,7 @@ int plain_gate(pq_state s, uint32_t code, uint64_t q0, uint64_t q1) {
 }
 
 extern "C" int pq_gate(pq_state s, uint32_t code, uint64_t q0, uint64_t q1) {
+  FULL_ONLY(s);
   return plain_gate(s, code, q0, q1);
 }
 
@@ -422,6 +448,7 @@ static int stage_host_ints(pq_state s, const int32_t *v, uint64_t *plane) {
 }
 
 extern "C" int pq_gate_flag_record(pq_state s, uint32_t code, uint64_t q, uint64_t rec) {
+  FULL_ONLY(s);
   ARGCHK(s, "null state");
   ARGCHK(is_flag_op(code), "not a flagged op code");
   if (q >= s->n) return fail(PQ_ERR_ARG, "qubit index out of range");
@@ -433,6 +460,7 @@ extern "C" int pq_gate_flag_record(pq_state s, uint32_t code, uint64_t q, uint64
 
 extern "C" int pq_gate2_flag_record(pq_state s, uint32_t code, uint64_t q0, uint64_t q1,
                                     uint64_t rec) {
+  FULL_ONLY(s);
   ARGCHK(s, "null state");
   ARGCHK(is_flag2_op(code), "not a flagged two-qubit op code");
   if (q0 >= s->n || q1 >= s->n || q0 == q1)
@@ -443,6 +471,7 @@ extern "C" int pq_gate2_flag_record(pq_state s, uint32_t code, uint64_t q0, uint
 
 extern "C" int pq_gate_flag_host(pq_state s, uint32_t code, uint64_t q,
                                  const int32_t *flags) {
+  FULL_ONLY(s);
   ARGCHK(s, "null state");
   ARGCHK(is_flag_op(code), "not a flagged op code");
   ARGCHK(flags || s->B == 0, "flags is null");
@@ -457,6 +486,7 @@ extern "C" int pq_gate_flag_host(pq_state s, uint32_t code, uint64_t q,
 
 extern "C" int pq_gate2_flag_host(pq_state s, uint32_t code, uint64_t q0, uint64_t q1,
                                   const int32_t *flags) {
+  FULL_ONLY(s);
   ARGCHK(s, "null state");
   ARGCHK(is_flag2_op(code), "not a flagged two-qubit op code");
   ARGCHK(flags || s->B == 0, "flags is null");
@@ -480,6 +510,7 @@ int plain_random_flag(pq_state s, uint32_t code, uint64_t q0, uint64_t q1, uint3
 }
 
 extern "C" int pq_gate_prob(pq_state s, uint32_t code, uint64_t q0, uint64_t q1, float prob) {
+  FULL_ONLY(s);
   ARGCHK(s, "null state");
   ARGCHK(is_gate(code), "not a gate op code");
   if (q0 >= s->n) return fail(PQ_ERR_ARG, "qubit index out of range");
@@ -503,6 +534,7 @@ int plain_noise(pq_state s, uint64_t q, uint32_t t0, uint32_t t1, uint32_t t2) {
 }
 
 extern "C" int pq_noise(pq_state s, uint64_t q, uint32_t t0, uint32_t t1, uint32_t t2) {
+  FULL_ONLY(s);
   return plain_noise(s, q, t0, t1, t2);
 }
 
@@ -522,6 +554,7 @@ int plain_measure(pq_state s, uint64_t q, uint32_t T, uint64_t rec,
 
 extern "C" int pq_measure(pq_state s, uint64_t q, float bias, int64_t rec,
                           const int32_t *coins, uint64_t *out_rec) {
+  FULL_ONLY(s);
   ARGCHK(s, "null state");
   if (q >= s->n) return fail(PQ_ERR_ARG, "qubit index out of range");
   uint64_t id;
@@ -556,6 +589,7 @@ static uint64_t io_chunk_shots(const pq_state_s *s) {
 }
 
 extern "C" int pq_device_to_host(pq_state s, int32_t *x, int32_t *z, int32_t *r) {
+  FULL_ONLY(s);
   ARGCHK(s, "null state");
   if (!s->tab) return PQ_OK;
   ARGCHK(x && z && r, "output buffer is null");
@@ -583,6 +617,7 @@ extern "C" int pq_device_to_host(pq_state s, int32_t *x, int32_t *z, int32_t *r)
 
 extern "C" int pq_host_to_device(pq_state s, const int32_t *x, const int32_t *z,
                                  const int32_t *r) {
+  FULL_ONLY(s);
   ARGCHK(s, "null state");
   if (!s->tab) return PQ_OK;
   ARGCHK(x && z && r, "input buffer is null");
@@ -612,6 +647,7 @@ extern "C" int pq_host_to_device(pq_state s, const int32_t *x, const int32_t *z,
 }
 
 extern "C" int pq_checksum(pq_state s, int64_t b0, int64_t b1, int64_t *out) {
+  FULL_ONLY(s);
   ARGCHK(s && out, "null argument");
   if (b1 < 0) b1 = (int64_t)s->B;
   ARGCHK(b0 >= 0 && b0 <= b1 && (uint64_t)b1 <= s->B, "shot range out of bounds");
@@ -628,6 +664,7 @@ extern "C" int pq_checksum(pq_state s, int64_t b0, int64_t b1, int64_t *out) {
 }
 
 extern "C" int pq_read_planes(pq_state s, uint64_t *host_words) {
+  FULL_ONLY(s);
   ARGCHK(s && host_words, "null argument");
   if (!s->tab) return PQ_OK;
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -637,6 +674,7 @@ extern "C" int pq_read_planes(pq_state s, uint64_t *host_words) {
 
 extern "C" int pq_device_handles(pq_state s, void **tableau, void **records,
                                  uint64_t *words, uint64_t *rows, void **stream) {
+  FULL_ONLY(s);
   ARGCHK(s, "null state");
   if (tableau) *tableau = s->tab;
   if (records) *records = s->recs;
@@ -671,6 +709,7 @@ extern "C" int pq_event_elapsed_ms(pq_state s, int a, int b, float *ms) {
 // diagnostics mirroring the reference's test-only functors
 // ---------------------------------------------------------------------------
 extern "C" int pq_row_multiply(pq_state s, uint64_t i, uint64_t k) {
+  FULL_ONLY(s);
   ARGCHK(s, "null state");
   ARGCHK(i < s->R && k < s->R, "row out of range");
   if (!s->tab) return PQ_OK;
@@ -680,6 +719,7 @@ extern "C" int pq_row_multiply(pq_state s, uint64_t i, uint64_t k) {
 }
 
 extern "C" int pq_row_product_sign(pq_state s, uint64_t i, uint64_t k, int32_t *out) {
+  FULL_ONLY(s);
   ARGCHK(s && out, "null argument");
   ARGCHK(i < s->R && k < s->R, "row out of range");
   if (!s->tab) return PQ_OK;
@@ -692,6 +732,7 @@ extern "C" int pq_row_product_sign(pq_state s, uint64_t i, uint64_t k, int32_t *
 }
 
 extern "C" int pq_measure_determined(pq_state s, uint64_t q, int32_t *out) {
+  FULL_ONLY(s);
   ARGCHK(s && out, "null argument");
   if (q >= s->n) return fail(PQ_ERR_ARG, "qubit index out of range");
   if (!s->tab) return PQ_OK;

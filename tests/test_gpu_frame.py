@@ -210,3 +210,43 @@ def test_three_forms_agree_at_n2000(ps, W):
             ra, da = ref.get_record(i)
             rb, db = st.get_record(i)
             assert np.array_equal(ra, rb) and np.array_equal(da, db), f"{form}: record {i}"
+
+
+def test_sampling_state(ps, W):
+    """A sampling state (phase column only) gives the records of a full state,
+    alone and in a frame batch, across a reset; entry points that need the
+    full tableau refuse it."""
+    ops, n = W.surface_code_memory_z(5, 5, 0.01)
+    B, seed = 300, 21
+    full = ps.CliffordState(n, B, seed, True)
+    prog_f = ps.Program(full, ops)
+    samp = ps.SamplingState(n, B, seed)
+    prog_s = ps.Program(samp, ops)
+    for it in range(2):
+        full.reset_clifford_state()
+        samp.reset_clifford_state()
+        prog_f.run(full, "persistent")
+        prog_s.run(samp, "frame")
+        assert samp.num_records == full.num_records
+        for i in range(full.num_records):
+            ra, da = full.get_record(i)
+            rb, db = samp.get_record(i)
+            assert np.array_equal(ra, rb) and np.array_equal(da, db), f"run {it} record {i}"
+    for call in (lambda: samp.device_to_host_numpy(), lambda: samp.apply_hadamard_gate(0),
+                 lambda: samp.measure_qubit(0), lambda: samp.read_planes(),
+                 lambda: samp.run_program(ops, "persistent")):
+        with pytest.raises(RuntimeError):
+            call()
+    # in a frame batch beside full states
+    batch = ps.Batch()
+    s2 = ps.SamplingState(n, B, seed)
+    batch.add(s2, ps.Program(s2, ops))
+    f2 = ps.CliffordState(n, 130, 5, True)
+    batch.add(f2, ps.Program(f2, ops))
+    batch.run("frame")
+    for i in range(full.num_records):
+        ra, da = full.get_record(i)
+        rb, db = s2.get_record(i)
+        assert np.array_equal(ra, rb) and np.array_equal(da, db), f"batch record {i}"
+    with pytest.raises(RuntimeError):
+        batch.run("persistent")
