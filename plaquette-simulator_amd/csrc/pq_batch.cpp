@@ -242,7 +242,7 @@ static int run_frame_batch(pq_batch b) {
     if (it != plans.end()) {
       p->frame = it->second;
     } else {
-      int rc = build_frame_plan(p);
+      int rc = build_frame_plan(p, UINT64_MAX);
       if (rc) return rc;
       plans.emplace(std::move(key), p->frame);
     }
@@ -265,6 +265,7 @@ static int run_frame_batch(pq_batch b) {
     bc.nseq0 = s->noise_seq;
     bc.noise = p->d_noise;
     bc.nnoise = (uint32_t)p->nnoise;
+    bc.group_major = 1;
     b->h_fargs[c] = frame_args(s, p);
     if (!s->tab) b->h_fargs[c].W = 0;
     lds = std::max(lds, p->frame->lds_bytes);
@@ -331,7 +332,7 @@ extern "C" int pq_batch_run(pq_batch b, int form) {
     bc.nseq0 = s->noise_seq;
     bc.noise = p->d_noise;
     bc.nnoise = (uint32_t)p->nnoise;
-    bc.pad = 0;
+    bc.group_major = 0;
     s->meas_seq += p->nmeas;
     s->xz_identity = false;
     s->noise_seq += p->nnoise;

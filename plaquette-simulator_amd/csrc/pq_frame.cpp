@@ -1,47 +1,52 @@
 // Frame form of a program: circuits whose X/Z tableau is the same in every
-// shot run with X/Z simulated once, here on the host, and only the per-shot
-// phase column R on the GPU.
+// shot run with X/Z simulated once, here on the host, and only per-shot words
+// on the GPU.
 //
 // X and Z change only through H, S, CX, CZ and measurements, and those act on
 // every shot alike (GateFunctors.hpp; MeasurementFunctors.hpp:225-265 picks
 // the pivot from X alone), so without flagged H/S/CX/CZ a state that starts
-// with the same X/Z in every shot keeps it.  Every update of R is then an XOR
-// of words whose positions X/Z fix:
-//  * a gate, Pauli or noise op XORs into R[i] a constant, a noise mask or a
-//    record's result bits, on the rows i its rule selects (the gate rules of
-//    GateFunctors.hpp read x_q, z_q of row i);
-//  * a random measurement (MeasureRandom, :225-265) sets R[p] = R[n+p],
-//    R[n+p] = coin and R[i] ^= R[p] ^ phase_i on the rows i it rowsums, with
-//    phase_i a function of X/Z only (RowProductSign, :97-129);
-//  * a determined measurement (MeasureDetermined, :189-201) gives
-//    R[2n] = c ^ XOR of R[n+i] over its rowsum rows.
-// The compiler below runs the shared X/Z once from the |0...0> tableau,
-// records those rows, constants and phases, and packs them into a stream the
-// GPU kernel runs per 64-shot word group (k_frame in pq_kernels.hip).
+// with the same X/Z in every shot keeps it.  The phase column R then differs
+// between shots only through noise, random flags, record flags and
+// measurement coins; simulate() below keeps it as R = Rv ^ cst ^ <E, P> (a
+// per-shot part Rv, a compile-time constant cst and a per-shot Pauli frame
+// E) and compiles the program into a stream the GPU kernel runs per 64-shot
+// word group (k_frame in pq_kernels.hip), with Rv, E and the record words
+// held in LDS.
 //
 // Stream layout (uint32 words), one op after another, each with an 8-word
 // header {kind, a, b, c, d, e, f, g}:
-//  kXor:    a = T rows, b = entries; then rows[T], offsets[T+1] (relative to
-//           the entry array), entries: LDS word index of the mask XORed in;
-//           ops of one XOR block commute, so a block splits freely.
+//  kLayer:  a = T tasks, b = body words; then offsets[T+1] (in updates,
+//           relative to the body), a pad word if the body would start at an
+//           odd word of the chunk, and the bodies.  A body is a list of
+//           updates, two words each: 16-bit LDS word indices {dst | s0 << 16,
+//           s1 | s2 << 16}, LDS word dst ^= s0 ^ s1 ^ s2 (unused sources name
+//           the zero word), in order.  Tasks touch disjoint frame words, one
+//           thread runs each.
 //  kRandom: a = p, b = record slot, c = measurement index in the program,
-//           d = coin threshold, e = rec_l index or kNone, f = entries; then
-//           entries row | phase << 31 (rows other than p and n+p).
+//           d = coin threshold, e = rec_l index or kNone, f = entries,
+//           g = LDS word of E.x[q]; then the rowsum rows (other than p, n+p).
 //  kDet:    a = G measurements (<= 64); then per measurement {slot, rec_l
-//           index or kNone, c, row offset}, the end offset, and the row lists
-//           (rows n+i).  The last one's result is R[2n].
+//           index or kNone, LDS word of E.x[q] | c << 31 | last << 30, row
+//           offset}, the end offset, and the row lists (rows n+i).  The
+//           program's last determined measurement (last = 1) puts its Rv part
+//           into Rv[2n].
+//  kXor:    a = T rows, b = entries; then rows[T], offsets[T+1] (relative to
+//           the entry array), entries: LDS words XORed into Rv[row] (the
+//           closing R = Rv ^ cst ^ <E, P> update).
 // The stream is cut into chunks that fit the kernel's LDS staging buffer;
 // each chunk also names the noise numbers [k0, k0+nk) whose masks it reads.
 // With two staging buffers (when they fit the LDS) chunk c uses buffer c % 2
 // and the kernel loads chunk c+1 while it runs chunk c; a chunk's noise
-// entries name the mask words of its own buffer.
+// sources name the mask words of its own buffer.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "plaquette_amd.h"
@@ -183,27 +188,74 @@ void cols_to_rows(const std::vector<uint64_t> &cols, uint32_t R, uint32_t n, uin
     }
 }
 
-// Mask source of an XOR entry: the constant all-ones word, a record's
-// result word (by rec_l index) or a noise mask part (2k + part: part 0 is
-// the X-or-Y mask, also a random flag; part 1 the Y-or-Z mask).
-enum : uint32_t { kSrcConst = 0, kSrcRec = 1, kSrcNoise = 2 };
-struct Entry {
-  uint32_t row, src, arg;
-};
+// Sources of the words a frame update XORs in: LDS words by absolute index
+// (frame words, record words, the all-ones word), or noise mask parts
+// kNoiseSrc | (2k + part) (part 0 the X-or-Y mask, also a random flag; part 1
+// the Y-or-Z mask), placed in the noise window when the packer picks it.
+constexpr uint32_t kNoiseSrc = 0x80000000u;
+constexpr uint32_t kMaxSrc = 3;       // sources per update
+constexpr uint32_t kTaskWords = 256;  // stream words of one task at most
+constexpr uint32_t kRecPool = 128;    // record words handed out before reuse
 
-struct Logical {
-  uint32_t kind = 0;  // kFrameXor / kFrameRandom / kFrameDet
-  std::vector<Entry> ents;      // xor
+// dst ^= XOR of s[0..ns) (LDS words)
+struct TEnt {
+  uint32_t dst = 0, ns = 0;
+  uint32_t s[kMaxSrc] = {};
+};
+// Updates one thread runs in order; the tasks of one layer touch disjoint
+// frame words, so they run in parallel.
+struct Task {
+  std::vector<TEnt> ents;
+  int64_t kmin = -1, kmax = -1;  // noise numbers read
+  uint32_t words = 0;            // stream words of the body
+};
+struct Meas {
+  uint32_t exq = 0;  // LDS word of the frame's x part on the measured qubit
   uint32_t p = 0, slot = 0, mi = 0, T = 0, ridx = kNone, c = 0;
-  std::vector<uint32_t> rows;   // random: row | phase << 31; det: rows
+  bool scratch = false;  // determined: the program's last, writes Rv[2n]
+  std::vector<uint32_t> rows;
+};
+// One barrier-separated step of the kernel: a layer of frame tasks, one
+// random measurement or a batch of determined measurements.
+struct Step {
+  uint32_t kind = kFrameLayer;
+  std::vector<Task> tasks;
+  std::vector<Meas> ms;
+  uint32_t words = 0;  // determined batch: stream words
 };
 
 bool frame_flag_breaks_xz(uint32_t c) {
   return c == PQ_OP_H_FLAG || c == PQ_OP_S_FLAG || c == PQ_OP_CX_FLAG || c == PQ_OP_CZ_FLAG;
 }
 
-// The shared-X/Z run of `ops` from the identity tableau.
-int simulate(pq_program_s *p, FramePlan &F, std::vector<Logical> &out) {
+// The shared-X/Z run of `ops` from the identity tableau, scheduled into
+// steps; `fin` receives the closing R update ({row, LDS word} pairs).
+//
+// Per shot the R column is kept split as R[i] = Rv[i] ^ cst[i] ^ <E, P_i>:
+// Rv the part the GPU holds, cst a constant the compiler tracks (the same in
+// every shot), E a Pauli frame (x and z words per qubit, 64 shots a word) and
+// <E, P_i> its symplectic product with row i's Pauli (bit 1 where they
+// anticommute).  This holds because every R update is one of:
+//  * a gate's phase rule (GateFunctors.hpp): a function of X/Z only, so a
+//    constant, into cst;
+//  * a Pauli, noise or flagged Pauli P (:304-307, 324-327, 373-376): R[i]
+//    flips where P anticommutes with row i, i.e. E ^= P (constant Paulis go
+//    into cst instead); a gate U then maps every row P_i to U P_i U^dag,
+//    which keeps <E, P_i> when E is mapped to U E U^dag alike (H swaps the
+//    x and z words, S: z ^= x, CX: x_t ^= x_c, z_c ^= z_t, CZ: z_b ^= x_a,
+//    z_a ^= x_b);
+//  * a random measurement (MeasureRandom, MeasurementFunctors.hpp:225-265)
+//    with pivot p: rowsums multiply row i by row n+p, so <E, .> adds up and
+//    Rv, cst follow the reference's sign rule; the new row n+p = Z_q gets
+//    R = coin, i.e. Rv[n+p] = coin ^ E.x[q] with cst 0;
+//  * a determined measurement (MeasureDetermined, :189-201): the product of
+//    its stabilizer rows is Z_q, so result = XOR Rv[rows] ^ (c ^ XOR
+//    cst[rows]) ^ E.x[q].
+// At the end R[i] = Rv[i] ^ cst[i] ^ <E, P_i> is written out.  The frame
+// words of a noisy circuit change with each gate and noise op, O(1) words
+// each, instead of the R words of every row the op anticommutes with.
+int simulate(pq_program_s *p, FramePlan &F, std::vector<Step> &steps,
+             std::vector<std::pair<uint32_t, uint32_t>> &fin) {
   const uint32_t n = (uint32_t)p->n, R = 2 * n + 1;
   XZ t;
   t.init_identity(n);
@@ -230,12 +282,16 @@ int simulate(pq_program_s *p, FramePlan &F, std::vector<Logical> &out) {
         last_read[{o.slot, it == producer.end() ? -1 : it->second}] = (int64_t)i;
       }
     }
-    std::vector<uint32_t> free_words;
+    // Up to kRecPool words are handed out fresh, then the longest-dead word
+    // is reused first: a word reused at once would order a measurement
+    // after the flag that read the word's previous value, and so keep
+    // consecutive measurements out of one batch.
+    std::deque<uint32_t> free_words;
     uint32_t high = 0;
     auto take = [&]() {
-      if (!free_words.empty()) {
-        const uint32_t w = free_words.back();
-        free_words.pop_back();
+      if (!free_words.empty() && high >= kRecPool) {
+        const uint32_t w = free_words.front();
+        free_words.pop_front();
         return w;
       }
       return high++;
@@ -270,6 +326,206 @@ int simulate(pq_program_s *p, FramePlan &F, std::vector<Logical> &out) {
     }
     F.nref = high;
   }
+  // LDS words: Rv [R], the frame [2n], records [nref], the all-ones and the
+  // zero word, then the staging buffers
+  const uint32_t E0 = R;
+  F.lds_rec = R + 2 * n;
+  F.lds_const = F.lds_rec + F.nref;  // the all-ones word, then the zero word
+  F.lds_noise = F.lds_const + 2;
+
+  // ---- scheduling of frame updates and measurements into steps ----
+  // Frame slot s lives at LDS word E0 + s; sx[q] / sz[q] are the slots of
+  // qubit q's x / z word (H swaps them, which costs no update).
+  std::vector<uint32_t> sx(n), sz(n);
+  for (uint32_t q = 0; q < n; ++q) {
+    sx[q] = q;
+    sz[q] = n + q;
+  }
+  std::vector<int64_t> last(2 * n, -1);  // last step touching each slot
+  std::vector<int64_t> own(2 * n, -1);   // its task there (-1: a measurement)
+  std::vector<uint8_t> nz(2 * n, 0);     // slot may be nonzero
+  std::vector<int64_t> rec_w(F.nref, -1), rec_r(F.nref, -1);
+  const int64_t krange = std::max<int64_t>(1, F.nb_words / 4);  // noise span of one task
+  auto is_frame = [&](uint32_t v) { return !(v & kNoiseSrc) && v >= E0 && v < E0 + 2 * n; };
+  auto is_rec = [&](uint32_t v) {
+    return !(v & kNoiseSrc) && v >= F.lds_rec && v < F.lds_rec + F.nref;
+  };
+  auto ent_noise = [&](const TEnt &e, int64_t &lo, int64_t &hi) {
+    for (uint32_t i = 0; i < e.ns; ++i)
+      if (e.s[i] & kNoiseSrc) {
+        const int64_t k = (e.s[i] & ~kNoiseSrc) / 2;
+        lo = lo < 0 ? k : std::min(lo, k);
+        hi = std::max(hi, k);
+      }
+  };
+  // e joins the latest update of the same word when no update after it
+  // reads that word or writes one of e's sources
+  auto append = [&](Task &T, const TEnt &e) {
+    int64_t lim = 8;
+    for (int64_t j = (int64_t)T.ents.size() - 1; j >= 0 && lim > 0; --j, --lim) {
+      TEnt &f = T.ents[(size_t)j];
+      if (f.dst == e.dst) {
+        if (f.ns + e.ns <= kMaxSrc) {
+          for (uint32_t i = 0; i < e.ns; ++i) f.s[f.ns++] = e.s[i];
+          return;
+        }
+        break;
+      }
+      bool clash = false;
+      for (uint32_t i = 0; i < f.ns; ++i) clash |= f.s[i] == e.dst;
+      for (uint32_t i = 0; i < e.ns; ++i) clash |= e.s[i] == f.dst;
+      if (clash) break;
+    }
+    T.ents.push_back(e);
+    T.words += 2;
+  };
+  // the frame updates of one op, in order
+  auto add_frame = [&](std::initializer_list<TEnt> list) {
+    TEnt keep[4];
+    uint32_t nk = 0;
+    for (TEnt e : list) {
+      uint32_t k = 0;
+      for (uint32_t i = 0; i < e.ns; ++i)
+        if (!is_frame(e.s[i]) || nz[e.s[i] - E0]) e.s[k++] = e.s[i];  // zero words add nothing
+      e.ns = k;
+      if (!k) continue;
+      nz[e.dst - E0] = 1;
+      keep[nk++] = e;
+    }
+    if (!nk) return;
+    int64_t smax = -1, rmax = -1, lo = -1, hi = -1;
+    uint32_t words = 0;
+    for (uint32_t j = 0; j < nk; ++j) {
+      const TEnt &e = keep[j];
+      smax = std::max(smax, last[e.dst - E0]);
+      for (uint32_t i = 0; i < e.ns; ++i) {
+        if (is_frame(e.s[i])) smax = std::max(smax, last[e.s[i] - E0]);
+        if (is_rec(e.s[i])) rmax = std::max(rmax, rec_w[e.s[i] - F.lds_rec]);
+      }
+      ent_noise(e, lo, hi);
+      words += 2;
+    }
+    int64_t step = -1, task = -1;
+    // fuse into the task that last touched these words when every word it
+    // touches was last touched there or in an earlier step
+    if (smax >= 0 && steps[(size_t)smax].kind == kFrameLayer && rmax < smax) {
+      int64_t tk = -1;
+      bool ok = true;
+      auto see = [&](uint32_t slot) {
+        if (last[slot] != smax) return;
+        if (own[slot] < 0 || (tk >= 0 && own[slot] != tk)) ok = false;
+        tk = own[slot];
+      };
+      for (uint32_t j = 0; j < nk; ++j) {
+        see(keep[j].dst - E0);
+        for (uint32_t i = 0; i < keep[j].ns; ++i)
+          if (is_frame(keep[j].s[i])) see(keep[j].s[i] - E0);
+      }
+      if (ok && tk >= 0) {
+        const Task &T = steps[(size_t)smax].tasks[(size_t)tk];
+        const int64_t nlo = T.kmin < 0 ? lo : (lo < 0 ? T.kmin : std::min(T.kmin, lo));
+        const int64_t nhi = std::max(T.kmax, hi);
+        if (T.words + words <= kTaskWords && (nlo < 0 || nhi - nlo < krange)) {
+          step = smax;
+          task = tk;
+        }
+      }
+    }
+    if (step < 0) {
+      // the first layer after every step these words and records were in
+      const int64_t lb = std::max(smax, rmax) + 1;
+      for (int64_t i = lb; i < (int64_t)steps.size(); ++i)
+        if (steps[(size_t)i].kind == kFrameLayer) {
+          step = i;
+          break;
+        }
+      if (step < 0) {
+        steps.emplace_back();
+        step = (int64_t)steps.size() - 1;
+      }
+      steps[(size_t)step].tasks.emplace_back();
+      task = (int64_t)steps[(size_t)step].tasks.size() - 1;
+    }
+    Task &T = steps[(size_t)step].tasks[(size_t)task];
+    for (uint32_t j = 0; j < nk; ++j) {
+      const TEnt &e = keep[j];
+      append(T, e);
+      last[e.dst - E0] = step;
+      own[e.dst - E0] = task;
+      for (uint32_t i = 0; i < e.ns; ++i) {
+        if (is_frame(e.s[i])) {
+          last[e.s[i] - E0] = step;
+          own[e.s[i] - E0] = task;
+        }
+        if (is_rec(e.s[i])) {
+          int64_t &r = rec_r[e.s[i] - F.lds_rec];
+          r = std::max(r, step);
+        }
+      }
+    }
+    if (lo >= 0) {
+      T.kmin = T.kmin < 0 ? lo : std::min(T.kmin, lo);
+      T.kmax = std::max(T.kmax, hi);
+    }
+  };
+  auto ent1 = [](uint32_t dst, uint32_t src) {
+    TEnt e;
+    e.dst = dst;
+    e.ns = 1;
+    e.s[0] = src;
+    return e;
+  };
+  // Measurement steps.  Random measurements rewrite Rv rows and take a new
+  // step at the end.  A determined one reads its rows and writes only the
+  // scratch row (whose value matters only after the last of them, which is
+  // flagged to write it), so it joins the first determined batch after the
+  // steps that wrote its rows, its frame word, its record word and its slot.
+  std::vector<int64_t> rv_w(R, -1);  // last step writing each Rv row
+  std::unordered_map<uint32_t, int64_t> slot_step;
+  int64_t sc_step = -1;
+  size_t sc_idx = 0;
+  auto place_meas = [&](Meas &&m, bool random, uint32_t exq_slot, uint32_t stream_words) {
+    int64_t lb = last[exq_slot];
+    if (m.ridx != kNone) lb = std::max(lb, std::max(rec_w[m.ridx], rec_r[m.ridx]));
+    auto it = slot_step.find(m.slot);
+    if (it != slot_step.end()) lb = std::max(lb, it->second);
+    int64_t step = -1;
+    if (!random) {
+      for (uint32_t r : m.rows) lb = std::max(lb, rv_w[r]);
+      for (int64_t i = lb + 1; i < (int64_t)steps.size(); ++i) {
+        Step &s = steps[(size_t)i];
+        if (s.kind != kFrameDet || s.ms.size() >= 64 || s.words + stream_words > F.sb_words)
+          continue;
+        bool dup = false;
+        for (const Meas &o : s.ms) dup |= o.slot == m.slot;
+        if (!dup) {
+          step = i;
+          break;
+        }
+      }
+    }
+    if (step < 0) {
+      steps.emplace_back();
+      step = (int64_t)steps.size() - 1;
+      steps[(size_t)step].kind = random ? kFrameRandom : kFrameDet;
+      steps[(size_t)step].words = 8 + 1;
+    }
+    Step &s = steps[(size_t)step];
+    s.words += stream_words;
+    if (m.ridx != kNone) rec_w[m.ridx] = step;
+    last[exq_slot] = std::max(last[exq_slot], step);
+    own[exq_slot] = -1;
+    slot_step[m.slot] = step;
+    if (random) {
+      for (uint32_t r : m.rows) rv_w[r] = step;
+      rv_w[m.p] = rv_w[n + m.p] = step;
+    } else {
+      sc_step = step;
+      sc_idx = s.ms.size();
+    }
+    s.ms.push_back(std::move(m));
+  };
+
   // Between measurements X/Z are held by columns (gates are word operations
   // over rows); measurements work on rows.
   Cols cl;
@@ -290,62 +546,57 @@ int simulate(pq_program_s *p, FramePlan &F, std::vector<Logical> &out) {
     cols_to_rows(cl.Z, R, n, rw, t.Z, t.nw);
     by_cols = false;
   };
-  std::vector<uint64_t> flip(rw, 0);  // rows whose r the gates so far flip
-  const uint64_t last_mask = (R % 64) ? ((1ull << (R % 64)) - 1) : ~0ull;
-  Logical cur;
-  cur.kind = kFrameXor;
-  // rows set in `bits` (rw words) each get mask source (src, arg)
-  auto add_rows = [&](const uint64_t *bits, uint32_t src, uint32_t arg) {
-    for (uint32_t j = 0; j < rw; ++j) {
-      uint64_t v = bits[j] & (j + 1 == rw ? last_mask : ~0ull);
-      while (v) {
-        const uint32_t b = (uint32_t)__builtin_ctzll(v);
-        v &= v - 1;
-        cur.ents.push_back(Entry{j * 64 + b, src, arg});
-      }
-    }
-  };
-  auto close_xor = [&]() {
-    add_rows(flip.data(), kSrcConst, 0);
+  std::vector<uint64_t> flip(rw, 0);  // rows whose cst the gates since the last fold flip
+  std::vector<uint8_t> cst(R, 0);
+  auto fold_flip = [&]() {
+    for (uint32_t i = 0; i < R; ++i)
+      if ((flip[i / 64] >> (i % 64)) & 1) cst[i] ^= 1;
     std::fill(flip.begin(), flip.end(), 0);
-    if (!cur.ents.empty()) out.push_back(std::move(cur));
-    cur = Logical();
-    cur.kind = kFrameXor;
   };
-  std::vector<uint64_t> tmp(rw), tmp2(rw), sel(rw);
+  std::vector<uint64_t> tmp(rw), tmp2(rw);
   bool seg_rows = false;  // the current gate segment runs row by row
-  // Paulis, noise and flagged Paulis on qubit q, given the rows with z_q
-  // (zr) and with x_q (xr): X flips r where z_q, Z where x_q, Y where one of
-  // them (GateFunctors.hpp:304-307, 324-327, 373-376)
   uint32_t kk = 0, mi = 0;
-  auto apply_pauli_like = [&](const pq_op &o, size_t oi, const uint64_t *zr, const uint64_t *xr) {
+  // constant Paulis on qubit q, given the rows with z_q (zr) and with x_q
+  // (xr): X flips r where z_q, Z where x_q, Y where one of them
+  auto const_pauli = [&](uint32_t code, const uint64_t *zr, const uint64_t *xr) {
+    for (uint32_t j = 0; j < rw; ++j)
+      flip[j] ^= code == PQ_OP_X ? zr[j] : code == PQ_OP_Z ? xr[j] : (zr[j] ^ xr[j]);
+  };
+  // noise and flagged Paulis: frame updates (X part into x, Z part into z)
+  auto frame_pauli = [&](const pq_op &o, size_t oi) {
+    const uint32_t q = o.q0;
+    if (o.code == PQ_OP_NOISE) {
+      const uint32_t k = kk++;
+      add_frame({ent1(E0 + sx[q], kNoiseSrc | (2 * k)), ent1(E0 + sz[q], kNoiseSrc | (2 * k + 1))});
+      return;
+    }
+    uint32_t src = F.lds_rec + op_ridx[oi];
+    if (o.slot == PQ_FLAG_RANDOM) src = kNoiseSrc | (2 * kk++);
+    if (o.code == PQ_OP_X_FLAG)
+      add_frame({ent1(E0 + sx[q], src)});
+    else if (o.code == PQ_OP_Z_FLAG)
+      add_frame({ent1(E0 + sz[q], src)});
+    else
+      add_frame({ent1(E0 + sx[q], src), ent1(E0 + sz[q], src)});
+  };
+  // frame map of the gates (conjugation, phases dropped)
+  auto frame_gate = [&](const pq_op &o) {
+    const uint32_t q = o.q0, q1 = o.q1;
     switch (o.code) {
-      case PQ_OP_X:
-        for (uint32_t j = 0; j < rw; ++j) flip[j] ^= zr[j];
+      case PQ_OP_H:
+        std::swap(sx[q], sz[q]);
         break;
-      case PQ_OP_Y:
-        for (uint32_t j = 0; j < rw; ++j) flip[j] ^= zr[j] ^ xr[j];
+      case PQ_OP_S:
+        add_frame({ent1(E0 + sz[q], E0 + sx[q])});
         break;
-      case PQ_OP_Z:
-        for (uint32_t j = 0; j < rw; ++j) flip[j] ^= xr[j];
+      case PQ_OP_CX:
+        add_frame({ent1(E0 + sx[q1], E0 + sx[q]), ent1(E0 + sz[q], E0 + sz[q1])});
         break;
-      case PQ_OP_NOISE: {
-        const uint32_t k = kk++;
-        add_rows(zr, kSrcNoise, 2 * k);      // X or Y part
-        add_rows(xr, kSrcNoise, 2 * k + 1);  // Y or Z part
+      case PQ_OP_CZ:
+        add_frame({ent1(E0 + sz[q1], E0 + sx[q]), ent1(E0 + sz[q], E0 + sx[q1])});
         break;
-      }
-      default: {  // X/Y/Z_FLAG
-        uint32_t src = kSrcRec, arg = op_ridx[oi];
-        if (o.slot == PQ_FLAG_RANDOM) {
-          src = kSrcNoise;
-          arg = 2 * kk++;
-        }
-        for (uint32_t j = 0; j < rw; ++j)
-          sel[j] = o.code == PQ_OP_X_FLAG ? zr[j] : o.code == PQ_OP_Z_FLAG ? xr[j] : (zr[j] ^ xr[j]);
-        add_rows(sel.data(), src, arg);
+      default:
         break;
-      }
     }
   };
   for (size_t oi = 0; oi < p->ops.size(); ++oi) {
@@ -355,11 +606,26 @@ int simulate(pq_program_s *p, FramePlan &F, std::vector<Logical> &out) {
       return pq_set_error(PQ_ERR_UNSUPPORTED,
                           ("frame form: op " + std::to_string(oi) +
                            " is a flagged H/S/CX/CZ (X/Z would differ between shots)").c_str());
+    switch (o.code) {
+      case PQ_OP_H: case PQ_OP_S: case PQ_OP_CX: case PQ_OP_CZ:
+        frame_gate(o);
+        break;
+      case PQ_OP_NOISE: case PQ_OP_X_FLAG: case PQ_OP_Y_FLAG: case PQ_OP_Z_FLAG:
+        frame_pauli(o, oi);
+        continue;  // X/Z and cst unchanged
+      case PQ_OP_X: case PQ_OP_Y: case PQ_OP_Z: case PQ_OP_M:
+        break;
+      default:
+        return pq_set_error(PQ_ERR_ARG, "frame form: unknown op code");
+    }
     if (o.code != PQ_OP_M && !by_cols && !seg_rows) {
       // a new gate segment: by columns when its gates outweigh the two
       // transposes (64x64 blocks of X and Z, there and back)
       uint64_t g = 0;
-      for (size_t k = oi; k < nops && p->ops[k].code != PQ_OP_M; ++k) ++g;
+      for (size_t k = oi; k < nops && p->ops[k].code != PQ_OP_M; ++k)
+        if (p->ops[k].code != PQ_OP_NOISE && !(p->ops[k].code >= PQ_OP_X_FLAG &&
+                                                p->ops[k].code <= PQ_OP_Z_FLAG))
+          ++g;
       const uint64_t by_row = g * R * 8, by_col = g * rw * 8 + 4ull * rw * t.nw * 600;
       if (by_col < by_row)
         need_cols();
@@ -368,7 +634,7 @@ int simulate(pq_program_s *p, FramePlan &F, std::vector<Logical> &out) {
     }
     if (o.code == PQ_OP_M) seg_rows = false;
     if (seg_rows) {
-      // the same updates row by row
+      // X/Z and the phase flips row by row
       auto bit = [&](uint32_t i) { flip[i / 64] ^= 1ull << (i % 64); };
       switch (o.code) {
         case PQ_OP_H:
@@ -402,20 +668,16 @@ int simulate(pq_program_s *p, FramePlan &F, std::vector<Logical> &out) {
             t.set(t.Z, i, q, zc ^ xt);
           }
           break;
-        case PQ_OP_X: case PQ_OP_Y: case PQ_OP_Z: case PQ_OP_NOISE:
-        case PQ_OP_X_FLAG: case PQ_OP_Y_FLAG: case PQ_OP_Z_FLAG: {
-          // rows by z_q (X part) and x_q (Z part) as column bitsets
+        default: {  // X, Y, Z
           std::fill(tmp.begin(), tmp.end(), 0);
           std::fill(tmp2.begin(), tmp2.end(), 0);
           for (uint32_t i = 0; i < R; ++i) {
             if (t.z(i, q)) tmp[i / 64] |= 1ull << (i % 64);
             if (t.x(i, q)) tmp2[i / 64] |= 1ull << (i % 64);
           }
-          apply_pauli_like(o, oi, tmp.data(), tmp2.data());
+          const_pauli(o.code, tmp.data(), tmp2.data());
           break;
         }
-        default:
-          return pq_set_error(PQ_ERR_ARG, "frame form: unknown op code");
       }
       continue;
     }
@@ -454,14 +716,14 @@ int simulate(pq_program_s *p, FramePlan &F, std::vector<Logical> &out) {
         }
         break;
       }
-      case PQ_OP_X: case PQ_OP_Y: case PQ_OP_Z: case PQ_OP_NOISE:
-      case PQ_OP_X_FLAG: case PQ_OP_Y_FLAG: case PQ_OP_Z_FLAG:
-        apply_pauli_like(o, oi, cl.z(q), cl.x(q));
+      case PQ_OP_X: case PQ_OP_Y: case PQ_OP_Z:
+        const_pauli(o.code, cl.z(q), cl.x(q));
         break;
       case PQ_OP_M: {
-        close_xor();
         need_rows();
-        Logical m;
+        fold_flip();
+        Meas m;
+        m.exq = E0 + sx[q];
         m.slot = o.slot;
         m.mi = mi++;
         m.T = o.t0;
@@ -473,40 +735,60 @@ int simulate(pq_program_s *p, FramePlan &F, std::vector<Logical> &out) {
             break;
           }
         if (piv != kNone) {  // MeasureRandom, :225-265
-          m.kind = kFrameRandom;
           m.p = piv;
+          const uint8_t cs = cst[n + piv];
           t.row_copy(piv, n + piv);
           t.row_zero(n + piv);
           t.set(t.Z, n + piv, q, true);
           for (uint32_t i = 0; i < 2 * n; ++i) {
             if (i == piv || i == n + piv || !t.x(i, q)) continue;
-            const bool ph = t.phase(i, piv);
+            cst[i] ^= cs ^ (t.phase(i, piv) ? 1 : 0);
             t.row_xor(i, piv);
-            m.rows.push_back(i | (ph ? 0x80000000u : 0u));
+            m.rows.push_back(i);
           }
+          cst[piv] = cs;
+          cst[n + piv] = 0;
+          const uint32_t words = 8 + (uint32_t)m.rows.size();
+          place_meas(std::move(m), true, sx[q], words);
         } else {  // MeasureDetermined, :189-201
-          m.kind = kFrameDet;
           t.row_zero(2 * n);
-          bool c = false;
+          uint8_t c = 0;
           for (uint32_t i = 0; i < n; ++i) {
             if (!t.x(i, q)) continue;
-            c ^= t.phase(2 * n, n + i);
+            c ^= (t.phase(2 * n, n + i) ? 1 : 0) ^ cst[n + i];
             t.row_xor(2 * n, n + i);
             m.rows.push_back(n + i);
           }
-          m.c = c ? 1u : 0u;
+          // the product is Z_q, whose frame part is E.x[q]
+          for (uint32_t j = 0; j < t.nw; ++j) {
+            const uint64_t zq = (q / 64 == j) ? (1ull << (q % 64)) : 0ull;
+            if (t.X[(size_t)(2 * n) * t.nw + j] != 0 || t.Z[(size_t)(2 * n) * t.nw + j] != zq)
+              return pq_set_error(PQ_ERR_HIP, "frame form: determined product is not Z_q");
+          }
+          cst[2 * n] = c;
+          m.c = c;
+          const uint32_t words = 4 + (uint32_t)m.rows.size();
+          place_meas(std::move(m), false, sx[q], words);
         }
-        out.push_back(std::move(m));
         break;
       }
       default:
-        return pq_set_error(PQ_ERR_ARG, "frame form: unknown op code");
+        break;
     }
   }
-  close_xor();
   need_rows();
+  fold_flip();
+  if (sc_step >= 0) steps[(size_t)sc_step].ms[sc_idx].scratch = true;
   if (kk != p->nnoise || mi != p->nmeas)
     return pq_set_error(PQ_ERR_HIP, "frame form: noise/measurement numbering mismatch");
+  // closing update: R[i] = Rv[i] ^ cst[i] ^ <E, P_i>
+  for (uint32_t i = 0; i < R; ++i) {
+    if (cst[i]) fin.emplace_back(i, F.lds_const);
+    for (uint32_t j = 0; j < n; ++j) {
+      if (t.z(i, j) && nz[sx[j]]) fin.emplace_back(i, E0 + sx[j]);
+      if (t.x(i, j) && nz[sz[j]]) fin.emplace_back(i, E0 + sz[j]);
+    }
+  }
   F.final_identity = t.is_identity();
   F.X = t.X;
   F.Z = t.Z;
@@ -514,8 +796,8 @@ int simulate(pq_program_s *p, FramePlan &F, std::vector<Logical> &out) {
   return PQ_OK;
 }
 
-// Packs logical ops into chunks of at most F.sb_words stream words and
-// F.nb_words noise-mask words.
+// Packs steps into chunks of at most F.sb_words stream words, each with a
+// noise window of at most F.nb_words mask words.
 struct Packer {
   FramePlan &F;
   uint32_t chunk_begin = 0;
@@ -538,174 +820,205 @@ struct Packer {
   void ensure(uint32_t words) {
     if (room() < words) close_chunk();
   }
-  uint32_t lds_index(const Entry &e) const {
-    if (e.src == kSrcConst) return F.lds_const;
-    if (e.src == kSrcRec) return F.lds_rec + e.arg;
+  uint32_t src_index(uint32_t v) const {
+    if (!(v & kNoiseSrc)) return v;
     const uint32_t buf = (uint32_t)(F.chunks.size() / 4) % F.nbufs;  // the open chunk's
-    return F.lds_noise + buf * F.nb_words + (e.arg - 2 * (uint32_t)k0);
+    return F.lds_noise + buf * F.nb_words + ((v & ~kNoiseSrc) - 2 * (uint32_t)k0);
   }
-  // One XOR op of `es` (all noise numbers inside the open chunk's window).
-  void emit_xor(std::vector<Entry> &es) {
-    std::stable_sort(es.begin(), es.end(),
-                     [](const Entry &a, const Entry &b) { return a.row < b.row; });
-    std::vector<uint32_t> rows, offs;
-    for (size_t i = 0; i < es.size(); ++i)
-      if (i == 0 || es[i].row != es[i - 1].row) {
-        rows.push_back(es[i].row);
-        offs.push_back((uint32_t)i);
-      }
-    offs.push_back((uint32_t)es.size());
-    const uint32_t T = (uint32_t)rows.size();
-    const uint32_t hdr[8] = {kFrameXor, T, (uint32_t)es.size(), 0, 0, 0, 0, 0};
+  void emit_layer(const std::vector<const Task *> &ts) {
+    uint32_t body = 0;
+    for (const Task *t : ts) body += t->words;
+    const uint32_t T = (uint32_t)ts.size();
+    const bool pad = (used() + 8 + T + 1) & 1;  // updates start at an even word of the chunk
+    const uint32_t hdr[8] = {kFrameLayer, T, body, 0, 0, 0, 0, 0};
     F.stream.insert(F.stream.end(), hdr, hdr + 8);
-    F.stream.insert(F.stream.end(), rows.begin(), rows.end());
-    F.stream.insert(F.stream.end(), offs.begin(), offs.end());
-    for (const Entry &e : es) F.stream.push_back(lds_index(e));
-    es.clear();
-  }
-  void add_xor(std::vector<Entry> ents) {
-    // noise entries in noise-number order, so a block splits at window ends
-    std::stable_sort(ents.begin(), ents.end(), [](const Entry &a, const Entry &b) {
-      const int64_t ka = a.src == kSrcNoise ? (int64_t)(a.arg / 2) : -1;
-      const int64_t kb = b.src == kSrcNoise ? (int64_t)(b.arg / 2) : -1;
-      return ka < kb;
-    });
-    const int64_t nbk = F.nb_words / 2;  // noise numbers per window
-    std::vector<Entry> part;
-    std::vector<uint32_t> seen(F.R, 0);
-    uint32_t gen = 1, T = 0;
-    auto flush = [&]() {
-      if (!part.empty()) emit_xor(part);
-      ++gen;
-      T = 0;
-    };
-    ensure(64);
-    for (const Entry &e : ents) {
-      if (e.src == kSrcNoise) {
-        const int64_t k = e.arg / 2;
-        if (k0 >= 0 && k >= k0 + nbk) {  // past the window: new chunk
-          flush();
-          close_chunk();
-        }
-        if (k0 < 0) {
-          k0 = k;
-          k1 = k + 1;
-        } else if (k + 1 > k1) {
-          k1 = k + 1;
-        }
-      }
-      // header, rows, offsets and entries if this entry joins
-      const uint32_t newrow = seen[e.row] != gen ? 1u : 0u;
-      const uint32_t need = 8 + 2 * (T + newrow) + 1 + (uint32_t)part.size() + 1;
-      if (need > room()) {
-        flush();
-        close_chunk();
-        if (e.src == kSrcNoise) {
-          k0 = e.arg / 2;
-          k1 = k0 + 1;
-        }
-      }
-      if (seen[e.row] != gen) {
-        seen[e.row] = gen;
-        ++T;
-      }
-      part.push_back(e);
+    uint32_t off = 0;
+    for (const Task *t : ts) {
+      F.stream.push_back(off);
+      off += t->words / 2;
     }
-    flush();
+    F.stream.push_back(off);
+    if (pad) F.stream.push_back(0);
+    const uint32_t zero = F.lds_const + 1;
+    for (const Task *t : ts)
+      for (const TEnt &e : t->ents) {
+        uint32_t v[kMaxSrc];
+        for (uint32_t i = 0; i < kMaxSrc; ++i) v[i] = i < e.ns ? src_index(e.s[i]) : zero;
+        F.stream.push_back(e.dst | (v[0] << 16));
+        F.stream.push_back(v[1] | (v[2] << 16));
+      }
   }
-  void add_random(const Logical &m) {
+  // A layer in as few pieces as room and noise windows allow (its tasks are
+  // independent, so any split runs them all before the next step).
+  int add_layer(const std::vector<Task> &tasks) {
+    std::vector<const Task *> rest;
+    for (const Task &t : tasks) rest.push_back(&t);
+    std::stable_sort(rest.begin(), rest.end(),
+                     [](const Task *a, const Task *b) { return a->kmin < b->kmin; });
+    const int64_t nbk = F.nb_words / 2;  // noise numbers per window
+    while (!rest.empty()) {
+      std::vector<const Task *> take, keep;
+      uint32_t words = 8 + 1 + 1;  // header, end offset, alignment pad
+      bool full = false;
+      for (const Task *t : rest) {
+        const bool fit_w = t->kmin < 0 || k0 < 0 || (t->kmin >= k0 && t->kmax < k0 + nbk);
+        if (full || !fit_w || words + 1 + t->words > room()) {
+          full |= fit_w && words + 1 + t->words > room();
+          keep.push_back(t);
+          continue;
+        }
+        take.push_back(t);
+        words += 1 + t->words;
+        if (t->kmin >= 0) {
+          if (k0 < 0) k0 = t->kmin;
+          k1 = std::max<int64_t>(k1, t->kmax + 1);
+        }
+      }
+      if (take.empty()) {
+        if (used() == 0)
+          return pq_set_error(PQ_ERR_HIP, "frame form: a task exceeds the stream buffer");
+        close_chunk();
+        continue;
+      }
+      emit_layer(take);
+      rest.swap(keep);
+      if (!rest.empty()) close_chunk();
+    }
+    return PQ_OK;
+  }
+  void add_random(const Meas &m) {
     ensure(8 + (uint32_t)m.rows.size());
     const uint32_t hdr[8] = {kFrameRandom, m.p, m.slot, m.mi, m.T, m.ridx,
-                             (uint32_t)m.rows.size(), 0};
+                             (uint32_t)m.rows.size(), m.exq};
     F.stream.insert(F.stream.end(), hdr, hdr + 8);
     F.stream.insert(F.stream.end(), m.rows.begin(), m.rows.end());
   }
-  void add_det_batch(const std::vector<const Logical *> &ms) {
+  void add_det_batch(const std::vector<Meas> &ms) {
     uint32_t total = 0;
-    for (const Logical *m : ms) total += (uint32_t)m->rows.size();
+    for (const Meas &m : ms) total += (uint32_t)m.rows.size();
     ensure(8 + 4 * (uint32_t)ms.size() + 1 + total);
     const uint32_t hdr[8] = {kFrameDet, (uint32_t)ms.size(), 0, 0, 0, 0, 0, 0};
     F.stream.insert(F.stream.end(), hdr, hdr + 8);
     uint32_t off = 0;
-    for (const Logical *m : ms) {
-      F.stream.push_back(m->slot);
-      F.stream.push_back(m->ridx);
-      F.stream.push_back(m->c);
+    for (const Meas &m : ms) {
+      F.stream.push_back(m.slot);
+      F.stream.push_back(m.ridx);
+      F.stream.push_back(m.exq | (m.c << 31) | (m.scratch ? 1u << 30 : 0u));
       F.stream.push_back(off);
-      off += (uint32_t)m->rows.size();
+      off += (uint32_t)m.rows.size();
     }
     F.stream.push_back(off);
-    for (const Logical *m : ms) F.stream.insert(F.stream.end(), m->rows.begin(), m->rows.end());
+    for (const Meas &m : ms) F.stream.insert(F.stream.end(), m.rows.begin(), m.rows.end());
+  }
+  // The closing update ({row, LDS word} pairs, by row) as XOR ops; a row
+  // whose entries outgrow the room continues in the next op.
+  void add_xor(const std::vector<std::pair<uint32_t, uint32_t>> &es) {
+    std::vector<uint32_t> rows, offs, ents;
+    auto flush = [&]() {
+      if (rows.empty()) return;
+      offs.push_back((uint32_t)ents.size());
+      const uint32_t hdr[8] = {kFrameXor, (uint32_t)rows.size(), (uint32_t)ents.size(),
+                               0, 0, 0, 0, 0};
+      F.stream.insert(F.stream.end(), hdr, hdr + 8);
+      F.stream.insert(F.stream.end(), rows.begin(), rows.end());
+      F.stream.insert(F.stream.end(), offs.begin(), offs.end());
+      F.stream.insert(F.stream.end(), ents.begin(), ents.end());
+      rows.clear();
+      offs.clear();
+      ents.clear();
+    };
+    ensure(64);
+    size_t i = 0;
+    while (i < es.size()) {
+      const bool newrow = rows.empty() || rows.back() != es[i].first;
+      const uint32_t T = (uint32_t)rows.size() + (newrow ? 1u : 0u);
+      if (8 + 2 * T + 1 + (uint32_t)ents.size() + 1 > room()) {
+        flush();
+        close_chunk();
+        continue;
+      }
+      if (newrow) {
+        rows.push_back(es[i].first);
+        offs.push_back((uint32_t)ents.size());
+      }
+      ents.push_back(es[i].second);
+      ++i;
+    }
+    flush();
   }
 };
 
 }  // namespace
 
-int build_frame_plan(pq_program_s *p) {
+// Staging sizes, smallest first: {noise window words, stream words, buffers}.
+struct Staging {
+  uint32_t nb, sb, bufs;
+};
+constexpr Staging kStaging[] = {{512, 2048, 1}, {1024, 4096, 1}, {2048, 4096, 1},
+                                {2048, 8192, 1}, {2048, 8192, 2}};
+constexpr uint32_t kFrameGroupsPerCu = 4;  // resident groups that hide a group's latencies
+constexpr uint64_t kCus = 256;             // compute units of an MI355X
+
+int build_frame_plan(pq_program_s *p, uint64_t groups) {
   auto F = std::make_shared<FramePlan>();
   const uint32_t n = (uint32_t)p->n, R = 2 * n + 1;
   F->R = R;
-  std::vector<Logical> ops;
-  int rc = simulate(p, *F, ops);
+  // the compiler sizes tasks and determined batches for the smallest staging
+  F->nb_words = kStaging[0].nb;
+  F->sb_words = kStaging[0].sb;
+  std::vector<Step> steps;
+  std::vector<std::pair<uint32_t, uint32_t>> fin;
+  int rc = simulate(p, *F, steps, fin);
   if (rc) return rc;
-  // LDS of one wave, in 8-byte words: R, records read by flags, the
-  // all-ones word, the noise-mask window, then the stream buffer (uint32)
-  F->lds_rec = R;
-  F->lds_const = R + F->nref;
-  F->lds_noise = F->lds_const + 1;
-  // a window of up to kFrameNoiseWords / 2 noise ops (none without noise)
-  F->nb_words = (uint32_t)std::min<uint64_t>(kFrameNoiseWords, (2 * p->nnoise + 63) / 64 * 64);
   // the stream buffer holds the largest single measurement op
   uint32_t need = 64;
-  for (const Logical &m : ops)
-    if (m.kind != kFrameXor) need = std::max<uint32_t>(need, 8 + 4 + 1 + (uint32_t)m.rows.size());
-  F->sb_words = std::max<uint32_t>(kFrameStreamWords, (need + 63) / 64 * 64);
-  // ... and 64 words of partial results per wave (up to 16 waves); two
-  // staging buffers (noise window and stream) when they fit
-  auto lds_for = [&](uint64_t nbufs) {
-    return 8ull * (F->lds_noise + nbufs * F->nb_words) + 4ull * nbufs * F->sb_words +
-           8ull * 64 * 16;
+  for (const Step &s : steps)
+    for (const Meas &m : s.ms) need = std::max<uint32_t>(need, 8 + 4 + 1 + (uint32_t)m.rows.size());
+  need = (need + 63) / 64 * 64;
+  // LDS in bytes: Rv, frame, records, the all-ones word, the noise windows
+  // (none without noise), the stream buffers and 64 words of partial results
+  // per wave.  Each group runs a chain of short dependent steps, so several
+  // groups per CU hide each other's latencies: the largest staging that
+  // keeps as many groups per CU as the launch has (up to kFrameGroupsPerCu)
+  // is taken; a launch of at most one group per CU gets the largest staging,
+  // double-buffered.
+  const uint64_t nb_cap = (2 * p->nnoise + 63) / 64 * 64;
+  auto lds_of = [&](const Staging &o) {
+    const uint64_t nb = std::min<uint64_t>(o.nb, nb_cap), sb = std::max(o.sb, need);
+    return 8ull * (F->lds_noise + o.bufs * nb) + 4ull * o.bufs * sb + 8ull * kFrameThreads;
   };
-  F->nbufs = lds_for(2) <= kFrameLdsMax ? 2 : 1;
-  const uint64_t lds = lds_for(F->nbufs);
-  if (lds > kFrameLdsMax)
+  auto per_cu = [&](const Staging &o) {
+    const uint64_t l = lds_of(o);
+    return l <= kFrameLdsMax ? kFrameLdsMax / l : 0;
+  };
+  const uint64_t want = std::max<uint64_t>(1, groups / kCus + (groups % kCus != 0));
+  const uint64_t target = std::min<uint64_t>(std::min<uint64_t>(kFrameGroupsPerCu, want),
+                                             per_cu(kStaging[0]));
+  if (!target)
     return pq_set_error(PQ_ERR_UNSUPPORTED,
-                        ("frame form needs " + std::to_string(lds) +
+                        ("frame form needs " + std::to_string(lds_of(kStaging[0])) +
                          " bytes of LDS per word group (too many qubits or flag records)").c_str());
-  F->lds_bytes = (uint32_t)lds;
+  const Staging *pick = &kStaging[0];
+  for (const Staging &o : kStaging)
+    if (per_cu(o) >= target) pick = &o;
+  F->nb_words = (uint32_t)std::min<uint64_t>(pick->nb, nb_cap);
+  F->sb_words = std::max(pick->sb, need);
+  F->nbufs = pick->bufs;
+  F->lds_bytes = (uint32_t)lds_of(*pick);
   Packer pk(*F);
-  for (size_t i = 0; i < ops.size();) {
-    const Logical &m = ops[i];
-    if (m.kind == kFrameXor) {
-      pk.add_xor(m.ents);
-      ++i;
-    } else if (m.kind == kFrameRandom) {
-      pk.add_random(m);
-      ++i;
+  for (const Step &s : steps) {
+    if (s.kind == kFrameLayer) {
+      if ((rc = pk.add_layer(s.tasks))) return rc;
+    } else if (s.kind == kFrameRandom) {
+      pk.add_random(s.ms[0]);
+      ++F->nrandom;
     } else {
-      // consecutive determined measurements, at most 64, distinct slots,
-      // fitting one chunk
-      std::vector<const Logical *> batch;
-      uint32_t words = 8 + 1;
-      std::vector<uint32_t> slots;
-      while (i < ops.size() && ops[i].kind == kFrameDet && batch.size() < 64) {
-        const Logical &d = ops[i];
-        const uint32_t w = 4 + (uint32_t)d.rows.size();
-        if (words + w > F->sb_words) break;
-        if (std::find(slots.begin(), slots.end(), d.slot) != slots.end()) break;
-        slots.push_back(d.slot);
-        batch.push_back(&d);
-        words += w;
-        ++i;
-      }
-      pk.add_det_batch(batch);
+      pk.add_det_batch(s.ms);
+      F->ndet += s.ms.size();
     }
   }
+  pk.add_xor(fin);
   pk.close_chunk();
-  for (const Logical &m : ops) {
-    if (m.kind == kFrameRandom) ++F->nrandom;
-    if (m.kind == kFrameDet) ++F->ndet;
-  }
   p->frame = std::move(F);
   return PQ_OK;
 }
@@ -764,6 +1077,7 @@ FrameArgs frame_args(pq_state s, pq_program_s *p) {
   a.R = (uint32_t)s->R;
   a.C = (uint32_t)s->C;
   a.W = s->W;
+  a.nnoise = p->nnoise;
   a.lds_rec = F.lds_rec;
   a.lds_const = F.lds_const;
   a.lds_noise = F.lds_noise;
@@ -783,7 +1097,7 @@ int run_frame(pq_state s, pq_program_s *p) {
     return pq_set_error(PQ_ERR_UNSUPPORTED,
                         "frame form runs from the |0...0> tableau: reset the state first");
   if (!p->frame) {
-    int rc = build_frame_plan(p);
+    int rc = build_frame_plan(p, s->W);
     if (rc) return rc;
   }
   int rc = upload_frame_plan(p, s->stream);
@@ -795,8 +1109,8 @@ int run_frame(pq_state s, pq_program_s *p) {
   s->meas_seq += p->nmeas;
   s->noise_seq += p->nnoise;
   if (p->nnoise)
-    HIPCHK(launch_program_noise(s->stream, p->d_planes, p->d_noise, (uint32_t)p->nnoise, s->W,
-                                s->params_dev));
+    HIPCHK(launch_frame_noise(s->stream, p->d_planes, p->d_noise, (uint32_t)p->nnoise, s->W,
+                              s->params_dev));
   if (p->timing_on) {
     if (p->ev_used + 2 > p->ev.size()) {
       hipEvent_t e0, e1;
@@ -832,7 +1146,7 @@ extern "C" int pq_frame_plan(uint64_t n, const pq_op *ops, uint64_t nops, uint64
   if (rc) return rc;
   rc = compile_program(&prog);
   if (rc) return rc;
-  rc = build_frame_plan(&prog);
+  rc = build_frame_plan(&prog, UINT64_MAX);
   if (rc) return rc;
   const FramePlan &F = *prog.frame;
   const uint64_t v[16] = {F.stream.size(), F.chunks.size() / 4, F.preload.size() / 2, F.nw,

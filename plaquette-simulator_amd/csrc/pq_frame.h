@@ -13,9 +13,9 @@
 struct pq_program_s;
 struct pq_state_s;
 
-// Per-wave LDS: noise-mask window (8-byte words) and stream buffer (uint32).
-constexpr uint32_t kFrameNoiseWords = 2048;
-constexpr uint32_t kFrameStreamWords = 8192;
+// LDS of one word group: its words (Rv, frame, records) plus staging
+// buffers for the noise-mask window (8-byte words) and the stream (uint32),
+// sized per plan (build_frame_plan) for the most groups per CU.
 constexpr uint64_t kFrameLdsMax = 160 * 1024;
 
 struct FramePlan {
@@ -36,7 +36,9 @@ struct FramePlan {
   ~FramePlan();
 };
 
-int build_frame_plan(pq_program_s *p);
+// `groups`: the word groups one launch of the plan runs (UINT64_MAX: many,
+// as in batches and the server); it sizes the LDS staging, not the results.
+int build_frame_plan(pq_program_s *p, uint64_t groups);
 int upload_frame_plan(pq_program_s *p, hipStream_t stream);
 // Kernel arguments of program p's frame plan on state s (at s's current
 // measurement sequence number).

@@ -1527,11 +1527,62 @@ __global__ __launch_bounds__(MAXT) void k_batch_t(
 // Noise planes of every circuit of a batch (blockIdx.y = circuit).  The
 // block row's waves are laid out as (noise-op lane) x (word-group lane): one
 // stream key per op and wave, one draw per lane per (op, word group).
+// Group-major noise planes [W][nk][2] (the frame form): wave task t covers
+// word group w = t / nkb and noise ops [32 kb, 32 kb + 32), kb = t % nkb;
+// lanes 2i and 2i+1 end up holding op 32 kb + i's X and Z mask words, so a
+// task writes one contiguous run of up to 512 bytes.  Op k's draws use stream
+// 2 (nseq0 + k) + 1, as by plane.
+__device__ __forceinline__ void frame_noise_tasks(uint64_t *__restrict__ planes,
+                                                  const NoiseDesc *__restrict__ desc,
+                                                  uint32_t nk, uint64_t W, uint64_t seed,
+                                                  uint64_t nseq0, uint64_t wave,
+                                                  uint64_t nwaves) {
+  const uint32_t lane = lane_id();
+  const uint64_t lane_g = lane_golden();
+  const uint64_t nkb = (nk + 31) / 32;
+  for (uint64_t t = wave; t < W * nkb; t += nwaves) {
+    const uint64_t w = t / nkb;
+    const uint32_t k0 = (uint32_t)(t % nkb) * 32;
+    const uint32_t cnt = min(32u, nk - k0);
+    // lane i < cnt derives op k0 + i's stream key
+    const uint64_t key_l = stream_key(seed, 2 * (nseq0 + k0 + min(lane, cnt - 1)) + 1);
+    uint64_t out = 0;
+    for (uint32_t i = 0; i < cnt; ++i) {
+      const NoiseDesc d = desc[k0 + i];
+      const uint32_t klo = __builtin_amdgcn_readlane((uint32_t)key_l, i);
+      const uint32_t khi = __builtin_amdgcn_readlane((uint32_t)(key_l >> 32), i);
+      const uint64_t key = ((uint64_t)khi << 32) | klo;
+      bool bx, bz;
+      noise_bits(d, draw_word_lane(key, w, lane_g), bx, bz);
+      const uint64_t mx = __ballot(bx);
+      const uint64_t mz = __ballot(bz);
+      if (lane == 2 * i) out = mx;
+      if (lane == 2 * i + 1) out = mz;
+    }
+    if (lane < 2 * cnt) planes[(w * nk + k0) * 2 + lane] = out;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_frame_noise(uint64_t *__restrict__ planes,
+                                                     const NoiseDesc *__restrict__ desc,
+                                                     uint32_t nk, uint64_t W,
+                                                     const ProgParams *__restrict__ pp) {
+  const uint64_t nwv = (uint64_t)gridDim.x * (blockDim.x / 64);
+  const uint64_t wave = blockIdx.x * (uint64_t)(blockDim.x / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  frame_noise_tasks(planes, desc, nk, W, pp->seed, pp->nseq0, wave, nwv);
+}
+
 __global__ __launch_bounds__(256) void k_batch_noise(const pq::BatchCircuit *__restrict__ circ) {
   const pq::BatchCircuit &bc = circ[blockIdx.y];
   const uint64_t W = bc.a.W;
   const uint64_t nwv = (uint64_t)gridDim.x * (blockDim.x / 64);
   const uint64_t wave = blockIdx.x * (uint64_t)(blockDim.x / 64) + (threadIdx.x >> 6);
+  if (bc.group_major) {
+    frame_noise_tasks(const_cast<uint64_t *>(bc.a.planes), bc.noise, bc.nnoise, W, bc.seed,
+                      bc.nseq0, blockIdx.x * (uint64_t)(blockDim.x / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6),
+                      nwv);
+    return;
+  }
   // waves along word groups (about 16 groups each, so one stream key serves
   // many draws) and along noise ops
   const uint64_t ww = W / 16 < 1 ? 1 : (W / 16 < nwv ? W / 16 : nwv);
@@ -1618,23 +1669,20 @@ __global__ void k_rowsum(uint64_t *__restrict__ tab, uint32_t n, uint32_t R,
 }
 
 // ---------------------------------------------------------------------------
-// Frame form (X/Z shared by all shots): one wave per 64-shot word group runs
-// the compiled op stream on the group's R column, held in LDS.  The stream,
-// its layout and the LDS map are described in pq_frame.cpp; the shared X/Z
-// themselves are written by k_frame_xz.
+// Frame form (X/Z shared by all shots): one workgroup per 64-shot word group
+// runs the compiled op stream on the group's R column and Pauli frame, held
+// in LDS.  The stream, its layout and the LDS map are described in
+// pq_frame.cpp; the shared X/Z themselves are written by k_frame_xz.
 // ---------------------------------------------------------------------------
 typedef __attribute__((address_space(3))) void lds_void;
 
 __device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// The kFrameThreads threads of a block share one word group: its R column,
-// record words, the noise-mask window and the stream chunk live in LDS, the
-// threads split each op's rows (ops touch distinct rows, so within an op no
-// two threads write one word) and a barrier separates ops.
-#ifndef PQ_FRAME_THREADS
-#define PQ_FRAME_THREADS 512
-#endif
-constexpr uint32_t kFrameThreads = PQ_FRAME_THREADS;
+// The kFrameThreads threads of a block share one word group: its R column
+// (the per-shot part Rv), Pauli frame, record words, the noise-mask window
+// and the stream chunk live in LDS, the threads split each op's rows or
+// tasks (within an op no two threads write one word) and a barrier separates
+// ops.
 constexpr uint32_t kFrameWaves = kFrameThreads / 64;
 
 __device__ __forceinline__ void frame_word_group(const FrameArgs &a, uint64_t w) {
@@ -1647,8 +1695,8 @@ __device__ __forceinline__ void frame_word_group(const FrameArgs &a, uint64_t w)
   uint32_t *sbuf0 = reinterpret_cast<uint32_t *>(dyn + a.lds_noise + a.nbufs * a.nb_words);
   uint64_t *part = reinterpret_cast<uint64_t *>(sbuf0 + a.nbufs * a.sb_words);  // [waves][64]
   uint64_t *Tr = a.tab + w * a.col_stride + a.r_off;  // the group's R column
-  // chunk c's stream words and noise masks (planes 2k0 .. 2(k0+nk)-1 of this
-  // group) into staging buffer c % nbufs by LDS DMA, one wave-wide 4-byte
+  // chunk c's stream words and noise masks (ops k0 .. k0+nk-1 of this group,
+  // contiguous) into staging buffer c % nbufs by LDS DMA, one wave-wide 4-byte
   // load per 64 dwords, the waves taking turns; lanes past the end reload
   // the last dword into the buffer's padding
   auto stage = [&](uint32_t c) {
@@ -1662,18 +1710,20 @@ __device__ __forceinline__ void frame_word_group(const FrameArgs &a, uint64_t w)
       __builtin_amdgcn_global_load_lds(a.stream + cb + j, (lds_void *)(sb + i), 4, 0, 0);
     }
     const uint32_t dw = 4 * nk;
+    const uint32_t *nsrc = reinterpret_cast<const uint32_t *>(a.planes + (w * a.nnoise + k0) * 2);
     for (uint32_t i = wv * 64; i < dw; i += kFrameThreads) {
       const uint32_t t = min(i + lane, dw - 1);
-      const uint32_t *src =
-          reinterpret_cast<const uint32_t *>(a.planes + (2ull * k0 + (t >> 1)) * a.W + w) +
-          (t & 1);
-      __builtin_amdgcn_global_load_lds(src, (lds_void *)(nb + i), 4, 0, 0);
+      __builtin_amdgcn_global_load_lds(nsrc + t, (lds_void *)(nb + i), 4, 0, 0);
     }
   };
   for (uint32_t i = tid; i < R; i += kFrameThreads) Rl[i] = Tr[i];
+  for (uint32_t i = tid; i < 2 * n; i += kFrameThreads) Rl[R + i] = 0;  // the frame E
   for (uint32_t i = tid; i < a.npreload; i += kFrameThreads)
     dyn[a.lds_rec + a.preload[2 * i + 1]] = a.recs[(2ull * a.preload[2 * i]) * a.W + w];
-  if (tid == 0) dyn[a.lds_const] = ~0ull;
+  if (tid == 0) {
+    dyn[a.lds_const] = ~0ull;  // the all-ones word
+    dyn[a.lds_const + 1] = 0;  // the zero word
+  }
   const uint64_t seed = a.seed, mseq0 = a.mseq0;
   if (a.nchunk) stage(0);
   for (uint32_t c = 0; c < a.nchunk; ++c) {
@@ -1688,8 +1738,31 @@ __device__ __forceinline__ void frame_word_group(const FrameArgs &a, uint64_t w)
     while (pos < cw) {
       const uint32_t kind = rfl(sbuf[pos]);
       const uint32_t h1 = rfl(sbuf[pos + 1]);
-      if (kind == kFrameXor) {
-        // R[row] ^= XOR of the LDS mask words its entries name
+      if (kind == kFrameLayer) {
+        // independent tasks, one per thread, each running its updates in
+        // order: 16-bit LDS word indices {dst, s0, s1, s2}, dyn[dst] ^=
+        // dyn[s0] ^ dyn[s1] ^ dyn[s2] (unused sources name the zero word);
+        // the next update is read before this one writes
+        const uint32_t T = h1, nbody = rfl(sbuf[pos + 2]);
+        const uint32_t *offs = sbuf + pos + 8;
+        const uint32_t bstart = (pos + 8 + T + 2) & ~1u;
+        const uint2 *body = reinterpret_cast<const uint2 *>(sbuf + bstart);
+        for (uint32_t t = tid; t < T; t += kFrameThreads) {
+          const uint32_t e1 = offs[t + 1];
+          uint32_t e = offs[t];
+          if (e >= e1) continue;
+          uint2 q = body[e];
+          for (; e < e1; ++e) {
+            const uint2 qn = body[min(e + 1, e1 - 1)];
+            const uint64_t v = dyn[q.x >> 16] ^ dyn[q.y & 0xFFFFu] ^ dyn[q.y >> 16];
+            dyn[q.x & 0xFFFFu] ^= v;
+            q = qn;
+          }
+        }
+        pos = bstart + nbody;
+        __syncthreads();
+      } else if (kind == kFrameXor) {
+        // Rv[row] ^= XOR of the LDS words its entries name
         const uint32_t T = h1, ne = rfl(sbuf[pos + 2]);
         const uint32_t *rows = sbuf + pos + 8;
         const uint32_t *offs = rows + T;
@@ -1709,17 +1782,14 @@ __device__ __forceinline__ void frame_word_group(const FrameArgs &a, uint64_t w)
         __syncthreads();
       } else if (kind == kFrameRandom) {
         // MeasureRandom (MeasurementFunctors.hpp:225-265) with pivot p: the
-        // rowsum rows first (all threads read R[n+p] before it changes),
-        // then R[p], R[n+p] and the record
+        // rowsum rows first (all threads read Rv[n+p] before it changes),
+        // then Rv[p], Rv[n+p] = coin ^ E.x[q] and the record
         const uint32_t p = h1, slot = rfl(sbuf[pos + 2]), mi = rfl(sbuf[pos + 3]);
         const uint32_t Tc = rfl(sbuf[pos + 4]), ridx = rfl(sbuf[pos + 5]);
-        const uint32_t ne = rfl(sbuf[pos + 6]);
+        const uint32_t ne = rfl(sbuf[pos + 6]), exq = rfl(sbuf[pos + 7]);
         const uint64_t Rs = Rl[n + p];
         const uint32_t *ents = sbuf + pos + 8;
-        for (uint32_t e = tid; e < ne; e += kFrameThreads) {
-          const uint32_t v = ents[e];
-          Rl[v & 0x7FFFFFFFu] ^= Rs ^ ((v >> 31) ? ~0ull : 0ull);
-        }
+        for (uint32_t e = tid; e < ne; e += kFrameThreads) Rl[ents[e]] ^= Rs;
         __syncthreads();
         if (wv == 0) {
           const uint64_t key = stream_key(seed, 2 * (mseq0 + mi));
@@ -1727,7 +1797,7 @@ __device__ __forceinline__ void frame_word_group(const FrameArgs &a, uint64_t w)
           const uint64_t coin = __ballot(u < Tc);
           if (lane == 0) {
             Rl[p] = Rs;
-            Rl[n + p] = coin;
+            Rl[n + p] = coin ^ dyn[exq];
             a.recs[(2ull * slot) * a.W + w] = coin;
             a.recs[(2ull * slot + 1) * a.W + w] = 0ull;
             if (ridx != kNone) dyn[a.lds_rec + ridx] = coin;
@@ -1738,7 +1808,8 @@ __device__ __forceinline__ void frame_word_group(const FrameArgs &a, uint64_t w)
       } else {
         // G determined measurements (MeasureDetermined, :189-201): lane j of
         // every wave XORs every kFrameWaves-th row of measurement j, wave 0
-        // combines; their rows exclude the scratch row, which takes the last
+        // combines and adds the constant and E.x[q]; their rows exclude the
+        // scratch row, which takes the Rv part of the one flagged last
         const uint32_t G = h1;
         const uint32_t *desc = sbuf + pos + 8;
         const uint32_t *rows = desc + 4 * G + 1;
@@ -1751,14 +1822,16 @@ __device__ __forceinline__ void frame_word_group(const FrameArgs &a, uint64_t w)
         }
         __syncthreads();
         if (wv == 0 && lane < G) {
-          uint64_t acc = desc[4 * lane + 2] ? ~0ull : 0ull;
+          uint64_t acc = 0;
 #pragma unroll
           for (uint32_t v = 0; v < kFrameWaves; ++v) acc ^= part[v * 64 + lane];
           const uint32_t slot = desc[4 * lane], ridx = desc[4 * lane + 1];
-          a.recs[(2ull * slot) * a.W + w] = acc;
+          const uint32_t xq = desc[4 * lane + 2];
+          const uint64_t res = acc ^ ((xq >> 31) ? ~0ull : 0ull) ^ dyn[xq & 0xFFFFu];
+          a.recs[(2ull * slot) * a.W + w] = res;
           a.recs[(2ull * slot + 1) * a.W + w] = ~0ull;
-          if (ridx != kNone) dyn[a.lds_rec + ridx] = acc;
-          if (lane == G - 1) Rl[2 * n] = acc;
+          if (ridx != kNone) dyn[a.lds_rec + ridx] = res;
+          if ((xq >> 30) & 1) Rl[2 * n] = acc;
         }
         pos += 8 + 4 * G + 1 + rfl(desc[4 * G]);
         __syncthreads();
@@ -1969,6 +2042,16 @@ hipError_t launch_frame_batch(hipStream_t s, const FrameArgs *circ, const uint64
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_frame_batch, dim3((uint32_t)nitems), dim3(kFrameThreads), lds_bytes, s,
                      circ, items);
+  return hipGetLastError();
+}
+
+hipError_t launch_frame_noise(hipStream_t s, uint64_t *planes, const NoiseDesc *desc,
+                              uint32_t nk, uint64_t W, const void *params) {
+  if (!nk || !W) return hipSuccess;
+  const uint64_t tasks = W * ((nk + 31) / 32);
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((tasks + 3) / 4, 1u << 16);
+  hipLaunchKernelGGL(k_frame_noise, dim3(blocks), dim3(256), 0, s, planes, desc, nk, W,
+                     (const ProgParams *)params);
   return hipGetLastError();
 }
 

@@ -450,7 +450,7 @@ extern "C" int pq_program_run(pq_state s, pq_program p, int form) {
       return pq_set_error(PQ_ERR_UNSUPPORTED,
                           "frame form runs from the |0...0> tableau: reset the state first");
     if (!p->frame) {
-      int rc0 = build_frame_plan(p);
+      int rc0 = build_frame_plan(p, s->W);
       if (rc0) return rc0;
     }
   }

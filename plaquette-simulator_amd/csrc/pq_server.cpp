@@ -152,7 +152,7 @@ int pq_server_s::admit(Req *r) {
       r->prog->frame = it->second;
       ++plan_hits;
       run_form = PQ_RUN_FRAME;
-    } else if ((rc = build_frame_plan(r->prog)) == PQ_OK) {
+    } else if ((rc = build_frame_plan(r->prog, UINT64_MAX)) == PQ_OK) {
       // bounded cache: plans in use stay alive through their programs
       if (plans.size() >= kMaxCachedPlans) plans.clear();
       plans.emplace(key, r->prog->frame);

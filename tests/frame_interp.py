@@ -3,7 +3,9 @@
 Runs the plan that ``frame_plan(n, ops)`` compiles (the stream the GPU
 kernel k_frame executes per 64-shot word group) on the CPU, word group by
 word group, with coins and noise drawn by the oracle's counter-based draw,
-so the host compiler can be checked against the oracle without a GPU.
+so the host compiler can be checked against the oracle without a GPU.  The
+LDS image is the kernel's: Rv [2n+1], the Pauli frame [2n], record words,
+the all-ones word, then the noise windows.
 """
 from __future__ import annotations
 
@@ -57,9 +59,9 @@ def run(plan, n, B, seed, ops, meas_seq0=0, noise_seq0=0):
         shots = [64 * w + i for i in range(64)]
         nbufs = plan.get("nbufs", 1)
         size = plan["lds_noise"] + nbufs * plan["nb_words"]
-        lds = [0] * size
+        lds = [0] * size                      # Rv, the frame E (zero), records, ...
         lds[2 * n] = M64                      # scratch row r = 1 (init)
-        lds[plan["lds_const"]] = M64
+        lds[plan["lds_const"]] = M64           # the all-ones word (the zero word follows)
         if plan["preload_n"]:
             raise NotImplementedError("preloaded records")
         for ci, (cb, cw, k0, nk) in enumerate(plan["chunks"]):
@@ -73,7 +75,29 @@ def run(plan, n, B, seed, ops, meas_seq0=0, noise_seq0=0):
             pos = 0
             while pos < cw:
                 kind, h1 = s[pos], s[pos + 1]
-                if kind == 1:
+                if kind == 4:
+                    T, nbody = h1, s[pos + 2]
+                    offs = s[pos + 8:pos + 8 + T + 1]
+                    bstart = (pos + 8 + T + 2) & ~1
+                    body = s[bstart:bstart + nbody]
+                    assert 2 * offs[T] == nbody
+                    written = set()
+                    for t in range(T):
+                        mine = set()
+                        for e in range(offs[t], offs[t + 1]):
+                            w0, w1 = body[2 * e], body[2 * e + 1]
+                            dst = w0 & 0xFFFF
+                            srcs = (w0 >> 16, w1 & 0xFFFF, w1 >> 16)
+                            assert 2 * n + 1 <= dst < 2 * n + 1 + 2 * n
+                            acc = 0
+                            for v in srcs:
+                                acc ^= lds[v]
+                            lds[dst] ^= acc
+                            mine.add(dst)
+                        assert not (mine & written), "tasks of one layer share a word"
+                        written |= mine
+                    pos = bstart + nbody
+                elif kind == 1:
                     T, ne = h1, s[pos + 2]
                     rows = s[pos + 8:pos + 8 + T]
                     offs = s[pos + 8 + T:pos + 8 + 2 * T + 1]
@@ -87,16 +111,16 @@ def run(plan, n, B, seed, ops, meas_seq0=0, noise_seq0=0):
                     pos += 8 + 2 * T + 1 + ne
                 elif kind == 2:
                     p, slot, mi, Tc, ridx, ne = h1, s[pos + 2], s[pos + 3], s[pos + 4], s[pos + 5], s[pos + 6]
+                    exq = s[pos + 7]
                     Rs = lds[n + p]
                     coin = 0
                     for i, b in enumerate(shots):
                         coin |= int((O.draw(seed, 2 * (meas_seq0 + mi), b) >> 40) < Tc) << i
-                    for v in s[pos + 8:pos + 8 + ne]:
-                        row = v & 0x7FFFFFFF
+                    for row in s[pos + 8:pos + 8 + ne]:
                         assert row not in (p, n + p) and row < 2 * n
-                        lds[row] ^= Rs ^ (M64 if v >> 31 else 0)
+                        lds[row] ^= Rs
                     lds[p] = Rs
-                    lds[n + p] = coin
+                    lds[n + p] = coin ^ lds[exq]
                     rec_res.setdefault(slot, [0] * W)[w] = coin
                     rec_det.setdefault(slot, [0] * W)[w] = 0
                     if ridx != 0xFFFFFFFF:
@@ -107,17 +131,18 @@ def run(plan, n, B, seed, ops, meas_seq0=0, noise_seq0=0):
                     desc = s[pos + 8:pos + 8 + 4 * G + 1]
                     rows = s[pos + 8 + 4 * G + 1:]
                     for j in range(G):
-                        slot, ridx, c, o0 = desc[4 * j:4 * j + 4]
+                        slot, ridx, xq, o0 = desc[4 * j:4 * j + 4]
                         o1 = desc[4 * (j + 1) + 3] if j + 1 < G else desc[4 * G]
-                        acc = M64 if c else 0
+                        acc = 0
                         for e in range(o0, o1):
                             assert rows[e] != 2 * n
                             acc ^= lds[rows[e]]
-                        rec_res.setdefault(slot, [0] * W)[w] = acc
+                        res = acc ^ (M64 if xq >> 31 else 0) ^ lds[xq & 0xFFFF]
+                        rec_res.setdefault(slot, [0] * W)[w] = res
                         rec_det.setdefault(slot, [0] * W)[w] = M64
                         if ridx != 0xFFFFFFFF:
-                            lds[plan["lds_rec"] + ridx] = acc
-                        if j == G - 1:
+                            lds[plan["lds_rec"] + ridx] = res
+                        if (xq >> 30) & 1:
                             lds[2 * n] = acc
                     pos += 8 + 4 * G + 1 + desc[4 * G]
                 else:
