@@ -1527,6 +1527,16 @@ __global__ __launch_bounds__(MAXT) void k_batch_t(
 // Noise planes of every circuit of a batch (blockIdx.y = circuit).  The
 // block row's waves are laid out as (noise-op lane) x (word-group lane): one
 // stream key per op and wave, one draw per lane per (op, word group).
+// Lane `lane` of v takes the wave-uniform value val (v_writelane_b32; the
+// lane select goes through M0, so the instruction reads one SGPR).
+__device__ __forceinline__ uint32_t write_lane(uint32_t v, uint32_t val, uint32_t lane) {
+  asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tv_writelane_b32 %0, %1, m0"
+               : "+v"(v)
+               : "s"(val), "s"(lane)
+               : "m0");
+  return v;
+}
+
 // Group-major noise planes [W][nk][2] (the frame form): wave task t covers
 // word group w = t / nkb and noise ops [32 kb, 32 kb + 32), kb = t % nkb;
 // lanes 2i and 2i+1 end up holding op 32 kb + i's X and Z mask words, so a
@@ -1546,7 +1556,7 @@ __device__ __forceinline__ void frame_noise_tasks(uint64_t *__restrict__ planes,
     const uint32_t cnt = min(32u, nk - k0);
     // lane i < cnt derives op k0 + i's stream key
     const uint64_t key_l = stream_key(seed, 2 * (nseq0 + k0 + min(lane, cnt - 1)) + 1);
-    uint64_t out = 0;
+    uint32_t out_lo = 0, out_hi = 0;
     for (uint32_t i = 0; i < cnt; ++i) {
       const NoiseDesc d = desc[k0 + i];
       const uint32_t klo = __builtin_amdgcn_readlane((uint32_t)key_l, i);
@@ -1556,10 +1566,13 @@ __device__ __forceinline__ void frame_noise_tasks(uint64_t *__restrict__ planes,
       noise_bits(d, draw_word_lane(key, w, lane_g), bx, bz);
       const uint64_t mx = __ballot(bx);
       const uint64_t mz = __ballot(bz);
-      if (lane == 2 * i) out = mx;
-      if (lane == 2 * i + 1) out = mz;
+      out_lo = write_lane(out_lo, (uint32_t)mx, 2 * i);
+      out_hi = write_lane(out_hi, (uint32_t)(mx >> 32), 2 * i);
+      out_lo = write_lane(out_lo, (uint32_t)mz, 2 * i + 1);
+      out_hi = write_lane(out_hi, (uint32_t)(mz >> 32), 2 * i + 1);
     }
-    if (lane < 2 * cnt) planes[(w * nk + k0) * 2 + lane] = out;
+    if (lane < 2 * cnt)
+      planes[(w * nk + k0) * 2 + lane] = ((uint64_t)out_hi << 32) | out_lo;
   }
 }
 
