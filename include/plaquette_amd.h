@@ -235,12 +235,14 @@ int pq_program_layout(pq_program p, uint64_t *segments, uint64_t *tiled_segments
  * it (host only; PQ_ERR_UNSUPPORTED if the ops hold flagged H/S/CX/CZ).
  * info[16] = {stream words, chunks,
  * preload pairs, words per X/Z bit row, flag records, LDS word offsets of
- * the records, the all-ones word and the noise window, noise-window words,
+ * the records, the all-ones word (the zero word follows it) and the noise
+ * window, noise-window words,
  * stream-buffer words, LDS bytes per word group, random measurements,
  * determined measurements, final X/Z is |0...0>, staging buffers, 0}; non-NULL buffers
  * receive the op stream, the chunk table ({begin, words, k0, nk} each), the
  * {slot, index} preload pairs and the final shared X/Z bit rows
- * ([2n+1][words per row] each).  No reference counterpart (tests, tools). */
+ * ([2n+1][words per row] each).  The staging sizes are those of a launch
+ * of many word groups.  No reference counterpart (tests, tools). */
 int pq_frame_plan(uint64_t n, const pq_op *ops, uint64_t nops, uint64_t info[16],
                   uint32_t *stream, uint32_t *chunks, uint32_t *preload, uint64_t *x,
                   uint64_t *z);
