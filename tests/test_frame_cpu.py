@@ -134,3 +134,59 @@ def test_gates_without_phase_flips(b, W):
     ob.s(2)
     plan = check(b, 3, 10, 1, ob.array())
     assert plan["stream_words"] == 0 and plan["chunks_n"] == 0
+
+
+def test_record_pool_reuse(b, W):
+    """More flag-read records live at once than the pool hands out fresh
+    (128), then reuse: every value must still reach its readers."""
+    rng = np.random.default_rng(21)
+    n = 12
+    ob = W.OpList()
+    for q in range(n):
+        ob.h(q)
+    for _ in range(3):
+        slots = [ob.measure(int(rng.integers(0, n)), float(rng.choice([0.5, 0.2]))) for _ in range(150)]
+        for s in rng.permutation(slots):
+            ob.flagged(int(rng.choice([W.OP_X_FLAG, W.OP_Y_FLAG, W.OP_Z_FLAG])),
+                       int(rng.integers(0, n)), int(s))
+            if rng.random() < 0.2:
+                ob.noise(int(rng.integers(0, n)), 0.1, 0.1, 0.1)
+    plan = check(b, n, 70, 21, ob.array())
+    assert plan["nref"] > 128
+
+
+def test_long_tasks_split(b, W):
+    """A long run of updates on one qubit pair outgrows one task (256 stream
+    words) and one task's noise span; the splits keep the order."""
+    ob = W.OpList()
+    ob.h(0)
+    for i in range(700):
+        ob.cx(0, 1)
+        ob.s(1)
+        ob.noise(0, 0.05, 0.05, 0.05)
+        if i % 50 == 0:
+            ob.noise(1, 0.2, 0.0, 0.1)
+    ob.measure(0)
+    ob.measure(1)
+    check(b, 2, 130, 5, ob.array())
+
+
+def test_determined_measurements_out_of_program_order(b, W):
+    """Determined measurements join an earlier batch when no random step
+    wrote their rows; the scratch row still ends with the program's last."""
+    rng = np.random.default_rng(4)
+    n = 10
+    ob = W.OpList()
+    for q in range(0, n, 2):
+        ob.h(q)
+        ob.cx(q, q + 1)
+    for _ in range(6):
+        for q in rng.permutation(n):
+            ob.measure(int(q), float(rng.choice([0.5, 0.1])))
+            if rng.random() < 0.3:
+                ob.noise(int(rng.integers(0, n)), 0.1, 0.05, 0.1)
+        for q in range(0, n, 2):
+            ob.h(q)
+            ob.cx(q, q + 1)
+    plan = check(b, n, 100, 4, ob.array())
+    assert plan["random"] > 0 and plan["determined"] > 0
