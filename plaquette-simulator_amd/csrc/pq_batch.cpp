@@ -44,6 +44,13 @@ struct pq_batch_s {
   uint64_t *d_items = nullptr;
   unsigned long long *d_counter = nullptr;
   uint64_t nitems = 0;
+  std::vector<uint64_t> h_items;   // the work list, host copy
+  // frame form: the work list split by LDS class (members whose plans let 4
+  // groups share a CU, then the rest), one launch each
+  uint64_t *d_fitems = nullptr;
+  uint64_t fitems_n[2] = {0, 0};
+  uint32_t flds[2] = {0, 0};
+  bool fsplit = false;
   bool built = false;
   uint32_t threads = kProgramThreadsMax;
   uint32_t blocks = 0;             // 0: CUs x blocks per CU
@@ -71,6 +78,7 @@ pq_batch_s::~pq_batch_s() {
   if (d_fargs) (void)hipFree(d_fargs);
   if (h_fargs) (void)hipHostFree(h_fargs);
   if (d_items) (void)hipFree(d_items);
+  if (d_fitems) (void)hipFree(d_fitems);
   if (d_counter) (void)hipFree(d_counter);
   for (hipEvent_t e : {ev_upload, ev_a, ev_b})
     if (e) (void)hipEventDestroy(e);
@@ -178,6 +186,10 @@ static int build(pq_batch b) {
   if (b->h_fargs) HIPCHK(hipHostFree(b->h_fargs));
   b->h_fargs = nullptr;
   b->nitems = list.size();
+  b->h_items = list;
+  if (b->d_fitems) HIPCHK(hipFree(b->d_fitems));
+  b->d_fitems = nullptr;
+  b->fsplit = false;
   if (!list.empty()) {
     HIPCHK(hipMalloc(&b->d_items, list.size() * 8));
     HIPCHK(hipMemcpy(b->d_items, list.data(), list.size() * 8, hipMemcpyHostToDevice));
@@ -224,8 +236,9 @@ static int enqueue(pq_batch b) {
 
 // The frame form of a batch: every member must allow it (no flagged
 // H/S/CX/CZ) and start from the |0...0> tableau; members with equal
-// structure share one plan.  One noise prologue for all members, one frame
-// kernel over the packed work list, then each member's shared X/Z.
+// structure share one plan.  One noise prologue for all members, frame
+// kernels over the packed work list (by LDS class), then each member's
+// shared X/Z.
 static int run_frame_batch(pq_batch b) {
   const uint32_t nc = (uint32_t)b->members.size();
   // everything that can refuse the run, before any member changes
@@ -248,7 +261,7 @@ static int run_frame_batch(pq_batch b) {
     }
   }
   HIPCHK(hipEventSynchronize(b->ev_upload));
-  uint32_t lds = 0, noise = 0;
+  uint32_t noise = 0;
   for (uint32_t c = 0; c < nc; ++c) {
     pq_state s = b->members[c].first;
     pq_program p = b->members[c].second;
@@ -268,11 +281,35 @@ static int run_frame_batch(pq_batch b) {
     bc.group_major = 1;
     b->h_fargs[c] = frame_args(s, p);
     if (!s->tab) b->h_fargs[c].W = 0;
-    lds = std::max(lds, p->frame->lds_bytes);
     if (p->nnoise) noise = b->noise_blocks;
     s->meas_seq += p->nmeas;
     s->noise_seq += p->nnoise;
     if (s->tab) s->xz_identity = p->frame->final_identity;
+  }
+  if (!b->fsplit) {
+    // small members: LDS for at least 4 groups per CU
+    std::vector<uint8_t> small(nc);
+    for (uint32_t c = 0; c < nc; ++c) {
+      const uint32_t l = b->members[c].second->frame->lds_bytes;
+      small[c] = l * 4ull <= 160ull * 1024;
+    }
+    std::vector<uint64_t> split;
+    split.reserve(b->h_items.size());
+    b->fitems_n[0] = b->fitems_n[1] = 0;
+    b->flds[0] = b->flds[1] = 0;
+    for (int k = 0; k < 2; ++k)
+      for (uint64_t it : b->h_items) {
+        const uint32_t c = (uint32_t)(it >> 40);
+        if (small[c] != k) continue;
+        split.push_back(it);
+        ++b->fitems_n[k];
+        b->flds[k] = std::max(b->flds[k], b->members[c].second->frame->lds_bytes);
+      }
+    if (!split.empty()) {
+      HIPCHK(hipMalloc(&b->d_fitems, split.size() * 8));
+      HIPCHK(hipMemcpy(b->d_fitems, split.data(), split.size() * 8, hipMemcpyHostToDevice));
+    }
+    b->fsplit = true;
   }
   HIPCHK(hipMemcpyAsync(b->d_circ, b->h_circ, nc * sizeof(BatchCircuit), hipMemcpyHostToDevice,
                         b->stream));
@@ -281,7 +318,12 @@ static int run_frame_batch(pq_batch b) {
   HIPCHK(hipEventRecord(b->ev_upload, b->stream));
   if (b->timed) HIPCHK(hipEventRecord(b->ev_a, b->stream));
   HIPCHK(launch_batch_noise(b->stream, b->d_circ, nc, noise));
-  HIPCHK(launch_frame_batch(b->stream, b->d_fargs, b->d_items, b->nitems, lds));
+  // two launches: the members whose plans fit 4 groups per CU would run at
+  // the largest member's occupancy in one launch sized for it
+  for (int k = 0, off = 0; k < 2; off += (int)b->fitems_n[k], ++k)
+    if (b->fitems_n[k])
+      HIPCHK(launch_frame_batch(b->stream, b->d_fargs, b->d_fitems + off, b->fitems_n[k],
+                                b->flds[k]));
   for (uint32_t c = 0; c < nc; ++c) {
     pq_state s = b->members[c].first;
     const FramePlan &F = *b->members[c].second->frame;
