@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -261,7 +262,6 @@ static int run_frame_batch(pq_batch b) {
     }
   }
   HIPCHK(hipEventSynchronize(b->ev_upload));
-  uint32_t noise = 0;
   for (uint32_t c = 0; c < nc; ++c) {
     pq_state s = b->members[c].first;
     pq_program p = b->members[c].second;
@@ -278,10 +278,8 @@ static int run_frame_batch(pq_batch b) {
     bc.nseq0 = s->noise_seq;
     bc.noise = p->d_noise;
     bc.nnoise = (uint32_t)p->nnoise;
-    bc.group_major = 1;
     b->h_fargs[c] = frame_args(s, p);
     if (!s->tab) b->h_fargs[c].W = 0;
-    if (p->nnoise) noise = b->noise_blocks;
     s->meas_seq += p->nmeas;
     s->noise_seq += p->nnoise;
     if (s->tab) s->xz_identity = p->frame->final_identity;
@@ -311,13 +309,20 @@ static int run_frame_batch(pq_batch b) {
     }
     b->fsplit = true;
   }
-  HIPCHK(hipMemcpyAsync(b->d_circ, b->h_circ, nc * sizeof(BatchCircuit), hipMemcpyHostToDevice,
-                        b->stream));
   HIPCHK(hipMemcpyAsync(b->d_fargs, b->h_fargs, nc * sizeof(FrameArgs), hipMemcpyHostToDevice,
+                        b->stream));
+  // the noise tasks (word group x 32 ops) of all members in one numbering
+  uint64_t tasks = 0;
+  for (uint32_t c = 0; c < nc; ++c) {
+    BatchCircuit &bc = b->h_circ[c];
+    bc.task0 = tasks;
+    tasks += bc.a.W * ((bc.nnoise + 31) / 32);
+  }
+  HIPCHK(hipMemcpyAsync(b->d_circ, b->h_circ, nc * sizeof(BatchCircuit), hipMemcpyHostToDevice,
                         b->stream));
   HIPCHK(hipEventRecord(b->ev_upload, b->stream));
   if (b->timed) HIPCHK(hipEventRecord(b->ev_a, b->stream));
-  HIPCHK(launch_batch_noise(b->stream, b->d_circ, nc, noise));
+  HIPCHK(launch_batch_frame_noise(b->stream, b->d_circ, nc, tasks));
   // two launches: the members whose plans fit 4 groups per CU would run at
   // the largest member's occupancy in one launch sized for it
   for (int k = 0, off = 0; k < 2; off += (int)b->fitems_n[k], ++k)
@@ -374,7 +379,7 @@ extern "C" int pq_batch_run(pq_batch b, int form) {
     bc.nseq0 = s->noise_seq;
     bc.noise = p->d_noise;
     bc.nnoise = (uint32_t)p->nnoise;
-    bc.group_major = 0;
+
     s->meas_seq += p->nmeas;
     s->xz_identity = false;
     s->noise_seq += p->nnoise;

@@ -1529,6 +1529,10 @@ __global__ __launch_bounds__(MAXT) void k_batch_t(
 // stream key per op and wave, one draw per lane per (op, word group).
 // Lane `lane` of v takes the wave-uniform value val (v_writelane_b32; the
 // lane select goes through M0, so the instruction reads one SGPR).
+// The kernels that call it keep nothing in M0 (no LDS DMA), so the clobber
+// only documents the write.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
 __device__ __forceinline__ uint32_t write_lane(uint32_t v, uint32_t val, uint32_t lane) {
   asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tv_writelane_b32 %0, %1, m0"
                : "+v"(v)
@@ -1536,53 +1540,82 @@ __device__ __forceinline__ uint32_t write_lane(uint32_t v, uint32_t val, uint32_
                : "m0");
   return v;
 }
+#pragma clang diagnostic pop
 
-// Group-major noise planes [W][nk][2] (the frame form): wave task t covers
-// word group w = t / nkb and noise ops [32 kb, 32 kb + 32), kb = t % nkb;
-// lanes 2i and 2i+1 end up holding op 32 kb + i's X and Z mask words, so a
-// task writes one contiguous run of up to 512 bytes.  Op k's draws use stream
-// 2 (nseq0 + k) + 1, as by plane.
-__device__ __forceinline__ void frame_noise_tasks(uint64_t *__restrict__ planes,
-                                                  const NoiseDesc *__restrict__ desc,
-                                                  uint32_t nk, uint64_t W, uint64_t seed,
-                                                  uint64_t nseq0, uint64_t wave,
-                                                  uint64_t nwaves) {
+// Group-major noise planes [W][nk][2] (the frame form).  One wave task
+// covers word group w and noise ops [k0, k0 + 32): lanes 2i and 2i+1 end up
+// holding op k0 + i's X and Z mask words, so a task writes one contiguous run
+// of up to 512 bytes.  Op k's draws use stream 2 (nseq0 + k) + 1, as by plane.
+__device__ __forceinline__ void frame_noise_task(uint64_t *__restrict__ planes,
+                                                 const NoiseDesc *__restrict__ desc,
+                                                 uint32_t nk, uint64_t w, uint32_t k0,
+                                                 uint64_t seed, uint64_t nseq0) {
   const uint32_t lane = lane_id();
   const uint64_t lane_g = lane_golden();
-  const uint64_t nkb = (nk + 31) / 32;
-  for (uint64_t t = wave; t < W * nkb; t += nwaves) {
-    const uint64_t w = t / nkb;
-    const uint32_t k0 = (uint32_t)(t % nkb) * 32;
-    const uint32_t cnt = min(32u, nk - k0);
-    // lane i < cnt derives op k0 + i's stream key
-    const uint64_t key_l = stream_key(seed, 2 * (nseq0 + k0 + min(lane, cnt - 1)) + 1);
-    uint32_t out_lo = 0, out_hi = 0;
-    for (uint32_t i = 0; i < cnt; ++i) {
-      const NoiseDesc d = desc[k0 + i];
-      const uint32_t klo = __builtin_amdgcn_readlane((uint32_t)key_l, i);
-      const uint32_t khi = __builtin_amdgcn_readlane((uint32_t)(key_l >> 32), i);
-      const uint64_t key = ((uint64_t)khi << 32) | klo;
-      bool bx, bz;
-      noise_bits(d, draw_word_lane(key, w, lane_g), bx, bz);
-      const uint64_t mx = __ballot(bx);
-      const uint64_t mz = __ballot(bz);
-      out_lo = write_lane(out_lo, (uint32_t)mx, 2 * i);
-      out_hi = write_lane(out_hi, (uint32_t)(mx >> 32), 2 * i);
-      out_lo = write_lane(out_lo, (uint32_t)mz, 2 * i + 1);
-      out_hi = write_lane(out_hi, (uint32_t)(mz >> 32), 2 * i + 1);
-    }
-    if (lane < 2 * cnt)
-      planes[(w * nk + k0) * 2 + lane] = ((uint64_t)out_hi << 32) | out_lo;
+  const uint32_t cnt = min(32u, nk - k0);
+  // lane i < cnt derives op k0 + i's stream key
+  const uint64_t key_l = stream_key(seed, 2 * (nseq0 + k0 + min(lane, cnt - 1)) + 1);
+  uint32_t out_lo = 0, out_hi = 0;
+  for (uint32_t i = 0; i < cnt; ++i) {
+    const NoiseDesc d = desc[k0 + i];
+    const uint32_t klo = __builtin_amdgcn_readlane((uint32_t)key_l, i);
+    const uint32_t khi = __builtin_amdgcn_readlane((uint32_t)(key_l >> 32), i);
+    const uint64_t key = ((uint64_t)khi << 32) | klo;
+    bool bx, bz;
+    noise_bits(d, draw_word_lane(key, w, lane_g), bx, bz);
+    const uint64_t mx = __ballot(bx);
+    const uint64_t mz = __ballot(bz);
+    out_lo = write_lane(out_lo, (uint32_t)mx, 2 * i);
+    out_hi = write_lane(out_hi, (uint32_t)(mx >> 32), 2 * i);
+    out_lo = write_lane(out_lo, (uint32_t)mz, 2 * i + 1);
+    out_hi = write_lane(out_hi, (uint32_t)(mz >> 32), 2 * i + 1);
   }
+  if (lane < 2 * cnt)
+    planes[(w * nk + k0) * 2 + lane] = ((uint64_t)out_hi << 32) | out_lo;
 }
 
+// Wave task t: word group t / nkb, ops from 32 (t % nkb), nkb = ceil(nk / 32).
 __global__ __launch_bounds__(256) void k_frame_noise(uint64_t *__restrict__ planes,
                                                      const NoiseDesc *__restrict__ desc,
                                                      uint32_t nk, uint64_t W,
                                                      const ProgParams *__restrict__ pp) {
   const uint64_t nwv = (uint64_t)gridDim.x * (blockDim.x / 64);
-  const uint64_t wave = blockIdx.x * (uint64_t)(blockDim.x / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  frame_noise_tasks(planes, desc, nk, W, pp->seed, pp->nseq0, wave, nwv);
+  const uint64_t wave =
+      blockIdx.x * (uint64_t)(blockDim.x / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t nkb = (nk + 31) / 32;
+  for (uint64_t t = wave; t < W * nkb; t += nwv)
+    frame_noise_task(planes, desc, nk, t / nkb, (uint32_t)(t % nkb) * 32, pp->seed, pp->nseq0);
+}
+
+// Frame batch: every circuit's tasks in one numbering (circuit c's from
+// circ[c].task0, circ[ncirc] standing for the end), each wave a contiguous
+// share; the wave finds its first circuit by binary search, then walks on.
+__global__ __launch_bounds__(256) void k_batch_frame_noise(const pq::BatchCircuit *__restrict__ circ,
+                                                           uint32_t ncirc, uint64_t tasks) {
+  const uint64_t nwv = (uint64_t)gridDim.x * (blockDim.x / 64);
+  const uint64_t wave =
+      blockIdx.x * (uint64_t)(blockDim.x / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t per = (tasks + nwv - 1) / nwv;
+  uint64_t t = wave * per;
+  const uint64_t t1 = min(t + per, tasks);
+  if (t >= t1) return;
+  uint32_t lo = 0, hi = ncirc;  // the last circuit whose first task is <= t
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (circ[mid].task0 <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  uint32_t c = lo;
+  for (; t < t1; ++t) {
+    while (c + 1 < ncirc && circ[c + 1].task0 <= t) ++c;
+    const pq::BatchCircuit &bc = circ[c];
+    const uint64_t nkb = (bc.nnoise + 31) / 32;
+    const uint64_t lt = t - bc.task0;
+    frame_noise_task(const_cast<uint64_t *>(bc.a.planes), bc.noise, bc.nnoise, lt / nkb,
+                     (uint32_t)(lt % nkb) * 32, bc.seed, bc.nseq0);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_batch_noise(const pq::BatchCircuit *__restrict__ circ) {
@@ -1590,12 +1623,7 @@ __global__ __launch_bounds__(256) void k_batch_noise(const pq::BatchCircuit *__r
   const uint64_t W = bc.a.W;
   const uint64_t nwv = (uint64_t)gridDim.x * (blockDim.x / 64);
   const uint64_t wave = blockIdx.x * (uint64_t)(blockDim.x / 64) + (threadIdx.x >> 6);
-  if (bc.group_major) {
-    frame_noise_tasks(const_cast<uint64_t *>(bc.a.planes), bc.noise, bc.nnoise, W, bc.seed,
-                      bc.nseq0, blockIdx.x * (uint64_t)(blockDim.x / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6),
-                      nwv);
-    return;
-  }
+
   // waves along word groups (about 16 groups each, so one stream key serves
   // many draws) and along noise ops
   const uint64_t ww = W / 16 < 1 ? 1 : (W / 16 < nwv ? W / 16 : nwv);
@@ -1880,6 +1908,7 @@ __global__ __launch_bounds__(256) void k_frame_xz(uint64_t *__restrict__ tab,
   const uint32_t q = c < n ? c : c - n;
   const uint64_t *M = c < n ? X : Z;
   const uint64_t v = ((M[(uint64_t)row * nw + q / 64] >> (q % 64)) & 1) ? ~0ull : 0ull;
+  // every word group of this (column, row range): 2 KB per block and group
   for (uint64_t w = blockIdx.z; w < W; w += gridDim.z) tab[(w * C + c) * R + row] = v;
 }
 
@@ -1906,7 +1935,9 @@ hipError_t launch_fill_r_identity(hipStream_t s, uint64_t *rcol, uint64_t W, uin
 hipError_t launch_fill_identity(hipStream_t s, uint64_t *tab, const Geom &g) {
   const uint64_t ncols = g.W * (uint64_t)g.C;
   if (!ncols || !g.R) return hipSuccess;
-  hipLaunchKernelGGL(k_fill_identity, dim3((uint32_t)std::min<uint64_t>(ncols, 1u << 20)),
+  // at most 16384 blocks, each filling columns in turn (many small blocks
+  // would be bound by their dispatch, not by HBM)
+  hipLaunchKernelGGL(k_fill_identity, dim3((uint32_t)std::min<uint64_t>(ncols, 16384)),
                      dim3(256), 0, s, tab, g.R, g.C, ncols);
   return hipGetLastError();
 }
@@ -2068,6 +2099,15 @@ hipError_t launch_frame_noise(hipStream_t s, uint64_t *planes, const NoiseDesc *
   return hipGetLastError();
 }
 
+hipError_t launch_batch_frame_noise(hipStream_t s, const BatchCircuit *circ, uint32_t ncirc,
+                                    uint64_t tasks) {
+  if (!ncirc || !tasks) return hipSuccess;
+  // about 8 tasks per wave, at most 65536 blocks
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((tasks + 31) / 32, 65536);
+  hipLaunchKernelGGL(k_batch_frame_noise, dim3(blocks), dim3(256), 0, s, circ, ncirc, tasks);
+  return hipGetLastError();
+}
+
 hipError_t launch_batch_noise(hipStream_t s, const BatchCircuit *circ, uint32_t ncirc,
                               uint32_t noise_blocks) {
   if (!ncirc || !noise_blocks) return hipSuccess;
@@ -2078,7 +2118,12 @@ hipError_t launch_batch_noise(hipStream_t s, const BatchCircuit *circ, uint32_t 
 hipError_t launch_frame_xz(hipStream_t s, uint64_t *tab, const Geom &g, const uint64_t *X,
                            const uint64_t *Z, uint32_t nw) {
   if (!g.W || !g.n) return hipSuccess;
-  const dim3 grid((g.R + 255) / 256, 2 * g.n, (uint32_t)std::min<uint64_t>(g.W, 64));
+  // about 65536 blocks, each writing its column slice in several word
+  // groups: one 2 KB block per (column slice, group) would be bound by
+  // dispatch, not by HBM, on a large state
+  const uint64_t blocks = (uint64_t)((g.R + 255) / 256) * 2 * g.n;
+  const uint64_t z = std::max<uint64_t>(1, std::min<uint64_t>(g.W, (65536 + blocks - 1) / blocks));
+  const dim3 grid((g.R + 255) / 256, 2 * g.n, (uint32_t)std::min<uint64_t>(z, 64));
   hipLaunchKernelGGL(k_frame_xz, grid, dim3(256), 0, s, tab, X, Z, g.n, g.R, g.C, g.W, nw);
   return hipGetLastError();
 }
