@@ -332,8 +332,10 @@ static int run_frame_batch(pq_batch b) {
   for (uint32_t c = 0; c < nc; ++c) {
     pq_state s = b->members[c].first;
     const FramePlan &F = *b->members[c].second->frame;
-    if (s->tab && !s->r_only)
+    if (s->tab && !s->r_only) {
       HIPCHK(launch_frame_xz(b->stream, s->tab, geom_of(s), F.d_X, F.d_Z, F.nw));
+      s->xz_pending = false;  // every X/Z word is written
+    }
   }
   if (b->timed) HIPCHK(hipEventRecord(b->ev_b, b->stream));
   return PQ_OK;
@@ -370,6 +372,7 @@ extern "C" int pq_batch_run(pq_batch b, int form) {
     pq_program p = b->members[c].second;
     int rc = prepare_records(s, p);
     if (rc) return rc;
+    if ((rc = materialize_tableau(s))) return rc;
     HIPCHK(hipStreamSynchronize(s->stream));  // the member's own queued work first
     BatchCircuit &bc = b->h_circ[c];
     bc.a = program_args(s, p, b->lds_budget);

@@ -103,13 +103,18 @@ pq_state_s::~pq_state_s() {
 }
 
 // Entry points that work on the full tableau refuse states that keep only
-// their phase column (sampling states: frame-form programs and records).
+// their phase column (sampling states: frame-form programs and records), and
+// fill X/Z a reset left pending.
 #define FULL_ONLY(st)                                                                   \
   do {                                                                                  \
     if ((st) && (st)->r_only)                                                           \
       return fail(PQ_ERR_UNSUPPORTED,                                                   \
                   "a sampling state keeps only its phase column: frame-form programs, " \
                   "records and reset only");                                            \
+    if (st) {                                                                           \
+      const int mrc_ = materialize_tableau(st);                                        \
+      if (mrc_) return mrc_;                                                            \
+    }                                                                                   \
   } while (0)
 
 static int check_device() {
@@ -230,12 +235,29 @@ extern "C" int pq_info(pq_state s, uint64_t *n, uint64_t *B, int64_t *seed,
   return PQ_OK;
 }
 
+int materialize_tableau(pq_state s) {
+  if (!s || !s->tab || !s->xz_pending) return PQ_OK;
+  HIPCHK(launch_fill_identity(s->stream, s->tab, geom_of(s)));
+  s->xz_pending = false;
+  return PQ_OK;
+}
+
+// |0...0> again (the reference's ResetCliffordState, with its intended
+// extent).  Only the R column is written now; X/Z are filled when something
+// needs them (materialize_tableau), so a frame-form run, which writes X/Z
+// itself, never pays for the fill.  A state whose raw pointer was handed out
+// is filled at once.
 extern "C" int pq_reset(pq_state s) {
   ARGCHK(s, "null state");
-  if (s->tab && s->r_only)
+  if (s->tab && s->r_only) {
     HIPCHK(launch_fill_r_identity(s->stream, s->tab, s->W, (uint32_t)s->R));
-  else if (s->tab)
+  } else if (s->tab && !s->exposed) {
+    HIPCHK(launch_fill_r_column(s->stream, s->tab, geom_of(s)));
+    s->xz_pending = true;
+  } else if (s->tab) {
     HIPCHK(launch_fill_identity(s->stream, s->tab, geom_of(s)));
+    s->xz_pending = false;
+  }
   s->xz_identity = true;
   return PQ_OK;
 }
@@ -676,6 +698,7 @@ extern "C" int pq_device_handles(pq_state s, void **tableau, void **records,
                                  uint64_t *words, uint64_t *rows, void **stream) {
   FULL_ONLY(s);
   ARGCHK(s, "null state");
+  if (tableau) s->exposed = true;
   if (tableau) *tableau = s->tab;
   if (records) *records = s->recs;
   if (words) *words = s->W;

@@ -33,6 +33,11 @@ struct pq_state_s {
   hipEvent_t ev[8] = {};
   // X/Z hold the |0...0> tableau in every shot (frame programs start there)
   bool xz_identity = false;
+  // reset left X/Z unwritten (the R column is reset): they hold |0...0>
+  // once materialize_tableau fills them; frame runs overwrite them instead
+  bool xz_pending = false;
+  // the raw tableau pointer was handed out: resets write X/Z at once
+  bool exposed = false;
   // only the R column is kept (tab is [W][R]): a server-internal state for
   // frame-form requests, whose X/Z nobody reads; only frame runs take it
   bool r_only = false;
@@ -59,6 +64,9 @@ inline void dev_free(void *p, hipStream_t async_stream) {
 int create_state_on(uint64_t n, uint64_t B, int64_t seed, hipStream_t stream, pq_state *out,
                     bool r_only = false);
 pq::Geom geom_of(const pq_state_s *s);
+// Fills a reset state's pending X/Z (the |0...0> tableau) before anything
+// reads or updates the full tableau.
+int materialize_tableau(pq_state s);
 int ensure_records(pq_state s, uint64_t count);
 uint64_t *record_ptr(pq_state s, uint64_t id);
 int plain_gate(pq_state s, uint32_t code, uint64_t q0, uint64_t q1);

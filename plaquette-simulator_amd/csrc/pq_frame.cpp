@@ -1126,8 +1126,10 @@ int run_frame(pq_state s, pq_program_s *p) {
     HIPCHK(hipEventRecord(p->ev[p->ev_used + 1], s->stream));
     p->ev_used += 2;
   }
-  if (!s->r_only)  // an R-column-only state keeps no X/Z
+  if (!s->r_only) {  // an R-column-only state keeps no X/Z
     HIPCHK(launch_frame_xz(s->stream, s->tab, geom_of(s), F.d_X, F.d_Z, F.nw));
+    s->xz_pending = false;  // every X/Z word is written
+  }
   s->xz_identity = F.final_identity;
   return PQ_OK;
 }

@@ -40,6 +40,15 @@ __global__ __launch_bounds__(256) void k_fill_r_identity(uint64_t *__restrict__ 
       rcol[w * R + row] = row == R - 1 ? ~0ull : 0ull;
 }
 
+// Column 2n (the R column) of every group of a full [W][C][R] tableau.
+__global__ __launch_bounds__(256) void k_fill_r_column(uint64_t *__restrict__ tab, uint32_t R,
+                                                       uint32_t C, uint64_t W) {
+  for (uint64_t w = blockIdx.x; w < W; w += gridDim.x) {
+    uint64_t *col = tab + (w * C + (C - 1)) * R;
+    for (uint32_t row = threadIdx.x; row < R; row += 256) col[row] = row == R - 1 ? ~0ull : 0ull;
+  }
+}
+
 // One block per column (w, c) at a time, its threads over the column's R
 // contiguous rows: one index split per column, coalesced stores.
 __global__ __launch_bounds__(256) void k_fill_identity(uint64_t *__restrict__ tab, uint32_t R,
@@ -1929,6 +1938,13 @@ hipError_t launch_fill_r_identity(hipStream_t s, uint64_t *rcol, uint64_t W, uin
   if (!W || !R) return hipSuccess;
   hipLaunchKernelGGL(k_fill_r_identity, dim3((uint32_t)std::min<uint64_t>(W, 1u << 20)), dim3(256),
                      0, s, rcol, R, W);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_r_column(hipStream_t s, uint64_t *tab, const Geom &g) {
+  if (!g.W || !g.R) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_r_column, dim3((uint32_t)std::min<uint64_t>(g.W, 1u << 20)), dim3(256),
+                     0, s, tab, g.R, g.C, g.W);
   return hipGetLastError();
 }
 

@@ -464,6 +464,7 @@ extern "C" int pq_program_run(pq_state s, pq_program p, int form) {
   if (form == PQ_RUN_FRAME && s->n > measure_max_qubits())
     return pq_set_error(PQ_ERR_UNSUPPORTED, "frame form supports at most 4095 qubits");
   if (form == PQ_RUN_FRAME) return run_frame(s, p);
+  if ((rc = materialize_tableau(s))) return rc;
   s->xz_identity = false;
   if (form == PQ_RUN_PLAIN) return run_plain(s, p);
   if (s->n > measure_max_qubits())

@@ -250,3 +250,55 @@ def test_sampling_state(ps, W):
         assert np.array_equal(ra, rb) and np.array_equal(da, db), f"batch record {i}"
     with pytest.raises(RuntimeError):
         batch.run("persistent")
+
+
+def test_lazy_reset_reads_and_runs(ps, W):
+    """A reset writes only the R column and leaves X/Z pending: whatever
+    reads or updates the full tableau next (a host copy, a checksum, a
+    gate, a persistent or plain run) sees |0...0>, and a frame run writes
+    X/Z itself."""
+    n, B, seed = 9, 130, 12
+    rng = np.random.default_rng(seed)
+    ob = W.OpList()
+    for _ in range(120):
+        u = rng.random()
+        q = int(rng.integers(0, n))
+        if u < 0.15:
+            ob.measure(q, 0.5)
+        elif u < 0.3:
+            ob.noise(q, 0.1, 0.05, 0.1)
+        elif u < 0.5:
+            c, t = rng.choice(n, 2, replace=False)
+            ob.cx(int(c), int(t))
+        else:
+            ob.gate(int(rng.choice([W.OP_H, W.OP_S, W.OP_X])), q)
+    ops = ob.array()
+    st = ps.CliffordState(n, B, seed, True)
+    orc = O.OracleState(n, B, seed)
+    ident = O.OracleState(n, B, seed)
+    say("lazy reset: persistent, reset, host copy")
+    st.run_program(ops, "persistent")
+    orc.run_program(ops)
+    st.reset_clifford_state()
+    orc.reset_clifford_state()
+    x, z, r = st.device_to_host_numpy()
+    assert np.array_equal(x, ident.x) and np.array_equal(z, ident.z) and np.array_equal(r, ident.r)
+    say("lazy reset: reset, checksum")
+    st.run_program(ops, "persistent")
+    orc.run_program(ops)
+    st.reset_clifford_state()
+    orc.reset_clifford_state()
+    assert st.check_sum(0, B) == orc.check_sum(0, B)
+    say("lazy reset: reset, frame run")
+    st.run_program(ops, "frame")
+    orc.run_program(ops)
+    assert_same(st, orc)
+    for form in ("persistent", "plain"):
+        say(f"lazy reset: reset, gate, {form} run")
+        st.reset_clifford_state()
+        orc.reset_clifford_state()
+        st.apply_hadamard_gate(3)
+        orc.apply_hadamard_gate(3)
+        st.run_program(ops, form)
+        orc.run_program(ops)
+        assert_same(st, orc)
