@@ -148,7 +148,10 @@ int pq_destroy(pq_state s);
 int pq_info(pq_state s, uint64_t *num_qubits, uint64_t *batch_size,
             int64_t *seed, uint64_t *num_records);
 /* ResetCliffordState (:309-316): re-initialise to |0...0> (the intended
- * semantics; the reference's {B,R,R} extent writes out of bounds). */
+ * semantics; the reference's {B,R,R} extent writes out of bounds).  The R
+ * column is written at once; X/Z are written by the next call that reads or
+ * updates the tableau, or by a frame-form run.  After pq_device_handles has
+ * handed out the tableau pointer, the whole tableau is written at once. */
 int pq_reset(pq_state s);
 
 /* ---- gates (plain form: one launch each) ---- */
@@ -208,7 +211,8 @@ int pq_checksum(pq_state s, int64_t b0, int64_t b1, int64_t *out);
 /* Raw bit-plane tableau, W*(2n+1)*R words in the device layout. */
 int pq_read_planes(pq_state s, uint64_t *host_words);
 /* Device handles (GetX/GetZ/GetR :180-198 return Kokkos Views; this gives
- * the bit-plane base, record arena base, W, R and the state's hipStream_t). */
+ * the bit-plane base, record arena base, W, R and the state's hipStream_t).
+ * Asking for the tableau pointer makes later resets write it in full. */
 int pq_device_handles(pq_state s, void **tableau, void **records,
                       uint64_t *words, uint64_t *rows, void **stream);
 
