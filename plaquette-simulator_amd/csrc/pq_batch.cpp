@@ -238,8 +238,8 @@ static int enqueue(pq_batch b) {
 // The frame form of a batch: every member must allow it (no flagged
 // H/S/CX/CZ) and start from the |0...0> tableau; members with equal
 // structure share one plan.  One noise prologue for all members, frame
-// kernels over the packed work list (by LDS class), then each member's
-// shared X/Z.
+// kernels over the packed work list (by LDS class); each member's shared X/Z
+// are written when something reads or updates its tableau.
 static int run_frame_batch(pq_batch b) {
   const uint32_t nc = (uint32_t)b->members.size();
   // everything that can refuse the run, before any member changes
@@ -331,10 +331,10 @@ static int run_frame_batch(pq_batch b) {
                                 b->flds[k]));
   for (uint32_t c = 0; c < nc; ++c) {
     pq_state s = b->members[c].first;
-    const FramePlan &F = *b->members[c].second->frame;
     if (s->tab && !s->r_only) {
-      HIPCHK(launch_frame_xz(b->stream, s->tab, geom_of(s), F.d_X, F.d_Z, F.nw));
-      s->xz_pending = false;  // every X/Z word is written
+      // X/Z are the plan's shared X/Z, written when needed (materialize_tableau)
+      s->xz_pending = false;
+      s->xz_plan = b->members[c].second->frame;
     }
   }
   if (b->timed) HIPCHK(hipEventRecord(b->ev_b, b->stream));

@@ -17,6 +17,7 @@
 
 #include "plaquette_amd.h"
 #include "pq_capi_internal.h"
+#include "pq_frame.h"
 #include "pq_common.h"
 #include "pq_kernels.h"
 
@@ -236,9 +237,15 @@ extern "C" int pq_info(pq_state s, uint64_t *n, uint64_t *B, int64_t *seed,
 }
 
 int materialize_tableau(pq_state s) {
-  if (!s || !s->tab || !s->xz_pending) return PQ_OK;
-  HIPCHK(launch_fill_identity(s->stream, s->tab, geom_of(s)));
-  s->xz_pending = false;
+  if (!s || !s->tab) return PQ_OK;
+  if (s->xz_pending) {
+    HIPCHK(launch_fill_identity(s->stream, s->tab, geom_of(s)));
+    s->xz_pending = false;
+  } else if (s->xz_plan) {
+    const FramePlan &F = *s->xz_plan;
+    HIPCHK(launch_frame_xz(s->stream, s->tab, geom_of(s), F.d_X, F.d_Z, F.nw));
+    s->xz_plan.reset();
+  }
   return PQ_OK;
 }
 
@@ -258,6 +265,7 @@ extern "C" int pq_reset(pq_state s) {
     HIPCHK(launch_fill_identity(s->stream, s->tab, geom_of(s)));
     s->xz_pending = false;
   }
+  s->xz_plan.reset();
   s->xz_identity = true;
   return PQ_OK;
 }

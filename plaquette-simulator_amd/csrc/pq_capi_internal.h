@@ -4,10 +4,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <memory>
 #include <vector>
 
 #include "plaquette_amd.h"
 #include "pq_kernels.h"
+
+struct FramePlan;
 
 struct pq_state_s {
   uint64_t n = 0, B = 0, R = 0, C = 0, W = 0;
@@ -38,6 +41,9 @@ struct pq_state_s {
   bool xz_pending = false;
   // the raw tableau pointer was handed out: resets write X/Z at once
   bool exposed = false;
+  // a frame run left X/Z unwritten: every group's X/Z is this plan's shared
+  // X/Z, written by materialize_tableau (never set with xz_pending)
+  std::shared_ptr<FramePlan> xz_plan;
   // only the R column is kept (tab is [W][R]): a server-internal state for
   // frame-form requests, whose X/Z nobody reads; only frame runs take it
   bool r_only = false;

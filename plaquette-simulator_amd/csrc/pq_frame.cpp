@@ -1127,8 +1127,10 @@ int run_frame(pq_state s, pq_program_s *p) {
     p->ev_used += 2;
   }
   if (!s->r_only) {  // an R-column-only state keeps no X/Z
-    HIPCHK(launch_frame_xz(s->stream, s->tab, geom_of(s), F.d_X, F.d_Z, F.nw));
-    s->xz_pending = false;  // every X/Z word is written
+    // every group's X/Z is the plan's shared X/Z: written when something
+    // reads or updates the tableau (materialize_tableau), not here
+    s->xz_pending = false;
+    s->xz_plan = p->frame;
   }
   s->xz_identity = F.final_identity;
   return PQ_OK;

@@ -255,8 +255,8 @@ def test_sampling_state(ps, W):
 def test_lazy_reset_reads_and_runs(ps, W):
     """A reset writes only the R column and leaves X/Z pending: whatever
     reads or updates the full tableau next (a host copy, a checksum, a
-    gate, a persistent or plain run) sees |0...0>, and a frame run writes
-    X/Z itself."""
+    gate, a persistent or plain run) sees |0...0>.  A frame run leaves the
+    plan's shared X/Z pending in turn, written on the same demand."""
     n, B, seed = 9, 130, 12
     rng = np.random.default_rng(seed)
     ob = W.OpList()
@@ -289,8 +289,16 @@ def test_lazy_reset_reads_and_runs(ps, W):
     st.reset_clifford_state()
     orc.reset_clifford_state()
     assert st.check_sum(0, B) == orc.check_sum(0, B)
-    say("lazy reset: reset, frame run")
+    say("lazy reset: reset, frame run, host copy")
     st.run_program(ops, "frame")
+    orc.run_program(ops)
+    assert_same(st, orc)
+    say("lazy reset: reset, frame run, persistent run (X/Z written on demand)")
+    st.reset_clifford_state()
+    orc.reset_clifford_state()
+    st.run_program(ops, "frame")
+    orc.run_program(ops)
+    st.run_program(ops, "persistent")
     orc.run_program(ops)
     assert_same(st, orc)
     for form in ("persistent", "plain"):
