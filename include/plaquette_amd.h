@@ -150,8 +150,9 @@ int pq_info(pq_state s, uint64_t *num_qubits, uint64_t *batch_size,
 /* ResetCliffordState (:309-316): re-initialise to |0...0> (the intended
  * semantics; the reference's {B,R,R} extent writes out of bounds).  The R
  * column is written at once; X/Z are written by the next call that reads or
- * updates the tableau, or by a frame-form run.  After pq_device_handles has
- * handed out the tableau pointer, the whole tableau is written at once. */
+ * updates the tableau (a frame-form run's own X/Z likewise).  After
+ * pq_device_handles has handed out the tableau pointer, resets write the
+ * whole tableau at once. */
 int pq_reset(pq_state s);
 
 /* ---- gates (plain form: one launch each) ---- */

@@ -333,8 +333,13 @@ static int run_frame_batch(pq_batch b) {
     pq_state s = b->members[c].first;
     if (s->tab && !s->r_only) {
       // X/Z are the plan's shared X/Z, written when needed (materialize_tableau)
+      // or now if the raw tableau pointer was handed out
       s->xz_pending = false;
       s->xz_plan = b->members[c].second->frame;
+      if (s->exposed) {
+        int rc = materialize_tableau(s);
+        if (rc) return rc;
+      }
     }
   }
   if (b->timed) HIPCHK(hipEventRecord(b->ev_b, b->stream));

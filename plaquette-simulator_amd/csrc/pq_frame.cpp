@@ -1128,9 +1128,14 @@ int run_frame(pq_state s, pq_program_s *p) {
   }
   if (!s->r_only) {  // an R-column-only state keeps no X/Z
     // every group's X/Z is the plan's shared X/Z: written when something
-    // reads or updates the tableau (materialize_tableau), not here
+    // reads or updates the tableau (materialize_tableau), or now if the raw
+    // tableau pointer was handed out
     s->xz_pending = false;
     s->xz_plan = p->frame;
+    if (s->exposed) {
+      int rc2 = materialize_tableau(s);
+      if (rc2) return rc2;
+    }
   }
   s->xz_identity = F.final_identity;
   return PQ_OK;

@@ -94,3 +94,44 @@ def test_program_and_parser(L):
             ok(L, L.pq_get_record(h, u64(i), res.ctypes.data_as(P), det.ctypes.data_as(P)))
             assert np.array_equal(res, oref) and np.array_equal(det, dref), (form, i)
         ok(L, L.pq_destroy(h))
+
+
+def test_raw_tableau_pointer_sees_resets_and_frame_runs(L):
+    """Once the raw tableau pointer is handed out, a reset and a frame-form
+    run write the whole tableau at once (no X/Z left for later): the memory
+    behind the pointer holds |0...0> after the reset and the run's tableau
+    after the run."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    text = b"H 0 1 2\nCX 0 3 1 4\nM 3 4\nCZ 2 3\nS 1\nM 0 1 2 3 4\n"
+    nops, nlines, used = u64(), u64(), u64()
+    ok(L, L.pq_circuit_parse(text, u64(len(text)), u32(0), None, u64(0), ctypes.byref(nops),
+                             ctypes.byref(nlines), ctypes.byref(used)))
+    ops = (Op * nops.value)()
+    ok(L, L.pq_circuit_parse(text, u64(len(text)), u32(0), ops, nops, ctypes.byref(nops),
+                             ctypes.byref(nlines), ctypes.byref(used)))
+    n, B, seed = 5, 200, 3
+    h = ctypes.c_void_p()
+    ok(L, L.pq_create(u64(n), u64(B), i64(seed), 1, ctypes.byref(h)))
+    tab, W, R = ctypes.c_void_p(), u64(), u64()
+    ok(L, L.pq_device_handles(h, ctypes.byref(tab), None, ctypes.byref(W), ctypes.byref(R), None))
+    words = W.value * R.value * R.value
+
+    def raw():
+        ok(L, L.pq_synchronize(h))
+        a = np.zeros(words, np.uint64)
+        assert hip.hipMemcpy(a.ctypes.data, tab.value, words * 8, 2) == 0
+        return a
+
+    ok(L, L.pq_run_program(h, ops, nops, 1))  # persistent: X/Z change
+    ok(L, L.pq_reset(h))
+    ident = np.zeros((W.value, R.value, R.value), np.uint64)
+    for c in range(R.value):
+        ident[:, c, c] = np.uint64(0xFFFFFFFFFFFFFFFF)
+    assert np.array_equal(raw(), ident.ravel()), "reset left the raw tableau stale"
+    ok(L, L.pq_run_program(h, ops, nops, 3))  # frame form
+    got = raw()
+    planes = np.zeros(words, np.uint64)
+    ok(L, L.pq_read_planes(h, planes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+    assert np.array_equal(got, planes), "frame run left the raw tableau's X/Z stale"
+    ok(L, L.pq_destroy(h))
