@@ -394,7 +394,10 @@ def bench_sampling(args):
                    "state": "phase column only"},
         "shots_per_s": B / (ms_step / 1e3),
         "meas_shots_per_s": meas * B / (ms_step / 1e3),
-        "kernel": "k_frame", "kernel_ms": kms / max(1, launches),
+        # HIP events around the run's kernels on the state's stream: with
+        # more word groups than the GPU holds at once they span the noise
+        # kernel and k_frame pipelined slice by slice
+        "kernel": "k_frame_noise + k_frame (pipelined slices)", "kernel_ms": kms / max(1, launches),
         "first_run_s": first,
         "device_bytes": {"phase_columns": words * R * 8,
                          "records": meas * 2 * words * 8,

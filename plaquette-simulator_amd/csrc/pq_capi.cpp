@@ -95,11 +95,14 @@ extern "C" uint64_t pq_draw(uint64_t seed, uint64_t stream, uint64_t idx) {
 // state
 // ---------------------------------------------------------------------------
 pq_state_s::~pq_state_s() {
+  if (aux) (void)hipStreamSynchronize(aux);
   if (stream && own_stream) (void)hipStreamSynchronize(stream);
   for (void *p : {(void *)tab, (void *)recs, (void *)scratch, params_dev, csum, (void *)stage})
     dev_free(p, alloc_stream);
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : aux_ev) (void)hipEventDestroy(e);
+  if (aux) (void)hipStreamDestroy(aux);
   if (stream && own_stream) (void)hipStreamDestroy(stream);
 }
 

@@ -44,6 +44,10 @@ struct pq_state_s {
   // a frame run left X/Z unwritten: every group's X/Z is this plan's shared
   // X/Z, written by materialize_tableau (never set with xz_pending)
   std::shared_ptr<FramePlan> xz_plan;
+  // frame runs of many word groups: the noise of the next slice of groups is
+  // drawn on this stream while k_frame runs the current one (created on use)
+  hipStream_t aux = nullptr;
+  std::vector<hipEvent_t> aux_ev;
   // only the R column is kept (tab is [W][R]): a server-internal state for
   // frame-form requests, whose X/Z nobody reads; only frame runs take it
   bool r_only = false;

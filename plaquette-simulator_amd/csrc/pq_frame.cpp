@@ -1077,6 +1077,8 @@ FrameArgs frame_args(pq_state s, pq_program_s *p) {
   a.R = (uint32_t)s->R;
   a.C = (uint32_t)s->C;
   a.W = s->W;
+  a.w0 = 0;
+  a.wn = s->W;
   a.nnoise = p->nnoise;
   a.lds_rec = F.lds_rec;
   a.lds_const = F.lds_const;
@@ -1108,8 +1110,23 @@ int run_frame(pq_state s, pq_program_s *p) {
   FrameArgs a = frame_args(s, p);
   s->meas_seq += p->nmeas;
   s->noise_seq += p->nnoise;
-  if (p->nnoise)
-    HIPCHK(launch_frame_noise(s->stream, p->d_planes, p->d_noise, (uint32_t)p->nnoise, s->W,
+  // A launch of more word groups than the GPU holds at once (CUs x groups
+  // per CU) runs in slices of about that many: the noise kernel, bound by
+  // its draws, fills slice i+1 on the auxiliary stream while k_frame, bound
+  // by LDS latency, runs slice i on the state's stream.
+  const uint64_t per_cu =
+      std::max<uint64_t>(1, std::min<uint64_t>(kFrameGroupsPerCu, kFrameLdsMax / F.lds_bytes));
+  const uint64_t held = kCus * per_cu;
+  const uint64_t slices =
+      (p->nnoise && s->own_stream && s->W >= 2 * held) ? std::min<uint64_t>(64, s->W / held) : 1;
+  if (slices > 1 && !s->aux) HIPCHK(hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking));
+  while (s->aux_ev.size() < slices + 1) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    s->aux_ev.push_back(e);
+  }
+  if (slices == 1 && p->nnoise)
+    HIPCHK(launch_frame_noise(s->stream, p->d_planes, p->d_noise, (uint32_t)p->nnoise, 0, s->W,
                               s->params_dev));
   if (p->timing_on) {
     if (p->ev_used + 2 > p->ev.size()) {
@@ -1121,7 +1138,25 @@ int run_frame(pq_state s, pq_program_s *p) {
     }
     HIPCHK(hipEventRecord(p->ev[p->ev_used], s->stream));
   }
-  HIPCHK(launch_frame(s->stream, a));
+  if (slices == 1) {
+    HIPCHK(launch_frame(s->stream, a));
+  } else {
+    // the auxiliary stream starts after everything queued on the state's
+    // stream (the parameters, the reset, the last run's k_frame reading
+    // the planes); each slice's k_frame waits for that slice's noise
+    HIPCHK(hipEventRecord(s->aux_ev[0], s->stream));
+    HIPCHK(hipStreamWaitEvent(s->aux, s->aux_ev[0], 0));
+    for (uint64_t i = 0; i < slices; ++i) {
+      const uint64_t w0 = i * s->W / slices, w1 = (i + 1) * s->W / slices;
+      HIPCHK(launch_frame_noise(s->aux, p->d_planes, p->d_noise, (uint32_t)p->nnoise, w0,
+                                w1 - w0, s->params_dev));
+      HIPCHK(hipEventRecord(s->aux_ev[i + 1], s->aux));
+      HIPCHK(hipStreamWaitEvent(s->stream, s->aux_ev[i + 1], 0));
+      a.w0 = w0;
+      a.wn = w1 - w0;
+      HIPCHK(launch_frame(s->stream, a));
+    }
+  }
   if (p->timing_on) {
     HIPCHK(hipEventRecord(p->ev[p->ev_used + 1], s->stream));
     p->ev_used += 2;

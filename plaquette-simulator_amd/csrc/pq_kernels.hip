@@ -1584,16 +1584,18 @@ __device__ __forceinline__ void frame_noise_task(uint64_t *__restrict__ planes,
 }
 
 // Wave task t: word group t / nkb, ops from 32 (t % nkb), nkb = ceil(nk / 32).
+// Word groups [w0, w0 + wn) of the planes.
 __global__ __launch_bounds__(256) void k_frame_noise(uint64_t *__restrict__ planes,
                                                      const NoiseDesc *__restrict__ desc,
-                                                     uint32_t nk, uint64_t W,
+                                                     uint32_t nk, uint64_t w0, uint64_t wn,
                                                      const ProgParams *__restrict__ pp) {
   const uint64_t nwv = (uint64_t)gridDim.x * (blockDim.x / 64);
   const uint64_t wave =
       blockIdx.x * (uint64_t)(blockDim.x / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint64_t nkb = (nk + 31) / 32;
-  for (uint64_t t = wave; t < W * nkb; t += nwv)
-    frame_noise_task(planes, desc, nk, t / nkb, (uint32_t)(t % nkb) * 32, pp->seed, pp->nseq0);
+  for (uint64_t t = wave; t < wn * nkb; t += nwv)
+    frame_noise_task(planes, desc, nk, w0 + t / nkb, (uint32_t)(t % nkb) * 32, pp->seed,
+                     pp->nseq0);
 }
 
 // Frame batch: every circuit's tasks in one numbering (circuit c's from
@@ -1894,7 +1896,7 @@ __device__ __forceinline__ void frame_word_group(const FrameArgs &a, uint64_t w)
 }
 
 __global__ __launch_bounds__(kFrameThreads) void k_frame(FrameArgs a) {
-  frame_word_group(a, blockIdx.x);
+  frame_word_group(a, a.w0 + blockIdx.x);
 }
 
 // Frame form of a mixed batch: block b runs work item items[b] = (circuit
@@ -2088,10 +2090,10 @@ hipError_t launch_program_noise(hipStream_t s, uint64_t *planes,
 }
 
 hipError_t launch_frame(hipStream_t s, const FrameArgs &a) {
-  if (!a.W) return hipSuccess;
+  if (!a.wn) return hipSuccess;
   hipError_t e = ensure_lds_attr((const void *)k_frame, a.lds_bytes);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_frame, dim3((uint32_t)a.W), dim3(kFrameThreads), a.lds_bytes, s, a);
+  hipLaunchKernelGGL(k_frame, dim3((uint32_t)a.wn), dim3(kFrameThreads), a.lds_bytes, s, a);
   return hipGetLastError();
 }
 
@@ -2106,11 +2108,11 @@ hipError_t launch_frame_batch(hipStream_t s, const FrameArgs *circ, const uint64
 }
 
 hipError_t launch_frame_noise(hipStream_t s, uint64_t *planes, const NoiseDesc *desc,
-                              uint32_t nk, uint64_t W, const void *params) {
-  if (!nk || !W) return hipSuccess;
-  const uint64_t tasks = W * ((nk + 31) / 32);
+                              uint32_t nk, uint64_t w0, uint64_t wn, const void *params) {
+  if (!nk || !wn) return hipSuccess;
+  const uint64_t tasks = wn * ((nk + 31) / 32);
   const uint32_t blocks = (uint32_t)std::min<uint64_t>((tasks + 3) / 4, 1u << 16);
-  hipLaunchKernelGGL(k_frame_noise, dim3(blocks), dim3(256), 0, s, planes, desc, nk, W,
+  hipLaunchKernelGGL(k_frame_noise, dim3(blocks), dim3(256), 0, s, planes, desc, nk, w0, wn,
                      (const ProgParams *)params);
   return hipGetLastError();
 }

@@ -310,3 +310,27 @@ def test_lazy_reset_reads_and_runs(ps, W):
         st.run_program(ops, form)
         orc.run_program(ops)
         assert_same(st, orc)
+
+
+def test_sliced_frame_run_equals_persistent(ps, W):
+    """More word groups than the GPU holds at once: the frame run goes in
+    slices, each slice's noise drawn on a second stream while the previous
+    slice runs.  d=5 (4 groups per CU, 1024 at once) with 2^17 shots is 2048
+    groups, so two slices; two runs with a reset between also check that the
+    second run's noise waits for the first run's kernels."""
+    ops, n = W.surface_code_memory_z(5, 3, 0.02)
+    B, seed = 1 << 17, 8
+    say(f"sliced frame run: n={n} B={B}")
+    full = ps.CliffordState(n, B, seed, True)
+    prog_f = ps.Program(full, ops)
+    samp = ps.SamplingState(n, B, seed)
+    prog_s = ps.Program(samp, ops)
+    for it in range(2):
+        full.reset_clifford_state()
+        samp.reset_clifford_state()
+        prog_f.run(full, "persistent")
+        prog_s.run(samp, "frame")
+        for i in range(full.num_records):
+            ra, da = full.get_record(i)
+            rb, db = samp.get_record(i)
+            assert np.array_equal(ra, rb) and np.array_equal(da, db), f"run {it} record {i}"
