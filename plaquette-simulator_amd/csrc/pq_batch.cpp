@@ -49,6 +49,7 @@ struct pq_batch_s {
   // frame form: the work list split by LDS class (members whose plans let 4
   // groups share a CU, then the rest), one launch each
   uint64_t *d_fitems = nullptr;
+  std::vector<hipEvent_t> member_ev;  // frame form: each member's queued work
   uint64_t fitems_n[2] = {0, 0};
   uint32_t flds[2] = {0, 0};
   bool fsplit = false;
@@ -80,6 +81,7 @@ pq_batch_s::~pq_batch_s() {
   if (h_fargs) (void)hipHostFree(h_fargs);
   if (d_items) (void)hipFree(d_items);
   if (d_fitems) (void)hipFree(d_fitems);
+  for (hipEvent_t e : member_ev) (void)hipEventDestroy(e);
   if (d_counter) (void)hipFree(d_counter);
   for (hipEvent_t e : {ev_upload, ev_a, ev_b})
     if (e) (void)hipEventDestroy(e);
@@ -267,7 +269,15 @@ static int run_frame_batch(pq_batch b) {
     pq_program p = b->members[c].second;
     int rc = prepare_records(s, p);
     if (rc) return rc;
-    HIPCHK(hipStreamSynchronize(s->stream));
+    // the member's own queued work (its reset, a new record arena) first,
+    // ordered on the device rather than waited for here
+    while (b->member_ev.size() <= c) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      b->member_ev.push_back(e);
+    }
+    HIPCHK(hipEventRecord(b->member_ev[c], s->stream));
+    HIPCHK(hipStreamWaitEvent(b->stream, b->member_ev[c], 0));
     if ((rc = upload_frame_plan(p, b->stream))) return rc;
     BatchCircuit &bc = b->h_circ[c];
     bc = BatchCircuit{};

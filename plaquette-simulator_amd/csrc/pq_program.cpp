@@ -342,21 +342,33 @@ extern "C" int pq_program_set_launch(pq_program p, uint32_t threads, uint32_t bl
 // Record slots: M ops write slot <= current count (== count appends), flagged
 // ops read slot < current count, checked in program order.
 int prepare_records(pq_state s, pq_program p) {
-  uint64_t cur = s->nrec;
-  for (uint64_t i = 0; i < p->ops.size(); ++i) {
-    const pq_op &o = p->ops[i];
-    if (o.code == PQ_OP_M) {
-      if (o.slot > cur)
-        return pq_set_error(PQ_ERR_RANGE, ("op " + std::to_string(i) + ": record id out of range").c_str());
-      if (o.slot == cur) ++cur;
-    } else if (is_flag(o.code) && !is_random_flag(o) && o.slot >= cur) {
-      return pq_set_error(PQ_ERR_RANGE, ("op " + std::to_string(i) + ": flag record out of range").c_str());
+  // the check depends only on the ops and the state's record count, so a
+  // program rerun on a state with the same count reuses it
+  if (!p->prep_valid || p->prep_start != s->nrec) {
+    uint64_t cur = s->nrec;
+    for (uint64_t i = 0; i < p->ops.size(); ++i) {
+      const pq_op &o = p->ops[i];
+      if (o.code == PQ_OP_M) {
+        if (o.slot > cur)
+          return pq_set_error(PQ_ERR_RANGE, ("op " + std::to_string(i) + ": record id out of range").c_str());
+        if (o.slot == cur) ++cur;
+      } else if (is_flag(o.code) && !is_random_flag(o) && o.slot >= cur) {
+        return pq_set_error(PQ_ERR_RANGE, ("op " + std::to_string(i) + ": flag record out of range").c_str());
+      }
     }
+    if (!p->prep_valid) {
+      for (const pq_op &o : p->ops)
+        if (o.code == PQ_OP_M) p->m_slots.push_back(o.slot);
+      std::sort(p->m_slots.begin(), p->m_slots.end());
+      p->m_slots.erase(std::unique(p->m_slots.begin(), p->m_slots.end()), p->m_slots.end());
+    }
+    p->prep_valid = true;
+    p->prep_start = s->nrec;
+    p->prep_end = cur;
   }
-  int rc = ensure_records(s, cur);
+  int rc = ensure_records(s, p->prep_end);
   if (rc) return rc;
-  for (const pq_op &o : p->ops)
-    if (o.code == PQ_OP_M) s->rec_valid[o.slot] = 0;
+  for (uint32_t slot : p->m_slots) s->rec_valid[slot] = 0;
   return PQ_OK;
 }
 

@@ -41,6 +41,11 @@ struct pq_program_s {
   std::vector<hipEvent_t> ev;  // start/end pairs of timed kernel launches
   size_t ev_used = 0;
   std::shared_ptr<FramePlan> frame;  // frame form plan, built on first use (may be shared)
+  // prepare_records' check of the record ids, for a state holding
+  // prep_start records (prep_end after the run), and the measured slots
+  bool prep_valid = false;
+  uint64_t prep_start = 0, prep_end = 0;
+  std::vector<uint32_t> m_slots;
   void drop_graph();
   ~pq_program_s();
 };
