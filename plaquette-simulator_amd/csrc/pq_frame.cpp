@@ -548,10 +548,11 @@ int simulate(pq_program_s *p, FramePlan &F, std::vector<Step> &steps,
   };
   std::vector<uint64_t> flip(rw, 0);  // rows whose cst the gates since the last fold flip
   std::vector<uint8_t> cst(R, 0);
-  auto fold_flip = [&]() {
-    for (uint32_t i = 0; i < R; ++i)
-      if ((flip[i / 64] >> (i % 64)) & 1) cst[i] ^= 1;
-    std::fill(flip.begin(), flip.end(), 0);
+  auto fold_flip = [&]() {  // word by word: most words have no flips
+    for (uint32_t j = 0; j < rw; ++j) {
+      for (uint64_t v = flip[j]; v; v &= v - 1) cst[j * 64 + (uint32_t)__builtin_ctzll(v)] ^= 1;
+      flip[j] = 0;
+    }
   };
   std::vector<uint64_t> tmp(rw), tmp2(rw);
   bool seg_rows = false;  // the current gate segment runs row by row
