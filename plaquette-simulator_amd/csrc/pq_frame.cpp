@@ -1189,7 +1189,7 @@ extern "C" int pq_frame_plan(uint64_t n, const pq_op *ops, uint64_t nops, uint64
   prog.ops.assign(ops, ops + nops);
   int rc = validate_ops(n, ops, nops);
   if (rc) return rc;
-  rc = compile_program(&prog);
+  rc = compile_frame_only(&prog);
   if (rc) return rc;
   rc = build_frame_plan(&prog, UINT64_MAX);
   if (rc) return rc;

@@ -231,6 +231,21 @@ int compile_program(pq_program_s *p) {
   return PQ_OK;
 }
 
+int compile_frame_only(pq_program_s *p) {
+  for (const pq_op &o : p->ops) {
+    if (o.code == PQ_OP_M) {
+      ++p->nmeas;
+    } else if (o.code == PQ_OP_NOISE) {
+      p->noise.push_back(NoiseDesc{0, o.t0, o.t1, o.t2, kNoisePauli});
+    } else {
+      if (is_random_flag(o)) p->noise.push_back(NoiseDesc{0, 0, o.t0, 0, kNoiseFlag});
+      if (o.code >= PQ_OP_H && o.code <= PQ_OP_CZ) ++p->ngates;
+    }
+  }
+  p->nnoise = p->noise.size();
+  return PQ_OK;
+}
+
 void pq_program_s::drop_graph() {
   if (gexec) (void)hipGraphExecDestroy(gexec);
   if (graph) (void)hipGraphDestroy(graph);
@@ -269,7 +284,8 @@ static int create_program(pq_state s, const pq_op *ops, uint64_t nops, hipStream
   p->W = s->W;
   p->R = s->R;
   p->ops.assign(ops, ops + nops);
-  rc = compile_program(p.get());
+  // a state that keeps only its R column runs the frame form alone
+  rc = s->r_only ? compile_frame_only(p.get()) : compile_program(p.get());
   if (rc) return rc;
   if ((rc = upload(&p->d_segs, p->segs, as))) return rc;
   if ((rc = upload(&p->d_instrs, p->instrs, as))) return rc;

@@ -51,6 +51,10 @@ struct pq_program_s {
 };
 
 int compile_program(pq_program_s *p);
+// What the frame form alone needs: the noise descriptors (noise ops and
+// random flags, numbered in program order as compile_program numbers them)
+// and the op counts, without the persistent form's segments.
+int compile_frame_only(pq_program_s *p);
 int validate_ops(uint64_t n, const pq_op *ops, uint64_t nops);
 
 constexpr uint64_t kProgramLdsBytes = 158 * 1024;  // dynamic-LDS cap (kProgramLdsBudget there)

@@ -17,6 +17,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -47,6 +48,8 @@ struct Req {
   // frame form on a state that keeps only its R column: the server returns
   // records, never the tableau, so X/Z need not exist per shot
   bool lite = false;
+  std::string plan_key;    // lite requests: frame_plan_key of the ops
+  bool own_build = false;  // this request's submit queued its plan's build
   double t_submit = 0, t_start = 0, t_done = 0;
   int status = 0;  // 0 queued, 1 running, 2 done, < 0 failed (PQ_ERR_*)
   int slot = -1;
@@ -79,9 +82,23 @@ struct pq_server_s {
   std::mutex plans_mu;
   std::unordered_map<std::string, std::shared_ptr<FramePlan>> plans;
   uint64_t plan_hits = 0, plan_builds = 0, frame_runs = 0;
+  // plans are built ahead on the builder thread, from submit on, so a
+  // request's plan is usually ready when a slot frees up; `building` holds
+  // the keys queued or in progress (plans_cv signals each finish)
+  struct BuildJob {
+    std::string key;
+    uint64_t n = 0;
+    std::vector<pq_op> ops;
+  };
+  std::deque<BuildJob> build_q;
+  std::set<std::string> building;
+  std::condition_variable plans_cv;
+  bool builder_stop = false;
+  std::thread builder;
   std::atomic<bool> stop{false};
   std::thread worker;
   void loop();
+  void build_loop();
   Req *next_fitting(double now);
   int admit(Req *r);
   void finish(Req *r);
@@ -145,12 +162,15 @@ int pq_server_s::admit(Req *r) {
   if (rc == PQ_OK && (r->lite || form == PQ_RUN_FRAME)) {
     // the frame form where the request allows it (no flagged H/S/CX/CZ),
     // with plans shared between requests of the same structure
-    const std::string key = frame_plan_key(r->n, r->ops.data(), r->ops.size());
-    std::lock_guard<std::mutex> g(plans_mu);
+    const std::string key =
+        r->plan_key.empty() ? frame_plan_key(r->n, r->ops.data(), r->ops.size()) : r->plan_key;
+    std::unique_lock<std::mutex> g(plans_mu);
+    // a build of this structure under way on the builder thread: wait for it
+    plans_cv.wait(g, [&] { return building.count(key) == 0; });
     auto it = plans.find(key);
     if (it != plans.end()) {
       r->prog->frame = it->second;
-      ++plan_hits;
+      if (!r->own_build) ++plan_hits;  // the request that queued the build is its first use
       run_form = PQ_RUN_FRAME;
     } else if ((rc = build_frame_plan(r->prog, UINT64_MAX)) == PQ_OK) {
       // bounded cache: plans in use stay alive through their programs
@@ -181,6 +201,34 @@ void pq_server_s::finish(Req *r) {
   if (r->st) pq_destroy(r->st);
   r->prog = nullptr;
   r->st = nullptr;
+}
+
+// Builds queued plans ahead of their requests' admission (host only: the
+// plan is compiled from the ops, as pq_frame_plan does).  A build that fails
+// leaves no plan: admission then builds again and reports or falls back.
+void pq_server_s::build_loop() {
+  std::unique_lock<std::mutex> g(plans_mu);
+  while (true) {
+    plans_cv.wait(g, [&] { return builder_stop || !build_q.empty(); });
+    if (builder_stop) return;
+    BuildJob job = std::move(build_q.front());
+    build_q.pop_front();
+    g.unlock();
+    pq_program_s prog;
+    prog.n = job.n;
+    prog.R = 2 * job.n + 1;
+    prog.ops = std::move(job.ops);
+    const bool ok =
+        compile_frame_only(&prog) == PQ_OK && build_frame_plan(&prog, UINT64_MAX) == PQ_OK;
+    g.lock();
+    if (ok) {
+      if (plans.size() >= kMaxCachedPlans) plans.clear();
+      plans.emplace(job.key, prog.frame);
+      ++plan_builds;
+    }
+    building.erase(job.key);
+    plans_cv.notify_all();
+  }
 }
 
 void pq_server_s::loop() {
@@ -267,12 +315,14 @@ extern "C" int pq_server_create(uint64_t budget_bytes, uint32_t slots, pq_server
     return pq_set_error(PQ_ERR_HIP, "kernel preparation failed");
   pq_server_s *raw = sv.get();
   raw->worker = std::thread([raw] { raw->loop(); });
+  raw->builder = std::thread([raw] { raw->build_loop(); });
   *out = sv.release();
   return PQ_OK;
 }
 
 extern "C" int pq_server_submit(pq_server sv, uint64_t n, uint64_t B, int64_t seed,
                                 const pq_op *ops, uint64_t nops, uint64_t *ticket) {
+  const double t_arrive = now_s();  // latency counts this call's own work too
   if (!sv || !ticket || (!ops && nops)) return pq_set_error(PQ_ERR_ARG, "null argument");
   if (n == 0 || B == 0) return pq_set_error(PQ_ERR_ARG, "request needs qubits and shots");
   if (n > pq::measure_max_qubits())
@@ -301,9 +351,20 @@ extern "C" int pq_server_submit(pq_server sv, uint64_t n, uint64_t B, int64_t se
     return pq_set_error(PQ_ERR_OOM, "request can never fit the server's memory budget");
   if (hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
     return pq_set_error(PQ_ERR_HIP, "event creation failed");
+  if (r->lite) {
+    // the plan of a structure not seen yet is built ahead, on the builder
+    r->plan_key = frame_plan_key(n, ops, nops);
+    std::lock_guard<std::mutex> g(sv->plans_mu);
+    if (!sv->plans.count(r->plan_key) && !sv->building.count(r->plan_key)) {
+      sv->building.insert(r->plan_key);
+      sv->build_q.push_back(pq_server_s::BuildJob{r->plan_key, n, r->ops});
+      r->own_build = true;
+      sv->plans_cv.notify_all();
+    }
+  }
   std::lock_guard<std::mutex> g(sv->mu);
   r->ticket = sv->next_ticket++;
-  r->t_submit = now_s();
+  r->t_submit = t_arrive;
   *ticket = r->ticket;
   sv->pending.push_back(r.get());
   sv->reqs[r->ticket] = std::move(r);
@@ -410,6 +471,12 @@ extern "C" int pq_server_destroy(pq_server sv) {
   sv->stop.store(true);
   sv->cv.notify_all();
   if (sv->worker.joinable()) sv->worker.join();
+  {
+    std::lock_guard<std::mutex> g(sv->plans_mu);
+    sv->builder_stop = true;
+  }
+  sv->plans_cv.notify_all();
+  if (sv->builder.joinable()) sv->builder.join();
   for (auto &kv : sv->reqs)
     if (kv.second->ev) (void)hipEventDestroy(kv.second->ev);
   for (hipStream_t s : sv->streams) {
