@@ -1020,6 +1020,21 @@ int build_frame_plan(pq_program_s *p, uint64_t groups) {
   }
   pk.add_xor(fin);
   pk.close_chunk();
+  // where a two-part run splits: the first chunk whose predecessors' noise
+  // reaches half the ops (they then read only ops below split_k)
+  {
+    const uint32_t nchunk = (uint32_t)(F->chunks.size() / 4);
+    uint32_t kmax = 0;
+    for (uint32_t c = 1; c < nchunk; ++c) {
+      const uint32_t nk = F->chunks[4 * (c - 1) + 3];
+      if (nk) kmax = std::max(kmax, F->chunks[4 * (c - 1) + 2] + nk);
+      if (kmax >= p->nnoise / 2 && kmax < p->nnoise) {
+        F->split_chunk = c;
+        F->split_k = kmax;
+        break;
+      }
+    }
+  }
   p->frame = std::move(F);
   return PQ_OK;
 }
@@ -1080,6 +1095,10 @@ FrameArgs frame_args(pq_state s, pq_program_s *p) {
   a.W = s->W;
   a.w0 = 0;
   a.wn = s->W;
+  a.c0 = 0;
+  a.c1 = (uint32_t)(F.chunks.size() / 4);
+  a.suspend = a.resume = 0;
+  a.side = nullptr;
   a.nnoise = p->nnoise;
   a.lds_rec = F.lds_rec;
   a.lds_const = F.lds_const;
@@ -1126,9 +1145,18 @@ int run_frame(pq_state s, pq_program_s *p) {
     HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     s->aux_ev.push_back(e);
   }
+  // A launch of at most that many groups runs the plan in two parts when
+  // it splits: the noise of the second part's ops is drawn on the auxiliary
+  // stream while k_frame runs the first part, and each group's frame and
+  // record words wait in p->d_side between the two k_frame launches.
+  const bool two_part = slices == 1 && p->nnoise && s->own_stream && F.split_chunk > 0;
+  if (two_part && !s->aux) HIPCHK(hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking));
+  if (two_part && !p->d_side)
+    HIPCHK(hipMalloc((void **)&p->d_side,
+                     std::max<uint64_t>(8, s->W * (uint64_t)(F.lds_const - F.R) * 8)));
   if (slices == 1 && p->nnoise)
     HIPCHK(launch_frame_noise(s->stream, p->d_planes, p->d_noise, (uint32_t)p->nnoise, 0, s->W,
-                              s->params_dev));
+                              s->params_dev, 0, two_part ? F.split_k : ~0u));
   if (p->timing_on) {
     if (p->ev_used + 2 > p->ev.size()) {
       hipEvent_t e0, e1;
@@ -1139,7 +1167,25 @@ int run_frame(pq_state s, pq_program_s *p) {
     }
     HIPCHK(hipEventRecord(p->ev[p->ev_used], s->stream));
   }
-  if (slices == 1) {
+  if (slices == 1 && two_part) {
+    // the second half of the noise starts once the first is drawn
+    HIPCHK(hipEventRecord(s->aux_ev[0], s->stream));
+    HIPCHK(hipStreamWaitEvent(s->aux, s->aux_ev[0], 0));
+    HIPCHK(launch_frame_noise(s->aux, p->d_planes, p->d_noise, (uint32_t)p->nnoise, 0, s->W,
+                              s->params_dev, F.split_k, ~0u));
+    HIPCHK(hipEventRecord(s->aux_ev[1], s->aux));
+    const uint32_t nchunk = a.c1;
+    a.side = p->d_side;
+    a.c1 = F.split_chunk;
+    a.suspend = 1;
+    HIPCHK(launch_frame(s->stream, a));
+    HIPCHK(hipStreamWaitEvent(s->stream, s->aux_ev[1], 0));
+    a.c0 = F.split_chunk;
+    a.c1 = nchunk;
+    a.suspend = 0;
+    a.resume = 1;
+    HIPCHK(launch_frame(s->stream, a));
+  } else if (slices == 1) {
     HIPCHK(launch_frame(s->stream, a));
   } else {
     // the auxiliary stream starts after everything queued on the state's

@@ -29,6 +29,9 @@ struct FramePlan {
   std::vector<uint32_t> preload;     // {slot, rec_l index} loaded at start
   std::vector<uint64_t> X, Z;        // final shared X/Z, [R][nw]
   bool final_identity = false;
+  // a two-part run: chunks [0, split_chunk) read only noise ops below
+  // split_k (0 when the plan does not split)
+  uint32_t split_chunk = 0, split_k = 0;
   uint64_t nrandom = 0, ndet = 0;
   uint32_t *d_stream = nullptr, *d_chunks = nullptr, *d_preload = nullptr;
   uint64_t *d_X = nullptr, *d_Z = nullptr;

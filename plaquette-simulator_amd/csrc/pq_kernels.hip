@@ -1558,10 +1558,11 @@ __device__ __forceinline__ uint32_t write_lane(uint32_t v, uint32_t val, uint32_
 __device__ __forceinline__ void frame_noise_task(uint64_t *__restrict__ planes,
                                                  const NoiseDesc *__restrict__ desc,
                                                  uint32_t nk, uint64_t w, uint32_t k0,
-                                                 uint64_t seed, uint64_t nseq0) {
+                                                 uint64_t seed, uint64_t nseq0,
+                                                 uint32_t kend) {
   const uint32_t lane = lane_id();
   const uint64_t lane_g = lane_golden();
-  const uint32_t cnt = min(32u, nk - k0);
+  const uint32_t cnt = min(32u, kend - k0);  // ops k0 .. min(k0 + 32, kend) - 1
   // lane i < cnt derives op k0 + i's stream key
   const uint64_t key_l = stream_key(seed, 2 * (nseq0 + k0 + min(lane, cnt - 1)) + 1);
   uint32_t out_lo = 0, out_hi = 0;
@@ -1584,18 +1585,19 @@ __device__ __forceinline__ void frame_noise_task(uint64_t *__restrict__ planes,
 }
 
 // Wave task t: word group t / nkb, ops from 32 (t % nkb), nkb = ceil(nk / 32).
-// Word groups [w0, w0 + wn) of the planes.
+// Word groups [w0, w0 + wn) and noise ops [ka, kb) of the planes.
 __global__ __launch_bounds__(256) void k_frame_noise(uint64_t *__restrict__ planes,
                                                      const NoiseDesc *__restrict__ desc,
                                                      uint32_t nk, uint64_t w0, uint64_t wn,
+                                                     uint32_t ka, uint32_t kb,
                                                      const ProgParams *__restrict__ pp) {
   const uint64_t nwv = (uint64_t)gridDim.x * (blockDim.x / 64);
   const uint64_t wave =
       blockIdx.x * (uint64_t)(blockDim.x / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint64_t nkb = (nk + 31) / 32;
+  const uint64_t nkb = (kb - ka + 31) / 32;
   for (uint64_t t = wave; t < wn * nkb; t += nwv)
-    frame_noise_task(planes, desc, nk, w0 + t / nkb, (uint32_t)(t % nkb) * 32, pp->seed,
-                     pp->nseq0);
+    frame_noise_task(planes, desc, nk, w0 + t / nkb, ka + (uint32_t)(t % nkb) * 32, pp->seed,
+                     pp->nseq0, kb);
 }
 
 // Frame batch: every circuit's tasks in one numbering (circuit c's from
@@ -1625,7 +1627,7 @@ __global__ __launch_bounds__(256) void k_batch_frame_noise(const pq::BatchCircui
     const uint64_t nkb = (bc.nnoise + 31) / 32;
     const uint64_t lt = t - bc.task0;
     frame_noise_task(const_cast<uint64_t *>(bc.a.planes), bc.noise, bc.nnoise, lt / nkb,
-                     (uint32_t)(lt % nkb) * 32, bc.seed, bc.nseq0);
+                     (uint32_t)(lt % nkb) * 32, bc.seed, bc.nseq0, bc.nnoise);
   }
 }
 
@@ -1768,23 +1770,28 @@ __device__ __forceinline__ void frame_word_group(const FrameArgs &a, uint64_t w)
       __builtin_amdgcn_global_load_lds(nsrc + t, (lds_void *)(nb + i), 4, 0, 0);
     }
   };
+  const uint32_t nside = a.lds_const - R;  // the frame and the record words
   for (uint32_t i = tid; i < R; i += kFrameThreads) Rl[i] = Tr[i];
-  for (uint32_t i = tid; i < 2 * n; i += kFrameThreads) Rl[R + i] = 0;  // the frame E
-  for (uint32_t i = tid; i < a.npreload; i += kFrameThreads)
-    dyn[a.lds_rec + a.preload[2 * i + 1]] = a.recs[(2ull * a.preload[2 * i]) * a.W + w];
+  if (a.resume) {
+    for (uint32_t i = tid; i < nside; i += kFrameThreads) Rl[R + i] = a.side[w * nside + i];
+  } else {
+    for (uint32_t i = tid; i < 2 * n; i += kFrameThreads) Rl[R + i] = 0;  // the frame E
+    for (uint32_t i = tid; i < a.npreload; i += kFrameThreads)
+      dyn[a.lds_rec + a.preload[2 * i + 1]] = a.recs[(2ull * a.preload[2 * i]) * a.W + w];
+  }
   if (tid == 0) {
     dyn[a.lds_const] = ~0ull;  // the all-ones word
     dyn[a.lds_const + 1] = 0;  // the zero word
   }
   const uint64_t seed = a.seed, mseq0 = a.mseq0;
-  if (a.nchunk) stage(0);
-  for (uint32_t c = 0; c < a.nchunk; ++c) {
+  if (a.c0 < a.c1) stage(a.c0);
+  for (uint32_t c = a.c0; c < a.c1; ++c) {
     const uint32_t cw = a.chunks[4 * c + 1];
     // chunk c has landed, and every thread is done with the buffer that
     // chunk c+1 goes to (chunk c-1's, or chunk c's own with one buffer)
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
-    if (a.nbufs > 1 && c + 1 < a.nchunk) stage(c + 1);
+    if (a.nbufs > 1 && c + 1 < a.c1) stage(c + 1);
     const uint32_t *sbuf = sbuf0 + (c % a.nbufs) * a.sb_words;
     uint32_t pos = 0;
     while (pos < cw) {
@@ -1889,10 +1896,12 @@ __device__ __forceinline__ void frame_word_group(const FrameArgs &a, uint64_t w)
         __syncthreads();
       }
     }
-    if (a.nbufs == 1 && c + 1 < a.nchunk) stage(c + 1);  // after the ops' last barrier
+    if (a.nbufs == 1 && c + 1 < a.c1) stage(c + 1);  // after the ops' last barrier
   }
   __syncthreads();
   for (uint32_t i = tid; i < R; i += kFrameThreads) Tr[i] = Rl[i];
+  if (a.suspend)
+    for (uint32_t i = tid; i < nside; i += kFrameThreads) a.side[w * nside + i] = Rl[R + i];
 }
 
 __global__ __launch_bounds__(kFrameThreads) void k_frame(FrameArgs a) {
@@ -2108,12 +2117,14 @@ hipError_t launch_frame_batch(hipStream_t s, const FrameArgs *circ, const uint64
 }
 
 hipError_t launch_frame_noise(hipStream_t s, uint64_t *planes, const NoiseDesc *desc,
-                              uint32_t nk, uint64_t w0, uint64_t wn, const void *params) {
-  if (!nk || !wn) return hipSuccess;
-  const uint64_t tasks = wn * ((nk + 31) / 32);
+                              uint32_t nk, uint64_t w0, uint64_t wn, const void *params,
+                              uint32_t ka, uint32_t kb) {
+  kb = std::min(kb, nk);
+  if (ka >= kb || !wn) return hipSuccess;
+  const uint64_t tasks = wn * ((kb - ka + 31) / 32);
   const uint32_t blocks = (uint32_t)std::min<uint64_t>((tasks + 3) / 4, 1u << 16);
-  hipLaunchKernelGGL(k_frame_noise, dim3(blocks), dim3(256), 0, s, planes, desc, nk, w0, wn,
-                     (const ProgParams *)params);
+  hipLaunchKernelGGL(k_frame_noise, dim3(blocks), dim3(256), 0, s, planes, desc, nk, w0, wn, ka,
+                     kb, (const ProgParams *)params);
   return hipGetLastError();
 }
 

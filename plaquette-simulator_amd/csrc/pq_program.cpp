@@ -257,7 +257,7 @@ pq_program_s::~pq_program_s() {
   drop_graph();
   for (hipEvent_t e : ev) (void)hipEventDestroy(e);
   for (void *q : {(void *)d_census, (void *)d_segs, (void *)d_segq, (void *)d_instrs,
-                  (void *)d_noise, (void *)d_planes})
+                  (void *)d_noise, (void *)d_planes, (void *)d_side})
     dev_free(q, alloc_stream);
 }
 

@@ -29,6 +29,7 @@ struct pq_program_s {
   pq::DInstr *d_instrs = nullptr;
   pq::NoiseDesc *d_noise = nullptr;
   uint64_t *d_planes = nullptr;  // noise planes [nnoise][2][W]
+  uint64_t *d_side = nullptr;    // frame form, two-part runs: [W][2n + nref] words
   uint32_t threads = 0, blocks = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
