@@ -243,7 +243,10 @@ int pq_program_layout(pq_program p, uint64_t *segments, uint64_t *tiled_segments
  * the records, the all-ones word (the zero word follows it) and the noise
  * window, noise-window words,
  * stream-buffer words, LDS bytes per word group, random measurements,
- * determined measurements, final X/Z is |0...0>, staging buffers, 0}; non-NULL buffers
+ * determined measurements, final X/Z is |0...0>, staging buffers, the chunk
+ * where a run of one round of word groups splits in two (0: no split; the
+ * chunks before it read only noise ops below the largest k0 + nk among
+ * them)}; non-NULL buffers
  * receive the op stream, the chunk table ({begin, words, k0, nk} each), the
  * {slot, index} preload pairs and the final shared X/Z bit rows
  * ([2n+1][words per row] each).  The staging sizes are those of a launch

@@ -520,10 +520,11 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
                             preload.mutable_data(), x.mutable_data(), z.mutable_data()),
               "frame_plan");
         py::dict d;
-        const char *names[15] = {"stream_words", "chunks_n", "preload_n", "nw", "nref",
+        const char *names[16] = {"stream_words", "chunks_n", "preload_n", "nw", "nref",
                                  "lds_rec", "lds_const", "lds_noise", "nb_words", "sb_words",
-                                 "lds_bytes", "random", "determined", "final_identity", "nbufs"};
-        for (int i = 0; i < 15; ++i) d[names[i]] = info[i];
+                                 "lds_bytes", "random", "determined", "final_identity", "nbufs",
+                                 "split_chunk"};
+        for (int i = 0; i < 16; ++i) d[names[i]] = info[i];
         d["stream"] = stream;
         d["chunks"] = chunks;
         d["preload"] = preload;

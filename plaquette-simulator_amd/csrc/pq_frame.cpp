@@ -1243,7 +1243,7 @@ extern "C" int pq_frame_plan(uint64_t n, const pq_op *ops, uint64_t nops, uint64
   const uint64_t v[16] = {F.stream.size(), F.chunks.size() / 4, F.preload.size() / 2, F.nw,
                           F.nref, F.lds_rec, F.lds_const, F.lds_noise, F.nb_words, F.sb_words,
                           F.lds_bytes, F.nrandom, F.ndet, F.final_identity ? 1u : 0u, F.nbufs,
-                          0};
+                          F.split_chunk};
   for (int i = 0; i < 16; ++i) info[i] = v[i];
   if (stream) std::copy(F.stream.begin(), F.stream.end(), stream);
   if (chunks) std::copy(F.chunks.begin(), F.chunks.end(), chunks);

@@ -190,3 +190,32 @@ def test_determined_measurements_out_of_program_order(b, W):
             ob.cx(q, q + 1)
     plan = check(b, n, 100, 4, ob.array())
     assert plan["random"] > 0 and plan["determined"] > 0
+
+
+@pytest.mark.parametrize("d,rounds,p", [(5, 5, 0.01), (7, 7, 0.001), (9, 9, 0.001)])
+def test_two_part_split_point(b, W, d, rounds, p):
+    """A run of one round of word groups splits at ``split_chunk``: the chunks
+    before it read only noise ops below the largest ``k0 + nk`` among them,
+    which covers at least half, and not all, of the ops the plan reads."""
+    ops, n = W.surface_code_memory_z(d, rounds, p)
+    plan = b.frame_plan(n, ops)
+    c = plan["split_chunk"]
+    assert 0 < c < plan["chunks_n"]
+    ch = plan["chunks"].astype(np.int64)
+    ends = np.where(ch[:, 3] > 0, ch[:, 2] + ch[:, 3], 0)
+    split_k, total = ends[:c].max(), ends.max()
+    assert 2 * split_k >= total - 1 and split_k < total
+    # the first chunk from which the split point is reached
+    assert 2 * ends[:c - 1].max(initial=0) < total - 1 or c == 1
+
+
+def test_no_split_without_noise_or_second_chunk(b, W):
+    ops, n = W.surface_code_memory_z(3, 3, 0.05)
+    assert b.frame_plan(n, ops)["chunks_n"] == 1
+    assert b.frame_plan(n, ops)["split_chunk"] == 0
+    ob = W.OpList()
+    for q in range(6):
+        ob.h(q)
+    for q in range(6):
+        ob.measure(q)
+    assert b.frame_plan(6, ob.array())["split_chunk"] == 0

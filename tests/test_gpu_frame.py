@@ -86,8 +86,12 @@ def test_long_program_many_chunks(ps, W):
 
 def test_d9_frame_equals_persistent(ps, W):
     """The d=9, 16384-shot surface code: frame and persistent forms give the
-    same planes and records, run after run."""
+    same planes and records, run after run.  Its 256 word groups fit one
+    round of the GPU, so each frame run goes in two parts (the frame and
+    record words cross a side buffer between the two launches)."""
+    import plaquette_simulator_bindings as b
     wl = W.config_surface_d9()
+    assert b.frame_plan(wl.num_qubits, wl.ops)["split_chunk"] > 0
     a = ps.CliffordState(wl.num_qubits, wl.batch_size, wl.seed, True)
     bst = ps.CliffordState(wl.num_qubits, wl.batch_size, wl.seed, True)
     pa = ps.Program(a, wl.ops)
