@@ -224,7 +224,11 @@ int pq_device_handles(pq_state s, void **tableau, void **records,
  * can be run many times; each run continues the state's coin and noise
  * sequence numbers exactly as the same ops issued one by one would.
  * M ops write record `slot` <= current record count (== count appends a new
- * record); flagged ops read records that exist at that point. */
+ * record); flagged ops read records that exist at that point.  A program's
+ * device buffers (its noise planes, the frame form's side buffer) serve all
+ * of its runs: runs of one program on two states of the same shape must not
+ * overlap (pq_synchronize the first state before running it on the other);
+ * runs on one state are ordered by that state's stream. */
 typedef struct pq_program_s *pq_program;
 int pq_program_create(pq_state s, const pq_op *ops, uint64_t nops,
                       pq_program *out);
