@@ -1,3 +1,4 @@
+#include <cstdlib>
 // Frame form of a program: circuits whose X/Z tableau is the same in every
 // shot run with X/Z simulated once, here on the host, and only per-shot words
 // on the GPU.
@@ -1022,13 +1023,19 @@ int build_frame_plan(pq_program_s *p, uint64_t groups) {
   pk.close_chunk();
   // where a two-part run splits: the first chunk whose predecessors' noise
   // reaches half the ops (they then read only ops below split_k)
+  // (PQ_FRAME_SPLIT_PCT: that share of the ops in percent, default 50)
   {
+    static const uint64_t pct = [] {
+      const char *e = getenv("PQ_FRAME_SPLIT_PCT");
+      const long v = e ? atol(e) : 50;
+      return (uint64_t)(v >= 1 && v <= 99 ? v : 50);
+    }();
     const uint32_t nchunk = (uint32_t)(F->chunks.size() / 4);
     uint32_t kmax = 0;
     for (uint32_t c = 1; c < nchunk; ++c) {
       const uint32_t nk = F->chunks[4 * (c - 1) + 3];
       if (nk) kmax = std::max(kmax, F->chunks[4 * (c - 1) + 2] + nk);
-      if (kmax >= p->nnoise / 2 && kmax < p->nnoise) {
+      if (100 * (uint64_t)kmax >= pct * p->nnoise && kmax < p->nnoise) {
         F->split_chunk = c;
         F->split_k = kmax;
         break;
