@@ -61,34 +61,133 @@ def algorithmic_bytes(ops, n, W, census):
     return out
 
 
-def cpu_baseline(w, target_s):
-    """The CPU oracle (reference int32 layout, OpenMP) on the first S shots of
-    the same circuit; S is scaled from a 64-shot calibration run so the timed
-    run takes about `target_s` seconds.  Shots are independent (coins and
-    noise keyed by shot index), so S shots are an exact subset of the job."""
-    import numpy as np
+# columns a gate reads and writes (x of qubit a is column a, z is n + a)
+def _gate_columns(code, a, b, n):
+    x, z = (lambda q: q), (lambda q: n + q)
+    base = {8: 1, 9: 2, 11: 3, 12: 4, 13: 5, 15: 6, 16: 7}.get(code, code)
+    if base == 1:
+        return {x(a), z(a)}, {x(a), z(a)}
+    if base == 2:
+        return {x(a), z(a)}, {z(a)}
+    if base == 3:
+        return {z(a)}, set()
+    if base in (4, 10):
+        return {x(a), z(a)}, set()
+    if base == 5:
+        return {x(a)}, set()
+    if base == 6:
+        return {x(a), z(a), x(b), z(b)}, {x(b), z(a)}
+    if base == 7:
+        return {x(a), z(a), x(b), z(b)}, {z(a), z(b)}
+    return set(), set()
+
+
+def fused_minimum_bytes(ops, n, W, census):
+    """Least HBM bytes one run of `ops` can move in the fused persistent form.
+
+    k_program fuses each gate segment -- a maximal run of non-measurement
+    ops -- into one pass over the tableau, so the model charges each segment
+    one read of every column its ops read and one write of every column they
+    change (R = 2n+1 words per column per 64-shot word group), one read and
+    one write of the r column, and the per-group mask words it needs (16 B
+    per noise op, 8 B per flag).  Measurements are charged their algorithmic
+    bytes from the engine's census, as in `algorithmic_bytes`.  Noise planes
+    are written by a separate prologue kernel and are not part of k_program."""
+    R = 2 * n + 1
+    col = R * W * 8
+    out = {"segment_columns": 0, "masks": 0, "measurements": 0, "segments": 0}
+    i, N = 0, len(ops)
+    while i < N:
+        if ops[i, 0] == 14:
+            i += 1
+            continue
+        rd, wr, masks = set(), set(), 0
+        while i < N and ops[i, 0] != 14:
+            c, a, b = int(ops[i, 0]), int(ops[i, 1]), int(ops[i, 2])
+            r_, w_ = _gate_columns(c, a, b, n)
+            rd |= r_
+            wr |= w_
+            if c == 10:
+                masks += 16 * W
+            elif c in (8, 9, 11, 12, 13, 15, 16):
+                masks += 8 * W
+            i += 1
+        out["segments"] += 1
+        out["segment_columns"] += (len(rd) + len(wr) + 2) * col
+        out["masks"] += masks
+    nr, nd = census["random"], census["determined"]
+    kr, kd = census["random_rows"], census["determined_steps"]
+    out["measurements"] = 8 * ((4 * R + 2) * nr + 2 * R * kr) + 8 * ((2 * n + R + 2) * nd + R * kd)
+    out["total"] = out["segment_columns"] + out["masks"] + out["measurements"]
+    return out
+
+
+def cpu_model():
+    """The host CPU's model name (lscpu, else /proc/cpuinfo)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     O.set_threads(cores)
+    return O
+
+
+def cpu_baseline(w, target_s, runs=3):
+    """The CPU oracle (reference int32 layout [B][2n+1][n], OpenMP over
+    (shot, row) for gates and over shots for measurements, as the Kokkos
+    index spaces run) on the first S shots of the same circuit.  S is scaled
+    from a 64-shot calibration run so one timed run takes about `target_s`
+    seconds; shots are independent (coins and noise keyed by shot index), so
+    S shots are an exact subset of the job and the rate scales by B/S.  As
+    benchmarks/run_circuit.cpp:25-29 times it: circuit execution only (state
+    set-up outside), median of `runs` runs, threads bound close
+    (OMP_PROC_BIND, set before the OpenMP runtime starts)."""
+    import numpy as np
+    O = _oracle()
     ops32 = w.ops.astype(np.int32)
     cal = O.OracleState(w.num_qubits, 64, w.seed)
     t = time.perf_counter()
     cal.run_program(ops32)
     t64 = time.perf_counter() - t
     S = int(min(w.batch_size, max(64, 64 * round(target_s / max(t64, 1e-3)))))
-    st = O.OracleState(w.num_qubits, S, w.seed)
-    t = time.perf_counter()
-    st.run_program(ops32)
-    dt = time.perf_counter() - t
+    times = []
+    for _ in range(runs):
+        st = O.OracleState(w.num_qubits, S, w.seed)
+        t = time.perf_counter()
+        st.run_program(ops32)
+        times.append(time.perf_counter() - t)
+        del st
+    dt = float(np.median(times))
     return {
         "value": w.gates * S / dt,
         "unit": "gate-shots/s",
         "cores": O.get_threads(),
         "kind": "port",
-        "sample": f"{w.name}: full circuit on shots [0,{S}) of {w.batch_size}, "
-                  f"{dt:.2f} s, int32 [B][2n+1][n] layout",
+        "sample": f"{w.name}: full circuit on shots [0,{S}) of {w.batch_size} "
+                  f"(rate scaled by {w.batch_size / S:.3g}), median of {runs} runs "
+                  f"{', '.join(f'{x:.2f}' for x in times)} s, int32 [B][2n+1][n] layout",
         "meas_shots_per_s": w.measurements * S / dt,
+        "cpu_model": cpu_model(),
+        "omp_proc_bind": os.environ.get("OMP_PROC_BIND", ""),
+        "runs_s": times,
+        "shot_sample": [S, w.batch_size],
     }
 
 
@@ -412,10 +511,7 @@ def cpu_baseline_prefix(w, target_s, shots=64):
     scaled from a 1000-op calibration run to about `target_s` seconds (the
     ops are uniform random, so a prefix has the circuit's mix)."""
     import numpy as np
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    O.set_threads(cores)
+    O = _oracle()
     ops32 = w.ops.astype(np.int32)
     cal = O.OracleState(w.num_qubits, shots, w.seed)
     k0 = min(1000, len(ops32))
@@ -431,7 +527,8 @@ def cpu_baseline_prefix(w, target_s, shots=64):
     gates = int(np.isin(codes, (1, 2, 3, 4, 5, 6, 7)).sum())
     meas = int((codes == 14).sum())
     return {"value": gates * shots / dt, "unit": "gate-shots/s", "cores": O.get_threads(),
-            "kind": "port",
+            "kind": "port", "cpu_model": cpu_model(),
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND", ""),
             "sample": f"{w.name}: first {k} ops ({gates} gates, {meas} measurements) on "
                       f"shots [0,{shots}), {dt:.2f} s, int32 [B][2n+1][n] layout",
             "meas_shots_per_s": meas * shots / dt}
@@ -548,6 +645,9 @@ def main():
     ap.add_argument("--sampling-d", type=int, default=25)
     ap.add_argument("--sampling-shots", type=int, default=1 << 20)
     args = ap.parse_args()
+    # the CPU baseline's OpenMP threads stay on neighbouring cores; the
+    # runtime reads this when it starts, so it is set before anything loads it
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this engine runs on a single MI355X "
               f"(--gpus {args.gpus} is not supported; the path would shard on shots, each "
@@ -646,10 +746,24 @@ def main():
         fk, fl = prog.kernel_ms()
         prog.set_timing(False)
         frame_ms = 1e3 * fwall / args.steps
+        # the same steps with the shared X/Z written into every word group, so
+        # the tableau ends as the persistent form leaves it: a checksum of the
+        # first 64 shots reads the tableau, which writes X/Z (k_frame_xz) first,
+        # and waits for it
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st.reset_clifford_state()
+            prog.run(st, "frame")
+            st.check_sum(0, 64)
+        fxz_ms = 1e3 * (time.perf_counter() - t0) / args.steps
         frame.update({"ms_per_step": frame_ms,
+                      "step": "records only: reset + noise + k_frame; the shared X/Z are "
+                              "written when the tableau is next read",
                       "gate_shots_per_s": w.gates * B / (frame_ms / 1e3),
                       "kernel": "k_frame", "kernel_ms": fk / max(1, fl),
-                      "speedup_vs_persistent": ms_step / frame_ms})
+                      "speedup_vs_persistent": ms_step / frame_ms,
+                      "ms_per_step_with_tableau": fxz_ms,
+                      "speedup_vs_persistent_with_tableau": ms_step / fxz_ms})
         progress(f"frame form {frame_ms:.3f} ms/step")
     except RuntimeError as e:
         frame = {"error": str(e)}
@@ -667,7 +781,9 @@ def main():
 
     gate_shots = w.gates * B
     value = gate_shots / (ms_step / 1e3)
-    achieved = abytes["total"] / (kernel_ms / 1e3) / 1e9
+    fbytes = fused_minimum_bytes(w.ops, n, words, census)
+    achieved = fbytes["total"] / (kernel_ms / 1e3) / 1e9
+    per_op_gbs = abytes["total"] / (kernel_ms / 1e3) / 1e9
     out = {
         "metric": METRIC,
         "value": value,
@@ -688,9 +804,17 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_program", "kernel_ms": kernel_ms,
-                     "algorithmic_bytes": abytes["total"]},
+                     "model": "fused minimum: each gate segment reads every column it "
+                              "uses and writes every column it changes once; "
+                              "measurements by census",
+                     "algorithmic_bytes": fbytes["total"],
+                     # the per-op, unfused count (every op's columns from HBM),
+                     # labelled apart: a fused kernel moves far less than this
+                     "per_op_model": {"bytes": abytes["total"], "achieved": per_op_gbs,
+                                      "frac": per_op_gbs / HBM_PEAK_GBS}},
         "meas_shots_per_s": w.measurements * B / (ms_step / 1e3),
-        "hbm_frac_step": abytes["total"] / (ms_step / 1e3) / (HBM_PEAK_GBS * 1e9),
+        "hbm_frac_step": fbytes["total"] / (ms_step / 1e3) / (HBM_PEAK_GBS * 1e9),
+        "fused_minimum_bytes": fbytes,
         "graph_ms_per_step": graph_ms,
         "graph_gate_shots_per_s": gate_shots / (graph_ms / 1e3),
         "plain_ms_per_step": plain_ms,

@@ -352,6 +352,10 @@ int pq_server_set_policy(pq_server sv, int policy, double age_limit_s);
  * shared by requests of the same structure (n and ops, noise thresholds
  * aside).  Results are the same either way. */
 int pq_server_set_form(pq_server sv, int form);
+/* Device bytes a request of these ops would reserve against the budget under
+ * the server's current form (host only). */
+int pq_server_request_bytes(pq_server sv, uint64_t num_qubits, uint64_t batch_size,
+                            const pq_op *ops, uint64_t nops, uint64_t *bytes);
 /* Requests run in the frame form, frame plans built, plans reused. */
 int pq_server_frame_stats(pq_server sv, uint64_t *frame_runs, uint64_t *plan_builds,
                           uint64_t *plan_hits);

@@ -1,6 +1,7 @@
 // pybind11 module `plaquette_simulator_bindings`: the reference's Python
 // surface (plaquette_simulator/src/Bindings.cpp:11-31, same class and method
 // names) over the gfx950 engine, plus programs, records and timing helpers.
+#include <hip/hip_runtime_api.h>
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -81,6 +82,78 @@ struct Server {
     }
   }
 };
+
+// Zero-copy export of device buffers in the DLPack exchange format (the
+// DLManagedTensor of dlpack.h, restated here: a C struct layout shared by
+// every framework that reads it).  Device type 10 is kDLROCM, so torch,
+// CuPy and others on ROCm see HIP device memory.
+namespace dlpack {
+struct Device {
+  int32_t device_type, device_id;
+};
+struct DataType {
+  uint8_t code, bits;
+  uint16_t lanes;
+};
+struct Tensor {
+  void *data;
+  Device device;
+  int32_t ndim;
+  DataType dtype;
+  int64_t *shape, *strides;
+  uint64_t byte_offset;
+};
+struct ManagedTensor {
+  Tensor dl_tensor;
+  void *manager_ctx;
+  void (*deleter)(ManagedTensor *);
+};
+constexpr int32_t kROCm = 10;
+constexpr uint8_t kUInt = 1;
+
+// Owns the shape and a reference to the Python object whose memory it is.
+struct Ctx {
+  int64_t shape[3];
+  py::object owner;
+};
+
+void deleter(ManagedTensor *t) {
+  if (!t) return;
+  {
+    py::gil_scoped_acquire gil;
+    delete static_cast<Ctx *>(t->manager_ctx);
+  }
+  delete t;
+}
+
+// A "dltensor" capsule over `ptr` (uint64 words, C order, shape dims).
+py::capsule export_words(void *ptr, int device, const std::vector<int64_t> &dims,
+                         py::object owner) {
+  auto *ctx = new Ctx();
+  for (size_t i = 0; i < dims.size(); ++i) ctx->shape[i] = dims[i];
+  ctx->owner = std::move(owner);
+  auto *t = new ManagedTensor();
+  t->dl_tensor.data = ptr;
+  t->dl_tensor.device = Device{kROCm, device};
+  t->dl_tensor.ndim = (int32_t)dims.size();
+  t->dl_tensor.dtype = DataType{kUInt, 64, 1};
+  t->dl_tensor.shape = ctx->shape;
+  t->dl_tensor.strides = nullptr;  // compact row-major
+  t->dl_tensor.byte_offset = 0;
+  t->manager_ctx = ctx;
+  t->deleter = deleter;
+  // a capsule nobody consumed frees the tensor; a consumer renames it
+  // "used_dltensor" and calls the deleter itself
+  return py::capsule(t, "dltensor", [](PyObject *cap) {
+    if (PyCapsule_IsValid(cap, "dltensor")) {
+      auto *mt = static_cast<ManagedTensor *>(PyCapsule_GetPointer(cap, "dltensor"));
+      if (mt && mt->deleter) mt->deleter(mt);
+    } else {
+      PyErr_Clear();
+    }
+  });
+}
+}  // namespace dlpack
 
 }  // namespace
 
@@ -241,6 +314,51 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
              s.HostToDevice(x, z, r);
            })
       .def("device_to_device", &State::DeviceToDevice)
+      // DLPack capsules over the device bit-planes (no copy).  The state's
+      // queued work is finished first; the capsule keeps the state alive.
+      // planes: uint64 [W][2n+1][2n+1] = word[w][col][row] (col < n: x, then
+      // z, then r).  records: uint64 [nrec][2][W] (results, determined); the
+      // record arena moves when records are added, so take a new capsule
+      // after measurements that create records.
+      .def("planes_dlpack",
+           [](py::object self) {
+             State &s = self.cast<State &>();
+             void *tab = nullptr;
+             uint64_t W = 0, R = 0;
+             check(pq_device_handles(s.Handle(), &tab, nullptr, &W, &R, nullptr), "planes_dlpack");
+             check(pq_synchronize(s.Handle()), "planes_dlpack");
+             int dev = 0;
+             (void)hipGetDevice(&dev);
+             return dlpack::export_words(tab, dev, {(int64_t)W, (int64_t)R, (int64_t)R}, self);
+           })
+      .def("read_records",
+           [](State &s) {
+             void *recs = nullptr;
+             uint64_t W = 0, nrec = 0;
+             check(pq_device_handles(s.Handle(), nullptr, &recs, &W, nullptr, nullptr),
+                   "read_records");
+             check(pq_info(s.Handle(), nullptr, nullptr, nullptr, &nrec), "read_records");
+             check(pq_synchronize(s.Handle()), "read_records");
+             py::array_t<uint64_t> a({(size_t)nrec, (size_t)2, (size_t)W});
+             if (nrec && W &&
+                 hipMemcpy(a.mutable_data(), recs, nrec * 2 * W * 8, hipMemcpyDeviceToHost) !=
+                     hipSuccess)
+               throw Plaquette::Exception("read_records: device copy failed");
+             return a;
+           })
+      .def("records_dlpack",
+           [](py::object self) {
+             State &s = self.cast<State &>();
+             void *recs = nullptr;
+             uint64_t W = 0, nrec = 0;
+             check(pq_device_handles(s.Handle(), nullptr, &recs, &W, nullptr, nullptr),
+                   "records_dlpack");
+             check(pq_info(s.Handle(), nullptr, nullptr, nullptr, &nrec), "records_dlpack");
+             check(pq_synchronize(s.Handle()), "records_dlpack");
+             int dev = 0;
+             (void)hipGetDevice(&dev);
+             return dlpack::export_words(recs, dev, {(int64_t)nrec, 2, (int64_t)W}, self);
+           })
       .def("get_record",
            [](State &s, size_t id) {
              const size_t B = s.GetBatchSize();
@@ -302,8 +420,8 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
       .def(
           "run",
           [](Program &p, State &s, const py::object &form) {
-            if (s.Handle() != p.s) throw std::invalid_argument("program belongs to another state");
-            check(pq_program_run(p.s, p.p, form_of(form)), "Program.run");
+            // any state of the shape the program was compiled for
+            check(pq_program_run(s.Handle(), p.p, form_of(form)), "Program.run");
           },
           py::arg("state"), py::arg("form") = "persistent")
       .def("info",
@@ -356,7 +474,6 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
       .def(
           "add",
           [](Batch &b, State &st, Program &p) {
-            if (st.Handle() != p.s) throw std::invalid_argument("program belongs to another state");
             check(pq_batch_add(b.b, st.Handle(), p.p), "Batch.add");
             b.keep.push_back(py::cast(&st));
             b.keep.push_back(py::cast(&p));
@@ -461,6 +578,14 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
           },
           py::arg("form"),
           "'frame': requests whose ops allow it run in the frame form (plans shared)")
+      .def("request_bytes",
+           [](Server &sv, uint64_t n, uint64_t B,
+              py::array_t<uint32_t, py::array::c_style | py::array::forcecast> ops) {
+             uint64_t k = 0, bytes = 0;
+             const pq_op *o = ops_ptr(ops, &k);
+             check(pq_server_request_bytes(sv.s, n, B, o, k, &bytes), "Server.request_bytes");
+             return bytes;
+           })
       .def("frame_stats",
            [](Server &sv) {
              uint64_t runs = 0, builds = 0, hits = 0;

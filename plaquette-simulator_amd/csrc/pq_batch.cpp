@@ -128,8 +128,13 @@ extern "C" int pq_batch_add(pq_batch b, pq_state s, pq_program p) {
     return pq_set_error(PQ_ERR_ARG, "program was compiled for a state of another shape");
   if (s->n > measure_max_qubits())
     return pq_set_error(PQ_ERR_UNSUPPORTED, "batched form supports at most 4095 qubits");
-  for (auto &m : b->members)
+  for (auto &m : b->members) {
     if (m.first == s) return pq_set_error(PQ_ERR_ARG, "state already in the batch");
+    // a program's noise planes and side buffer serve one run at a time
+    if (m.second == p)
+      return pq_set_error(PQ_ERR_ARG,
+                          "program already in the batch: give each member a program of its own");
+  }
   if (b->members.size() >= (1u << 23)) return pq_set_error(PQ_ERR_UNSUPPORTED, "too many members");
   b->members.emplace_back(s, p);
   b->built = false;
@@ -278,6 +283,8 @@ static int run_frame_batch(pq_batch b) {
     }
     HIPCHK(hipEventRecord(b->member_ev[c], s->stream));
     HIPCHK(hipStreamWaitEvent(b->stream, b->member_ev[c], 0));
+    // and the program's last standalone run, wherever it ran
+    if (p->done_ev) HIPCHK(hipStreamWaitEvent(b->stream, p->done_ev, 0));
     if ((rc = upload_frame_plan(p, b->stream))) return rc;
     BatchCircuit &bc = b->h_circ[c];
     bc = BatchCircuit{};
@@ -389,6 +396,7 @@ extern "C" int pq_batch_run(pq_batch b, int form) {
     if (rc) return rc;
     if ((rc = materialize_tableau(s))) return rc;
     HIPCHK(hipStreamSynchronize(s->stream));  // the member's own queued work first
+    if (p->done_ev) HIPCHK(hipEventSynchronize(p->done_ev));  // the program's last run
     BatchCircuit &bc = b->h_circ[c];
     bc.a = program_args(s, p, b->lds_budget);
     bc.a.census = nullptr;

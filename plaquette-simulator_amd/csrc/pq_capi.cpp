@@ -101,6 +101,7 @@ pq_state_s::~pq_state_s() {
     dev_free(p, alloc_stream);
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
+  if (order_ev) (void)hipEventDestroy(order_ev);
   for (hipEvent_t e : aux_ev) (void)hipEventDestroy(e);
   if (aux) (void)hipStreamDestroy(aux);
   if (stream && own_stream) (void)hipStreamDestroy(stream);
@@ -291,11 +292,21 @@ extern "C" int pq_copy_from(pq_state dst, pq_state src) {
   FULL_ONLY(src);
   ARGCHK(dst && src, "null state");
   ARGCHK(dst->n == src->n && dst->B == src->B, "DeviceToDevice needs equal n and B");
-  if (!dst->tab) return PQ_OK;
+  if (!dst->tab || dst == src) return PQ_OK;
   dst->xz_identity = src->xz_identity;
-  HIPCHK(hipStreamSynchronize(src->stream));
+  // Kokkos::deep_copy has finished before either state changes again
+  // (CliffordState.hpp:268-274).  Here the copy runs on the source's stream
+  // after the destination's queued work, and the destination's stream waits
+  // for it: later work on either state is ordered after the copy, and the
+  // host does not block.
+  for (pq_state st : {dst, src})
+    if (!st->order_ev) HIPCHK(hipEventCreateWithFlags(&st->order_ev, hipEventDisableTiming));
+  HIPCHK(hipEventRecord(dst->order_ev, dst->stream));
+  HIPCHK(hipStreamWaitEvent(src->stream, dst->order_ev, 0));
   HIPCHK(hipMemcpyAsync(dst->tab, src->tab, dst->W * dst->C * dst->R * 8,
-                        hipMemcpyDeviceToDevice, dst->stream));
+                        hipMemcpyDeviceToDevice, src->stream));
+  HIPCHK(hipEventRecord(src->order_ev, src->stream));
+  HIPCHK(hipStreamWaitEvent(dst->stream, src->order_ev, 0));
   return PQ_OK;
 }
 

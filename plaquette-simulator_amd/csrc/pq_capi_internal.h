@@ -34,6 +34,8 @@ struct pq_state_s {
   hipStream_t alloc_stream = nullptr;
   bool own_stream = true;
   hipEvent_t ev[8] = {};
+  // orders this state's stream against another state's (DeviceToDevice)
+  hipEvent_t order_ev = nullptr;
   // X/Z hold the |0...0> tableau in every shot (frame programs start there)
   bool xz_identity = false;
   // reset left X/Z unwritten (the R column is reset): they hold |0...0>

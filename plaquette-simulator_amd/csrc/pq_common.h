@@ -70,7 +70,8 @@ PQ_HD void add_mod4(uint64_t &a0, uint64_t &a1, uint64_t b0, uint64_t b1) {
 //  or the part of a layer inside one staging chunk -- also the run length
 //  (bits 8-19);
 //  PQ_OP_NOISE alone: qa, na;  M: qa, slot = record id, t0 = coin
-//  threshold, (na, nb) = coin stream (low, high 32 bits).
+//  threshold, (na, nb) = coin stream (low, high 32 bits), pad = kMeasSuperseded
+//  when a later M of the same measurement segment writes the same record.
 struct DInstr {
   uint32_t code, qa, qb, slot, na, nb, t0, pad;
 };
@@ -80,6 +81,11 @@ constexpr uint32_t kLayerStart = 0x80000000u;
 constexpr uint32_t kBatchLenMask = 0xFFu;
 constexpr uint32_t kRunShift = 8;
 constexpr uint32_t kRunMask = 0xFFFu;
+// An M whose record a later M of its segment overwrites: the measurement still
+// runs (it may leave the scratch row), but its record write is skipped, so the
+// last write in program order is the one that stays even when both run at
+// once in a batch of fully determined measurements.
+constexpr uint32_t kMeasSuperseded = 1u;
 
 // Segment of a compiled program: a run of gate batches or a run of
 // measurements.  Gate segments hold `count` instructions in batches.  A gate

@@ -142,8 +142,18 @@ int compile_program(pq_program_s *p) {
     while (j < nops && (ops[j].code == PQ_OP_M) == meas) ++j;
     DSegment sg{meas ? 1u : 0u, (uint32_t)p->instrs.size(), 0, 0, 0, 0, 0, 0};
     if (meas) {
+      // a segment's fully determined measurements run in parallel, one wave
+      // each; of several writes to one record only the last in program order
+      // may happen (MeasureQubit with an existing id overwrites it,
+      // CliffordState.hpp:608-611)
       for (uint64_t k = i; k < j; ++k) {
-        DInstr I{PQ_OP_M, ops[k].q0, 0, ops[k].slot, nmeas, 0, ops[k].t0, 0};
+        uint32_t pad = 0;
+        for (uint64_t l = k + 1; l < j; ++l)
+          if (ops[l].slot == ops[k].slot) {
+            pad = kMeasSuperseded;
+            break;
+          }
+        DInstr I{PQ_OP_M, ops[k].q0, 0, ops[k].slot, nmeas, 0, ops[k].t0, pad};
         ++nmeas;
         p->instrs.push_back(I);
       }
@@ -256,6 +266,7 @@ void pq_program_s::drop_graph() {
 pq_program_s::~pq_program_s() {
   drop_graph();
   for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  if (done_ev) (void)hipEventDestroy(done_ev);
   for (void *q : {(void *)d_census, (void *)d_segs, (void *)d_segq, (void *)d_instrs,
                   (void *)d_noise, (void *)d_planes, (void *)d_side})
     dev_free(q, alloc_stream);
@@ -356,8 +367,9 @@ extern "C" int pq_program_set_launch(pq_program p, uint32_t threads, uint32_t bl
 }
 
 // Record slots: M ops write slot <= current count (== count appends), flagged
-// ops read slot < current count, checked in program order.
-int prepare_records(pq_state s, pq_program p) {
+// ops read slot < current count, checked in program order.  Sets
+// p->prep_end to the record count after the run; allocates nothing.
+int check_records(pq_state s, pq_program p) {
   // the check depends only on the ops and the state's record count, so a
   // program rerun on a state with the same count reuses it
   if (!p->prep_valid || p->prep_start != s->nrec) {
@@ -382,7 +394,13 @@ int prepare_records(pq_state s, pq_program p) {
     p->prep_start = s->nrec;
     p->prep_end = cur;
   }
-  int rc = ensure_records(s, p->prep_end);
+  return PQ_OK;
+}
+
+int prepare_records(pq_state s, pq_program p) {
+  int rc = check_records(s, p);
+  if (rc) return rc;
+  rc = ensure_records(s, p->prep_end);
   if (rc) return rc;
   for (uint32_t slot : p->m_slots) s->rec_valid[slot] = 0;
   return PQ_OK;
@@ -433,7 +451,7 @@ ProgramArgs program_args(pq_state s, pq_program p, uint64_t lds_budget) {
   const uint64_t waves = p->threads / 64;
   const uint64_t budget = lds_budget;
   const uint64_t base = program_lds_bytes(a.n, 0);
-  const uint64_t slab = 16ull * a.n;
+  const uint64_t slab = 8 * det_region_bytes_words(a.n);
   uint64_t ndw = (budget > base && slab) ? (budget - base) / slab : 0;
   a.det_waves = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(waves, ndw));
   a.segq = p->d_segq;
@@ -466,7 +484,26 @@ static int enqueue_persistent(pq_state s, pq_program p, bool timed) {
   return PQ_OK;
 }
 
+static int program_run(pq_state s, pq_program p, int form);
+
+// A program's device buffers (noise planes, the frame form's side buffer)
+// serve every run, so a run on a stream other than the last run's waits for
+// that run's device work first.
 extern "C" int pq_program_run(pq_state s, pq_program p, int form) {
+  if (!s || !p) return pq_set_error(PQ_ERR_ARG, "null argument");
+  const bool device_run = s->tab && form != PQ_RUN_PLAIN;
+  if (device_run && p->done_ev && p->last_stream != s->stream)
+    HIPCHK(hipStreamWaitEvent(s->stream, p->done_ev, 0));
+  const int rc = program_run(s, p, form);
+  if (rc == PQ_OK && device_run) {
+    if (!p->done_ev) HIPCHK(hipEventCreateWithFlags(&p->done_ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(p->done_ev, s->stream));
+    p->last_stream = s->stream;
+  }
+  return rc;
+}
+
+static int program_run(pq_state s, pq_program p, int form) {
   if (!s || !p) return pq_set_error(PQ_ERR_ARG, "null argument");
   if (p->n != s->n || p->W != s->W)
     return pq_set_error(PQ_ERR_ARG, "program was compiled for a state of another shape");

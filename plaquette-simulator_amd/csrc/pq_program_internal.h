@@ -47,6 +47,10 @@ struct pq_program_s {
   bool prep_valid = false;
   uint64_t prep_start = 0, prep_end = 0;
   std::vector<uint32_t> m_slots;
+  // end of the last run's device work (on last_stream): a run on another
+  // stream waits for it, since the noise planes and side buffer are shared
+  hipEvent_t done_ev = nullptr;
+  hipStream_t last_stream = nullptr;
   void drop_graph();
   ~pq_program_s();
 };
@@ -60,7 +64,10 @@ int validate_ops(uint64_t n, const pq_op *ops, uint64_t nops);
 
 constexpr uint64_t kProgramLdsBytes = 158 * 1024;  // dynamic-LDS cap (kProgramLdsBudget there)
 
-// Shared with the batch runner.
+// Shared with the batch runner.  check_records validates the record ids
+// and sets p->prep_end without allocating; prepare_records also sizes the
+// state's record arena and marks the measured records' host copies stale.
+int check_records(pq_state s, pq_program p);
 int prepare_records(pq_state s, pq_program p);
 pq::ProgramArgs program_args(pq_state s, pq_program p, uint64_t lds_budget);
 // pq_program_create for a server-owned state: stream-ordered allocation and

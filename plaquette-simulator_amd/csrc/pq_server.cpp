@@ -455,6 +455,19 @@ extern "C" int pq_server_set_form(pq_server sv, int form) {
   return PQ_OK;
 }
 
+extern "C" int pq_server_request_bytes(pq_server sv, uint64_t n, uint64_t B, const pq_op *ops,
+                                       uint64_t nops, uint64_t *bytes) {
+  if (!sv || !bytes || (!ops && nops)) return pq_set_error(PQ_ERR_ARG, "null argument");
+  bool lite;
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    lite = sv->form == PQ_RUN_FRAME && frame_eligible(ops, nops);
+  }
+  uint64_t nrec = 0;
+  *bytes = request_bytes(n, B, ops, nops, &nrec, lite);
+  return PQ_OK;
+}
+
 extern "C" int pq_server_frame_stats(pq_server sv, uint64_t *frame_runs, uint64_t *plan_builds,
                                      uint64_t *plan_hits) {
   if (!sv) return pq_set_error(PQ_ERR_ARG, "null server");

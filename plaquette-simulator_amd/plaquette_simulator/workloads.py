@@ -269,3 +269,27 @@ def config_mixed_batch(count=256, seed=11, p=1e-3, dmin=3, dmax=25, smin=64, sma
         out.append(Workload(f"sc_d{d}_b{shots}_{i}", n, shots, cseed, ops,
                             {"d": d, "rounds": d, "p": p}))
     return out
+
+
+def serving_request(rng, dmin=3, dmax=21, smin=256, smax=65536, pmin=5e-4, pmax=5e-3):
+    """One circuit-sampling request of the serving stream: a surface-code
+    memory-Z circuit with d uniform over odd {dmin..dmax}, rounds d, shots
+    log-uniform in [smin, smax], p uniform in [pmin, pmax] and a state seed.
+    Returns (d, shots, p, seed); the op list is surface_code_memory_z(d, d, p)."""
+    d = int(rng.choice(list(range(dmin, dmax + 1, 2))))
+    shots = int(round(math.exp(rng.uniform(math.log(smin), math.log(smax)))))
+    p = float(rng.uniform(pmin, pmax))
+    seed = int(rng.integers(0, 2 ** 31 - 1))
+    return d, shots, p, seed
+
+
+def config_serving_stream(lam, count, seed=2024):
+    """`count` Poisson arrivals at rate `lam` per second of serving requests:
+    a list of (arrival time in s, d, shots, p, seed)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for _ in range(count):
+        t += rng.exponential(1.0 / lam)
+        out.append((t,) + serving_request(rng))
+    return out
