@@ -275,6 +275,20 @@ int pq_program_set_timing(pq_program p, int enable);
 int pq_program_kernel_ms(pq_program p, float *total_ms, uint64_t *launches);
 /* Create, run once (and wait), destroy. */
 int pq_run_program(pq_state s, const pq_op *ops, uint64_t nops, int form);
+/* No reference counterpart.  For callers that cannot allocate device memory
+ * (tight serving loops, harnesses that forbid hipMalloc): pq_run_program(s,
+ * ops, nops, PQ_RUN_PERSISTENT) with the same results, but no hipMalloc,
+ * hipFree or other allocation.  The compiled program, its noise planes
+ * (noise ops x 2 x W words) and the run's parameters live in one library-wide
+ * device array of PQ_STATIC_SCRATCH_BYTES per GPU, held until the process
+ * ends; a program needing more fails with PQ_ERR_UNSUPPORTED (d=9 surface
+ * code at 16384 shots needs 25 MB).  The state's record arena must already
+ * hold every record the ops write (records made by earlier measurements or
+ * runs), else PQ_ERR_OOM.  Returns after the run has finished.  Every call on
+ * one GPU shares the array: calls that overlap on one GPU (from several host
+ * threads) race and must not be made. */
+#define PQ_STATIC_SCRATCH_BYTES (64ull << 20)
+int pq_run_program_static(pq_state s, const pq_op *ops, uint64_t nops);
 /* Wait for all work queued on the state. */
 int pq_synchronize(pq_state s);
 
