@@ -207,6 +207,11 @@ int pq_host_to_device(pq_state s, const int32_t *x, const int32_t *z,
                       const int32_t *r);
 /* DeviceToDevice (:268-274): dst's tableau := src's (same n, B). */
 int pq_copy_from(pq_state dst, pq_state src);
+/* No reference counterpart (tests, tools): the number of tableau words and of
+ * record words (over the records both states hold, plus one if the record
+ * counts differ) in which two states of equal n and B differ, counted on the
+ * device after both states' queued work. */
+int pq_compare(pq_state a, pq_state b, uint64_t *planes_differ, uint64_t *records_differ);
 /* CheckSum(b0, b1) (:327-338) with a 64-bit total; b1 < 0 means B. */
 int pq_checksum(pq_state s, int64_t b0, int64_t b1, int64_t *out);
 /* Raw bit-plane tableau, W*(2n+1)*R words in the device layout. */

@@ -331,6 +331,13 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
              (void)hipGetDevice(&dev);
              return dlpack::export_words(tab, dev, {(int64_t)W, (int64_t)R, (int64_t)R}, self);
            })
+      .def("compare",
+           [](State &s, State &other) {
+             uint64_t dp = 0, dr = 0;
+             check(pq_compare(s.Handle(), other.Handle(), &dp, &dr), "compare");
+             return py::make_tuple(dp, dr);
+           },
+           "(tableau words, record words) in which the two states differ (on the device)")
       .def("read_records",
            [](State &s) {
              void *recs = nullptr;

@@ -707,6 +707,36 @@ extern "C" int pq_checksum(pq_state s, int64_t b0, int64_t b1, int64_t *out) {
   return PQ_OK;
 }
 
+extern "C" int pq_compare(pq_state a, pq_state b, uint64_t *planes_differ,
+                          uint64_t *records_differ) {
+  FULL_ONLY(a);
+  FULL_ONLY(b);
+  ARGCHK(a && b, "null state");
+  ARGCHK(a->n == b->n && a->B == b->B, "pq_compare needs equal n and B");
+  // on a's stream, after b's queued work
+  if (a != b) {
+    for (pq_state st : {a, b})
+      if (!st->order_ev) HIPCHK(hipEventCreateWithFlags(&st->order_ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(b->order_ev, b->stream));
+    HIPCHK(hipStreamWaitEvent(a->stream, b->order_ev, 0));
+  }
+  const uint64_t nr = std::min(a->nrec, b->nrec);
+  HIPCHK(launch_zero_word(a->stream, (unsigned long long *)a->csum));
+  HIPCHK(launch_count_diff(a->stream, a->tab, b->tab, a->W * a->C * a->R,
+                           (unsigned long long *)a->csum));
+  unsigned long long v[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(&v[0], a->csum, 8, hipMemcpyDeviceToHost, a->stream));
+  HIPCHK(launch_zero_word(a->stream, (unsigned long long *)a->csum));
+  if (nr && a->recs && b->recs)
+    HIPCHK(launch_count_diff(a->stream, a->recs, b->recs, nr * 2 * a->W,
+                             (unsigned long long *)a->csum));
+  HIPCHK(hipMemcpyAsync(&v[1], a->csum, 8, hipMemcpyDeviceToHost, a->stream));
+  HIPCHK(hipStreamSynchronize(a->stream));
+  if (planes_differ) *planes_differ = v[0];
+  if (records_differ) *records_differ = v[1] + (a->nrec != b->nrec ? 1 : 0);
+  return PQ_OK;
+}
+
 extern "C" int pq_read_planes(pq_state s, uint64_t *host_words) {
   FULL_ONLY(s);
   ARGCHK(s && host_words, "null argument");

@@ -635,24 +635,6 @@ constexpr int kSmallRegs = 3, kSmallThreads = 768;
 constexpr int kLargeRegs = 8, kLargeThreads = 512;
 constexpr uint32_t kMeasBatch = 64;   // most measurements pivot-checked per pass
 constexpr uint32_t kMeasWindow0 = 8;  // first window of a run
-// A fully determined measurement's chain (the destabilizers with x_q set,
-// ascending) is listed by the pivot check when it has at most kChainList
-// members; its stabilizer rows then come into the wave's LDS region by LDS
-// DMA, up to det_rows(n) rows per round trip, and are folded from there.
-constexpr uint32_t kChainList = 8;
-__host__ __device__ inline uint32_t det_row_words(uint32_t n) {
-  return (2 * n + 1 + 31) / 32 * 32;  // a row (x, z, r) in 32-word DMA chunks
-}
-__host__ __device__ inline uint32_t det_rows(uint32_t n) {
-  return n <= kSmallQubits ? 4u : 2u;
-}
-// Chains come from LDS up to 512 qubits (the scratch row of a chain lives in
-// registers, at most 8 columns per lane); larger states keep the 2n-word slab.
-constexpr uint32_t kLdsChainMaxQubits = 512;
-__host__ __device__ inline uint64_t det_region_words(uint32_t n) {
-  const uint64_t rows = n <= kLdsChainMaxQubits ? (uint64_t)det_rows(n) * det_row_words(n) : 0;
-  return rows > 2ull * n ? rows : 2ull * n;
-}
 
 // Gate segment, processed in chunks of up to kStage instructions.  Each
 // chunk's instructions and the per-word shot masks they need (fused noise
@@ -1378,99 +1360,10 @@ __device__ void determined_wave(uint64_t *__restrict__ T, uint64_t w, uint32_t n
   }
 }
 
-// determined_wave with its chain listed by the pivot check (rows lrow[0..cnt)
-// ascending, per-shot member masks lmask) and the stabilizer rows brought
-// into the wave's LDS region `buf` by LDS DMA, up to det_rows(n) per round
-// trip (one row: det_row_words(n) words, lane l of a wave-wide 4-byte DMA
-// carrying half l & 1 of column word l / 2), then folded from LDS: the same
-// steps and arithmetic as determined_wave, with one global round trip per
-// det_rows(n) chain steps instead of one per step.  n <= 64 * NREG.
-template <int NREG>
-__device__ void determined_wave_lds(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
-                                    uint32_t R, const uint32_t *lrow, const uint64_t *lmask,
-                                    uint32_t cnt, uint64_t *buf, uint64_t *rec, uint64_t W,
-                                    bool write_scratch, unsigned long long *census) {
-  const uint32_t lane = lane_id();
-  const uint64_t Ru = R;
-  const uint64_t scratch = 2ull * n;
-  const uint32_t rw = det_row_words(n), rb = det_rows(n);
-  typedef __attribute__((address_space(3))) void lds_void;
-  uint64_t sx[NREG], sz[NREG];
-#pragma unroll
-  for (int u = 0; u < NREG; ++u) sx[u] = sz[u] = 0;
-  uint64_t racc = 0;
-  for (uint32_t t0 = 0; t0 < cnt; t0 += rb) {
-    const uint32_t nb = min(rb, cnt - t0);
-    // the previous rows' LDS reads are done before the DMA overwrites them
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-#pragma unroll 1
-    for (uint32_t b = 0; b < nb; ++b) {
-      const uint64_t srow = n + lrow[t0 + b];
-#pragma unroll 1
-      for (uint32_t ch = 0; ch < rw / 32; ++ch) {
-        const uint32_t c = min(ch * 32 + (lane >> 1), 2 * n);  // columns past r reload r
-        const uint32_t *src = reinterpret_cast<const uint32_t *>(T + (uint64_t)c * Ru + srow) +
-                              (lane & 1);
-        __builtin_amdgcn_global_load_lds(src, (lds_void *)(buf + (uint64_t)b * rw + ch * 32), 4,
-                                         0, 0);
-      }
-    }
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-#pragma unroll 1
-    for (uint32_t b = 0; b < nb; ++b) {
-      const uint64_t m = lmask[t0 + b];
-      const uint64_t *row = buf + (uint64_t)b * rw;
-      uint64_t c0 = 0, c1 = 0;
-#pragma unroll
-      for (int u = 0; u < NREG; ++u) {
-        const uint32_t j = lane + 64 * u;
-        if (j < n) {
-          const uint64_t cx = row[j], cz = row[n + j];
-          phase_acc(sx[u], sz[u], cx, cz, c0, c1);
-          sx[u] ^= m & cx;
-          sz[u] ^= m & cz;
-        }
-      }
-#pragma unroll
-      for (uint32_t off = 32; off > 0; off >>= 1) {
-        const uint64_t t0_ = __shfl_xor(c0, off, 64);
-        const uint64_t t1_ = __shfl_xor(c1, off, 64);
-        add_mod4(c0, c1, t0_, t1_);
-      }
-      racc ^= m & (row[2 * n] ^ c1);
-    }
-  }
-  if (write_scratch) {
-#pragma unroll
-    for (int u = 0; u < NREG; ++u) {
-      const uint32_t j = lane + 64 * u;
-      if (j < n) {
-        T[(uint64_t)j * Ru + scratch] = sx[u];
-        T[(uint64_t)(n + j) * Ru + scratch] = sz[u];
-      }
-    }
-  }
-  if (lane == 0) {
-    if (rec) {
-      rec[w] = racc;
-      rec[W + w] = ~0ull;
-    }
-    if (census) {
-      atomicAdd(census + 1, 1ull);
-      atomicAdd(census + 3, (unsigned long long)cnt);
-    }
-    if (write_scratch) T[scratch * Ru + scratch] = racc;
-  }
-}
-
 // Dynamic LDS of k_program: s_src[2n+1] u64 | s_list[n+1] u32 (padded) |
-// s_rm[kMeasBatch] u64 | the pivot check's chain lists (s_cnt[kMeasBatch]
-// u32, s_lrow[kMeasBatch][kChainList] u32, s_lmask[kMeasBatch][kChainList]
-// u64) | ndw wave regions of det_region_words(n) u64 (a determined chain's
-// row buffers, or its 2n-word slab).
+// s_rm[kMeasBatch] u64 | ndw slabs of 2n u64.
 __host__ __device__ inline size_t program_lds_words(uint32_t n, uint32_t ndw) {
-  return (2ull * n + 1) + ((uint64_t)n + 2) / 2 + kMeasBatch + kMeasBatch / 2 +
-         (uint64_t)kMeasBatch * kChainList * 3 / 2 + (uint64_t)ndw * det_region_words(n);
+  return (2ull * n + 1) + ((uint64_t)n + 2) / 2 + kMeasBatch + (uint64_t)ndw * 2 * n;
 }
 
 // Wall-clock ticks per phase (census mode, thread 0).
@@ -1487,12 +1380,7 @@ __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t m
   uint64_t *s_src = dyn;
   uint32_t *s_list = reinterpret_cast<uint32_t *>(dyn + (2 * n + 1));
   uint64_t *s_rm = dyn + (2 * n + 1) + (n + 2) / 2;
-  uint32_t *s_cnt = reinterpret_cast<uint32_t *>(s_rm + kMeasBatch);
-  uint32_t *s_lrow = s_cnt + kMeasBatch;
-  uint64_t *s_lmask = s_rm + kMeasBatch + kMeasBatch / 2 + kMeasBatch * kChainList / 2;
-  uint64_t *s_slab = s_lmask + kMeasBatch * kChainList;
-  const uint64_t region = det_region_words(n);
-  const bool lds_chains = n <= 64u * NREG && n <= kLdsChainMaxQubits;
+  uint64_t *s_slab = s_rm + kMeasBatch;
   const uint64_t C = 2ull * n + 1;
   const uint64_t Ru = a.R;
   const uint32_t lane = lane_id(), wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -1527,37 +1415,17 @@ __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t m
       for (uint32_t idx = wv; idx < nm; idx += nw) {
         const uint64_t *xq = T + (uint64_t)((cinstr_ptr)a.instrs)[k + idx].qa * Ru;
         uint64_t acc = 0;
-        uint32_t cnt = 0;
-        uint32_t *lrow = s_lrow + idx * kChainList;
-        uint64_t *lmask = s_lmask + idx * kChainList;
-        for (uint32_t p0 = 0; p0 < n; p0 += 2 * 64) {
-          uint64_t v[2], d[2];
+        for (uint32_t p0 = 0; p0 < n; p0 += 4 * 64) {
+          uint64_t v[4];
 #pragma unroll
-          for (int u = 0; u < 2; ++u) {
+          for (int u = 0; u < 4; ++u) {
             const uint32_t p = p0 + 64 * u + lane;
             v[u] = p < n ? xq[n + p] : 0ull;
-            d[u] = (lds_chains && p < n) ? xq[p] : 0ull;
           }
-          acc |= v[0] | v[1];
-          if (lds_chains) {
-            // the chain in ascending row order, for the determined case
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-              const uint64_t bal = __ballot(d[u] != 0);
-              const uint32_t pos = cnt + (uint32_t)__popcll(bal & ((1ull << lane) - 1));
-              if (d[u] && pos < kChainList) {
-                lrow[pos] = p0 + 64 * u + lane;
-                lmask[pos] = d[u];
-              }
-              cnt += (uint32_t)__popcll(bal);
-            }
-          }
+          acc |= v[0] | v[1] | v[2] | v[3];
         }
         acc = wave_or(acc);
-        if (lane == 0) {
-          s_rm[idx] = acc;
-          s_cnt[idx] = cnt;
-        }
+        if (lane == 0) s_rm[idx] = acc;
       }
       __syncthreads();
       // first measurement with a pivot: one LDS read per lane and a ballot
@@ -1573,21 +1441,15 @@ __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t m
       // measurements [k, k+jr) are determined in every shot: one wave each
       if (jr > 0) {
         if (wv < ndw) {
-          uint64_t *slab = s_slab + (uint64_t)wv * region;
+          uint64_t *slab = s_slab + (uint64_t)wv * 2 * n;
           for (uint32_t idx = wv; idx < jr; idx += ndw) {
             const DInstr I = ld_instr((cinstr_ptr)a.instrs, k + idx);
             // a record that a later measurement of the segment overwrites is
             // not written: waves of one batch are not ordered
             uint64_t *rec =
                 (I.pad & kMeasSuperseded) ? nullptr : a.recs + (2ull * I.slot) * a.W;
-            const uint32_t cnt = lds_chains ? s_cnt[idx] : ~0u;
-            if (cnt <= kChainList)
-              determined_wave_lds<NREG>(T, w, n, a.R, s_lrow + idx * kChainList,
-                                        s_lmask + idx * kChainList, cnt, slab, rec, a.W,
-                                        idx == jr - 1, a.census);
-            else
-              determined_wave<NREG>(T, w, n, a.R, I.qa, slab, rec, a.W, idx == jr - 1,
-                                    a.census);
+            determined_wave<NREG>(T, w, n, a.R, I.qa, slab, rec, a.W, idx == jr - 1,
+                                  a.census);
           }
         }
         __syncthreads();
@@ -2176,6 +2038,32 @@ hipError_t launch_pack(hipStream_t s, uint64_t *tab, const Geom &g, uint64_t w0,
   return hipGetLastError();
 }
 
+// Words of a and b that differ (n words each), accumulated into *out.
+__global__ __launch_bounds__(256) void k_count_diff(const uint64_t *__restrict__ a,
+                                                    const uint64_t *__restrict__ b, uint64_t n,
+                                                    unsigned long long *out) {
+  unsigned long long c = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    c += a[i] != b[i];
+#pragma unroll
+  for (uint32_t off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+  if (lane_id() == 0 && c) atomicAdd(out, c);
+}
+
+hipError_t launch_zero_word(hipStream_t s, unsigned long long *p) {
+  hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_diff(hipStream_t s, const uint64_t *a, const uint64_t *b, uint64_t n,
+                             unsigned long long *out_dev) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_count_diff, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, a, b, n,
+                     out_dev);
+  return hipGetLastError();
+}
+
 hipError_t launch_checksum(hipStream_t s, const uint64_t *tab, const Geom &g,
                            uint64_t b0, uint64_t b1,
                            unsigned long long *out_dev) {
@@ -2221,8 +2109,6 @@ hipError_t launch_program(hipStream_t s, const ProgramArgs &a, uint32_t threads,
 uint32_t program_max_threads(uint32_t n) {
   return n <= kSmallQubits ? kSmallThreads : kLargeThreads;
 }
-
-uint64_t det_region_bytes_words(uint32_t n) { return det_region_words(n); }
 
 size_t program_lds_bytes(uint32_t n, uint32_t det_waves) {
   // the gate-segment staging area and the measurement area share the LDS

@@ -451,7 +451,7 @@ ProgramArgs program_args(pq_state s, pq_program p, uint64_t lds_budget) {
   const uint64_t waves = p->threads / 64;
   const uint64_t budget = lds_budget;
   const uint64_t base = program_lds_bytes(a.n, 0);
-  const uint64_t slab = 8 * det_region_bytes_words(a.n);
+  const uint64_t slab = 16ull * a.n;
   uint64_t ndw = (budget > base && slab) ? (budget - base) / slab : 0;
   a.det_waves = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(waves, ndw));
   a.segq = p->d_segq;

@@ -89,3 +89,19 @@ def test_state_files(ps, tmp_path):
     for i, (res, det) in enumerate(wire.unpack_records(r["words"], 200)):
         a, b = st.get_record(i)
         assert np.array_equal(res, a) and np.array_equal(det, b)
+
+
+def test_compare_states(ps):
+    from plaquette_simulator import workloads as W
+    ops, n = W.surface_code_memory_z(3, 3, 0.01)
+    a = ps.CliffordState(n, 200, 4, True)
+    b = ps.CliffordState(n, 200, 4, True)
+    a.run_program(ops, "persistent")
+    b.run_program(ops, "graph")
+    assert a.compare(b) == (0, 0)
+    b.apply_hadamard_gate(0)
+    dp, dr = a.compare(b)
+    assert dp > 0 and dr == 0
+    c = ps.CliffordState(n, 200, 5, True)  # another seed: other coins and noise
+    c.run_program(ops, "persistent")
+    assert a.compare(c)[1] > 0
