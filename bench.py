@@ -548,12 +548,18 @@ def bench_random_n1024(args):
     words = (B + 63) // 64
     st = ps.CliffordState(n, B, w.seed, True)
     prog = ps.Program(st, w.ops)
+    # workgroups per word group (0: chosen for the device; 64 word groups
+    # cannot fill 256 compute units one workgroup each)
+    prog.set_slices(args.slices)
     prog.set_census(True)
     st.reset_clifford_state()
     prog.run(st, "persistent")
     census = prog.read_census()
     prog.set_census(False)
-    progress("n1024: census run done")
+    slices, failed = prog.slice_status(st)
+    if failed:
+        raise RuntimeError("a barrier between the slices of a word group gave up")
+    progress(f"n1024: census run done ({slices} workgroups per word group)")
     abytes = algorithmic_bytes(w.ops, n, words, census)
     steps = max(1, min(args.steps, 3))
     prog.kernel_ms()
@@ -602,7 +608,7 @@ def bench_random_n1024(args):
         "higher_is_better": True, "dtype": "u64", "data": "synthetic",
         "config": {"workload": "random_clifford_n1024", "qubits": n, "shots": B,
                    "ops": int(len(w.ops)), "gates": w.gates, "measurements": w.measurements,
-                   "seed": w.seed, "word_groups": words},
+                   "seed": w.seed, "word_groups": words, "slices": slices},
         "meas_shots_per_s": w.measurements * B / (ms_step / 1e3),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_program",
@@ -611,6 +617,19 @@ def bench_random_n1024(args):
         "census": census,
         "frame_form": frame,
     }
+    # measured HBM bytes of one k_program launch of this workload (rocprofv3
+    # request counters by size, one workgroup per word group)
+    traffic_file = os.path.join(ROOT, "profiles", "k_program_random_n1024_traffic.json")
+    if os.path.exists(traffic_file):
+        tr = json.load(open(traffic_file))
+        rf = out["roofline"]
+        rf["traffic"] = tr.get("bytes_per_launch")
+        rf["traffic_source"] = ("profiles/k_program_random_n1024_traffic.json: " +
+                                tr.get("method", "rocprofv3 counters"))
+        if tr.get("bytes_per_launch"):
+            gbs = tr["bytes_per_launch"] / (kernel_ms / 1e3) / 1e9
+            rf["traffic_gbs"] = gbs
+            rf["traffic_frac"] = gbs / HBM_PEAK_GBS
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_prefix(w, args.cpu_seconds)
     print(json.dumps(out))
@@ -626,6 +645,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--shots", type=int, default=16384)
     ap.add_argument("--threads", type=int, default=0, help="persistent kernel block size")
+    ap.add_argument("--slices", type=int, default=0,
+                    help="workgroups per 64-shot word group (0: chosen for the device)")
     ap.add_argument("--mixed", action="store_true",
                     help="measure the 256-circuit mixed batch instead of the d=9 workload")
     ap.add_argument("--mixed-count", type=int, default=256)
@@ -824,10 +845,15 @@ def main():
         "algorithmic_bytes": abytes,
         "census": census,
     }
+    # measured HBM bytes of one k_program launch of this workload: rocprofv3's
+    # L2 memory-side request counters by size, committed beside the kernel
+    # trace (counters cannot be read from inside the timed run)
     traffic_file = os.path.join(ROOT, "profiles", "k_program_surface_d9_traffic.json")
     if os.path.exists(traffic_file):
         tr = json.load(open(traffic_file))
         out["roofline"]["traffic"] = tr.get("bytes_per_launch")
+        out["roofline"]["traffic_source"] = ("profiles/k_program_surface_d9_traffic.json: " +
+                                             tr.get("method", "rocprofv3 counters"))
         # measured HBM bytes (rocprof, same kernel and workload) per launch time
         if tr.get("bytes_per_launch"):
             gbs = tr["bytes_per_launch"] / (kernel_ms / 1e3) / 1e9

@@ -265,6 +265,19 @@ int pq_frame_plan(uint64_t n, const pq_op *ops, uint64_t nops, uint64_t info[16]
                   uint64_t *z);
 /* Launch shape of the persistent kernel (0 keeps the current value). */
 int pq_program_set_launch(pq_program p, uint32_t threads, uint32_t blocks);
+/* Workgroups per 64-shot word group in persistent runs: 0 chooses (states of
+ * 512+ qubits with at most half as many word groups as the device has
+ * compute units split each group's rows between two workgroups of one
+ * cooperative launch),
+ * 1 keeps one workgroup per group, k > 1 asks for k (PQ_ERR_ARG at the run
+ * if they do not fit the device at once).  Graph-form runs and server runs
+ * use one.  No reference counterpart (launch tuning, tests). */
+int pq_program_set_slices(pq_program p, uint32_t slices);
+/* Slices the last persistent run used, and whether any of the slices'
+ * barriers gave up waiting (nonzero: the results of that run are not to be
+ * trusted; it stays set).  Waits for the state's stream.  No reference
+ * counterpart. */
+int pq_program_slice_status(pq_state s, pq_program p, uint32_t *slices, uint32_t *failed);
 /* Work census of persistent/graph runs (enable zeroes the counters):
  * out[0..3] = {random measurements, determined measurements, rows rowsummed
  * by random measurements, rowsum steps of determined measurements}, each

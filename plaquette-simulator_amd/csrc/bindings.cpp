@@ -450,6 +450,18 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
       .def("set_launch", [](Program &p, uint32_t threads, uint32_t blocks) {
         check(pq_program_set_launch(p.p, threads, blocks), "set_launch");
       })
+      .def("set_slices",
+           [](Program &p, uint32_t slices) {
+             check(pq_program_set_slices(p.p, slices), "set_slices");
+           })
+      .def(
+          "slice_status",
+          [](Program &p, State &s) {
+            uint32_t used = 1, failed = 0;
+            check(pq_program_slice_status(s.Handle(), p.p, &used, &failed), "slice_status");
+            return py::make_tuple(used, failed);
+          },
+          py::arg("state"))
       .def("set_census",
            [](Program &p, bool on) { check(pq_program_set_census(p.p, on ? 1 : 0), "set_census"); })
       .def("read_census",

@@ -275,6 +275,43 @@ __device__ uint32_t block_excl_sum(uint32_t v, ScanShared &sh, uint32_t *total) 
 //     destabilizer x_q set, reducing each step's phase across lanes.
 //  4. Record: result = coin (random) | scratch r (determined), determined = D.
 // ---------------------------------------------------------------------------
+// Workgroups that share one 64-shot word group (`slices` of them) meet at
+// barriers on a device counter of that group: each arrival adds 1, and the
+// k-th barrier waits until the counter reaches k * slices.  The launch is
+// cooperative, so all of them are resident; a wait still gives up after
+// kSliceSpinLimit polls, sets *err and lets every later barrier pass at once,
+// so no kernel can hang on it (the host reports the error).  Thread 0 releases
+// the workgroup's writes at device scope before arriving and acquires after
+// the wait; the workgroup barriers around it order the other threads.
+constexpr uint32_t kSliceSpinLimit = 1u << 22;
+struct SliceSync {
+  uint32_t slice = 0, slices = 1;
+  unsigned int *ctr = nullptr;
+  unsigned int *err = nullptr;
+  uint32_t epoch = 0;
+};
+__device__ __forceinline__ void slice_barrier(SliceSync &ss) {
+  if (ss.slices <= 1) return;
+  __syncthreads();
+  ++ss.epoch;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    atomicAdd(ss.ctr, 1u);
+    const unsigned int target = ss.epoch * ss.slices;
+    uint32_t spins = 0;
+    while (__hip_atomic_load(ss.ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(ss.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      __builtin_amdgcn_s_sleep(2);
+      if (++spins >= kSliceSpinLimit) {
+        atomicOr(ss.err, 1u);
+        break;
+      }
+    }
+    __threadfence();
+  }
+  __syncthreads();
+}
+
 struct MeasShared {
   ScanShared scan;
   uint64_t gm[64];
@@ -292,8 +329,13 @@ __device__ void measure_word(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
                              uint64_t *rec_res, uint64_t *rec_det,
                              uint64_t *s_src, uint32_t *s_list,
                              MeasShared &sh, bool force_determined = false,
-                             unsigned long long *census = nullptr) {
+                             unsigned long long *census = nullptr,
+                             SliceSync *ss = nullptr) {
   const uint32_t tid = threadIdx.x, NT = blockDim.x;
+  // with slices, every slice finds the same pivots and coins; slice 0 alone
+  // rewrites the pivot rows and runs the determined shots, and the rowsum
+  // rows are split between the slices
+  const uint32_t slice = ss ? ss->slice : 0, S = ss ? ss->slices : 1;
   const uint64_t Ru = R;
   const uint64_t *xq = T + (uint64_t)q * Ru;
 
@@ -309,15 +351,40 @@ __device__ void measure_word(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
   uint64_t rm;
   uint64_t before = block_excl_or(acc, sh.scan, &rm);
   if (force_determined) rm = 0;
-  for (uint32_t p = p0; p < p1 && !force_determined; ++p) {
-    const uint64_t v = xq[n + p];
-    const uint64_t first = v & ~before;
-    before |= v;
-    if (first) {
-      const uint32_t g = atomicAdd(&sh.ngroups, 1u);
-      sh.gp[g] = p;
-      sh.gm[g] = first;
+  if (S == 1) {
+    for (uint32_t p = p0; p < p1 && !force_determined; ++p) {
+      const uint64_t v = xq[n + p];
+      const uint64_t first = v & ~before;
+      before |= v;
+      if (first) {
+        const uint32_t g = atomicAdd(&sh.ngroups, 1u);
+        sh.gp[g] = p;
+        sh.gm[g] = first;
+      }
     }
+  } else {
+    // slices rewrite shared words group by group, so they must all walk the
+    // groups in one order: ascending pivot row, placed by a prefix count
+    uint32_t cnt = 0;
+    uint64_t seen = before;
+    for (uint32_t p = p0; p < p1 && !force_determined; ++p) {
+      const uint64_t v = xq[n + p];
+      cnt += (v & ~seen) ? 1u : 0u;
+      seen |= v;
+    }
+    uint32_t total;
+    uint32_t g = block_excl_sum(cnt, sh.scan, &total);
+    for (uint32_t p = p0; p < p1 && !force_determined; ++p) {
+      const uint64_t v = xq[n + p];
+      const uint64_t first = v & ~before;
+      before |= v;
+      if (first) {
+        sh.gp[g] = p;
+        sh.gm[g] = first;
+        ++g;
+      }
+    }
+    if (tid == 0) sh.ngroups = total;
   }
   if (tid < 64 && rm) {
     uint64_t coin;
@@ -338,16 +405,34 @@ __device__ void measure_word(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
   for (uint32_t g = 0; g < ngroups; ++g) {
     const uint32_t p = sh.gp[g];
     const uint64_t M = sh.gm[g];
-    for (uint32_t c = tid; c < C; c += NT) {
-      uint64_t *col = T + (uint64_t)c * Ru;
-      const uint64_t a = col[n + p];
-      const uint64_t b = col[p];
-      col[p] = (b & ~M) | (a & M);
-      uint64_t na = a & ~M;
-      if (c == n + q) na |= M;
-      if (c == 2 * n) na |= coin & M;
-      col[n + p] = na;
-      s_src[c] = a;
+    if (S == 1) {
+      for (uint32_t c = tid; c < C; c += NT) {
+        uint64_t *col = T + (uint64_t)c * Ru;
+        const uint64_t a = col[n + p];
+        const uint64_t b = col[p];
+        col[p] = (b & ~M) | (a & M);
+        uint64_t na = a & ~M;
+        if (c == n + q) na |= M;
+        if (c == 2 * n) na |= coin & M;
+        col[n + p] = na;
+        s_src[c] = a;
+      }
+    } else {
+      // every slice takes the source row before slice 0 rewrites it
+      for (uint32_t c = tid; c < C; c += NT) s_src[c] = T[(uint64_t)c * Ru + n + p];
+      slice_barrier(*ss);
+      if (slice == 0) {
+        for (uint32_t c = tid; c < C; c += NT) {
+          uint64_t *col = T + (uint64_t)c * Ru;
+          const uint64_t a = s_src[c];
+          const uint64_t b = col[p];
+          col[p] = (b & ~M) | (a & M);
+          uint64_t na = a & ~M;
+          if (c == n + q) na |= M;
+          if (c == 2 * n) na |= coin & M;
+          col[n + p] = na;
+        }
+      }
     }
     if (tid == 0) sh.nlist = 0;
     __syncthreads();
@@ -357,7 +442,7 @@ __device__ void measure_word(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
     const uint32_t nl = sh.nlist;
     const uint64_t rsrc = s_src[2 * n];
     uint32_t my_rows = 0;
-    for (uint32_t i = tid; i < 2 * n; i += NT) {
+    for (uint32_t i = slice * NT + tid; i < 2 * n; i += NT * S) {
       if (i == p || i == n + p) continue;
       const uint64_t mi = M & xq[i];
       if (!mi) continue;
@@ -389,13 +474,15 @@ __device__ void measure_word(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
     }
     if (census && my_rows) atomicAdd(&sh.krows, my_rows);
     __syncthreads();
+    // all slices' rowsums are in before the next pivot group or phase
+    if (S > 1) slice_barrier(*ss);
   }
 
   // ---- 3. determined shots ----
   const uint64_t D = ~rm;
   uint64_t rdet = 0;
   uint32_t nchain = 0;
-  if (D) {
+  if (D && slice == 0) {
     // ordered list of destabilizers i with x_q set in some determined shot
     uint32_t cnt = 0;
     for (uint32_t i = p0; i < p1; ++i) cnt += (xq[i] & D) ? 1u : 0u;
@@ -435,15 +522,17 @@ __device__ void measure_word(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
       rdet = racc;
     }
   }
-  if (tid == 0) {
+  if (tid == 0 && slice == 0) {
     rec_res[w] = (coin & rm) | (rdet & D);
     rec_det[w] = D;
-    if (census) {
+  }
+  if (tid == 0 && census) {
+    if (slice == 0) {
       atomicAdd(census + 0, rm ? 1ull : 0ull);
       atomicAdd(census + 1, D ? 1ull : 0ull);
-      atomicAdd(census + 2, (unsigned long long)sh.krows);
       atomicAdd(census + 3, (unsigned long long)nchain);
     }
+    atomicAdd(census + 2, (unsigned long long)sh.krows);
   }
   __syncthreads();
 }
@@ -691,7 +780,8 @@ __device__ __forceinline__ void stage_chunk(const pq::ProgramArgs &a, uint64_t w
 
 __device__ void run_gate_segment(uint64_t *__restrict__ T, uint64_t w,
                                  const pq::ProgramArgs &a, const DSegment &sg,
-                                 DInstr *s_ins, uint64_t *s_msk) {
+                                 DInstr *s_ins, uint64_t *s_msk, uint32_t slice = 0,
+                                 uint32_t slices = 1) {
   const uint64_t Ru = a.R;
   const uint32_t n = a.n;
   const uint32_t seg_end = sg.begin + sg.count;
@@ -699,7 +789,8 @@ __device__ void run_gate_segment(uint64_t *__restrict__ T, uint64_t w,
     const uint32_t cn = min(kStage, seg_end - cb);
     stage_chunk(a, w, cb, cn, s_ins, s_msk);
     __syncthreads();
-    for (uint32_t row = threadIdx.x; row < a.R; row += blockDim.x) {
+    for (uint32_t row = slice * blockDim.x + threadIdx.x; row < a.R;
+         row += blockDim.x * slices) {
       uint64_t *Tr = T + row;
       uint64_t r = Tr[(uint64_t)(2 * n) * Ru];
       uint32_t k = 0;
@@ -1027,7 +1118,8 @@ __device__ __forceinline__ uint64_t tile_run_ops(lds_u64 *tl, uint32_t TRp, uint
 // own and they are XOR-combined into r at the end of the tile.
 __device__ void run_gate_segment_tiled(uint64_t *__restrict__ T, uint64_t w,
                                        const pq::ProgramArgs &a, const DSegment &sg,
-                                       uint64_t *dyn, uint32_t TR) {
+                                       uint64_t *dyn, uint32_t TR, uint32_t slice = 0,
+                                       uint32_t slices = 1) {
   const uint64_t Ru = a.R;
   const uint32_t n = a.n;
   const uint32_t nq = sg.nq;
@@ -1052,7 +1144,7 @@ __device__ void run_gate_segment_tiled(uint64_t *__restrict__ T, uint64_t w,
   const uint32_t ncol = 2 * nq;
   const uint32_t nwords = ncol * TRp;  // tile words
   __syncthreads();
-  for (uint32_t r0 = 0; r0 < a.R; r0 += TR) {
+  for (uint32_t r0 = slice * TR; r0 < a.R; r0 += TR * slices) {
     const uint32_t TRv = min(TR, a.R - r0);
     const bool active = sub < P && ti < TRv;
     const uint32_t lr = active ? ti : TR;  // this lane's word in a column
@@ -1375,7 +1467,8 @@ struct PhaseTimes {
 template <int NREG>
 __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t mseq0,
                              uint64_t w, uint64_t *dyn, MeasShared &sh,
-                             PhaseTimes *tm) {
+                             PhaseTimes *tm, SliceSync *ss = nullptr) {
+  const uint32_t slice = ss ? ss->slice : 0, S = ss ? ss->slices : 1;
   const uint32_t n = a.n;
   uint64_t *s_src = dyn;
   uint32_t *s_list = reinterpret_cast<uint32_t *>(dyn + (2 * n + 1));
@@ -1396,10 +1489,12 @@ __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t m
       if (timing) t0 = wall_clock64();
       const uint32_t TR = tile_rows(a.lds_words, sg.nq);
       if (TR)
-        run_gate_segment_tiled(T, w, a, sg, dyn, TR);
+        run_gate_segment_tiled(T, w, a, sg, dyn, TR, slice, S);
       else
         run_gate_segment(T, w, a, sg, reinterpret_cast<DInstr *>(dyn),
-                         dyn + kStage * (sizeof(DInstr) / 8));
+                         dyn + kStage * (sizeof(DInstr) / 8), slice, S);
+      // every slice's rows are in before a measurement reads whole columns
+      if (S > 1) slice_barrier(*ss);
       if (timing) tm->gate += wall_clock64() - t0;
       continue;
     }
@@ -1442,7 +1537,8 @@ __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t m
       if (jr > 0) {
         if (wv < ndw) {
           uint64_t *slab = s_slab + (uint64_t)wv * 2 * n;
-          for (uint32_t idx = wv; idx < jr; idx += ndw) {
+          // the slices' determined waves share the batch
+          for (uint32_t idx = slice * ndw + wv; idx < jr; idx += ndw * S) {
             const DInstr I = ld_instr((cinstr_ptr)a.instrs, k + idx);
             // a record that a later measurement of the segment overwrites is
             // not written: waves of one batch are not ordered
@@ -1453,6 +1549,8 @@ __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t m
           }
         }
         __syncthreads();
+        // the scratch row and records of the batch are in before what follows
+        if (S > 1) slice_barrier(*ss);
       }
       if (timing) {
         const uint64_t t1 = wall_clock64();
@@ -1466,7 +1564,9 @@ __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t m
         const uint64_t *cp = a.coin_planes ? a.coin_planes + m * a.W : nullptr;
         uint64_t *rec = a.recs + (2ull * I.slot) * a.W;
         measure_word(T, w, n, a.R, I.qa, I.t0, key, cp, rec, rec + a.W, s_src,
-                     s_list, sh, false, a.census);
+                     s_list, sh, false, a.census, ss);
+        // slice 0's pivot rows, record and scratch row are in for every slice
+        if (S > 1) slice_barrier(*ss);
         k += jr + 1;
         if (timing) tm->gen += wall_clock64() - t0;
       } else {
@@ -1480,7 +1580,7 @@ __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t m
 // keep 3 columns per lane in registers, which leaves room for 768-thread
 // workgroups (12 waves); larger states keep 8 per lane (n <= 512) in
 // 512-thread workgroups.
-template <int NREG, int MAXT>
+template <int NREG, int MAXT, bool SLICED>
 __global__ __launch_bounds__(MAXT) void k_program_t(pq::ProgramArgs a,
                                                    const ProgParams *__restrict__ pp) {
   extern __shared__ uint64_t dyn[];
@@ -1499,8 +1599,21 @@ __global__ __launch_bounds__(MAXT) void k_program_t(pq::ProgramArgs a,
   }
   PhaseTimes times;
   PhaseTimes *tm = (census != nullptr && threadIdx.x == 0) ? &times : nullptr;
-  for (uint64_t wi = blockIdx.x; wi < a.nwords; wi += gridDim.x)
-    program_word<NREG>(la, seed, mseq0, a.word0 + wi, dyn, sh, tm);
+  if constexpr (SLICED) {
+    // cooperative launch: one workgroup per (word group, slice), all resident
+    const uint64_t wi = blockIdx.x / a.slices;
+    if (wi < a.nwords) {
+      SliceSync ss;
+      ss.slice = blockIdx.x % a.slices;
+      ss.slices = a.slices;
+      ss.ctr = a.sync + wi;
+      ss.err = a.sync_err;
+      program_word<NREG>(la, seed, mseq0, a.word0 + wi, dyn, sh, tm, &ss);
+    }
+  } else {
+    for (uint64_t wi = blockIdx.x; wi < a.nwords; wi += gridDim.x)
+      program_word<NREG>(la, seed, mseq0, a.word0 + wi, dyn, sh, tm);
+  }
   if (census) {
     __syncthreads();
     if (threadIdx.x < 4) atomicAdd(census + threadIdx.x, s_census[threadIdx.x]);
@@ -2051,6 +2164,18 @@ __global__ __launch_bounds__(256) void k_count_diff(const uint64_t *__restrict__
   if (lane_id() == 0 && c) atomicAdd(out, c);
 }
 
+__global__ __launch_bounds__(256) void k_zero_u32(unsigned int *p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+hipError_t launch_zero_u32(hipStream_t s, unsigned int *p, uint32_t n) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_u32, dim3(std::min<uint32_t>((n + 255) / 256, 64)), dim3(256), 0,
+                     s, p, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero_word(hipStream_t s, unsigned long long *p) {
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, s, p);
   return hipGetLastError();
@@ -2091,19 +2216,57 @@ hipError_t launch_record_to_int(hipStream_t s, const uint64_t *rec, uint64_t B,
   return hipGetLastError();
 }
 
+// The persistent kernel build for n qubits, one workgroup per word group or
+// sliced word groups.
+static const void *program_kernel(uint32_t n, bool sliced) {
+  if (n <= kSmallQubits)
+    return sliced ? (const void *)k_program_t<kSmallRegs, kSmallThreads, true>
+                  : (const void *)k_program_t<kSmallRegs, kSmallThreads, false>;
+  return sliced ? (const void *)k_program_t<kLargeRegs, kLargeThreads, true>
+                : (const void *)k_program_t<kLargeRegs, kLargeThreads, false>;
+}
+
 hipError_t launch_program(hipStream_t s, const ProgramArgs &a, uint32_t threads,
                           uint32_t blocks, const void *params) {
   if (!a.nwords) return hipSuccess;
   const size_t lds = a.lds_words * 8;
+  if (a.slices > 1) {
+    // one workgroup per (word group, slice), all resident at once: the
+    // slices wait for one another
+    const void *fn = program_kernel(a.n, true);
+    const uint32_t tb = std::min<uint32_t>(threads, program_max_threads(a.n));
+    ProgramArgs la = a;
+    const ProgParams *pp = (const ProgParams *)params;
+    void *args[] = {&la, &pp};
+    return hipLaunchCooperativeKernel(fn, dim3((uint32_t)(a.nwords * a.slices)), dim3(tb), args,
+                                      (unsigned int)lds, s);
+  }
   if (a.n <= kSmallQubits)
-    hipLaunchKernelGGL((k_program_t<kSmallRegs, kSmallThreads>), dim3(blocks),
+    hipLaunchKernelGGL((k_program_t<kSmallRegs, kSmallThreads, false>), dim3(blocks),
                        dim3(std::min<uint32_t>(threads, kSmallThreads)), lds, s, a,
                        (const ProgParams *)params);
   else
-    hipLaunchKernelGGL((k_program_t<kLargeRegs, kLargeThreads>), dim3(blocks),
+    hipLaunchKernelGGL((k_program_t<kLargeRegs, kLargeThreads, false>), dim3(blocks),
                        dim3(std::min<uint32_t>(threads, kLargeThreads)), lds, s, a,
                        (const ProgParams *)params);
   return hipGetLastError();
+}
+
+bool program_slices_fit(uint32_t n, uint64_t nwords, uint32_t slices, uint32_t threads,
+                        uint64_t lds_bytes) {
+  const void *fn = program_kernel(n, true);
+  int dev = 0, cus = 0, per_cu = 0, coop = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return false;
+  if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
+      !coop)
+    return false;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return false;
+  const uint32_t tb = std::min<uint32_t>(threads, program_max_threads(n));
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)tb, (size_t)lds_bytes) !=
+      hipSuccess)
+    return false;
+  return (uint64_t)per_cu * (uint64_t)cus >= nwords * slices;
 }
 
 uint32_t program_max_threads(uint32_t n) {
@@ -2205,10 +2368,9 @@ hipError_t launch_set_params(hipStream_t s, void *params, uint64_t seed,
 
 hipError_t prepare_kernels(uint32_t n) {
   hipError_t e = ensure_lds_attr((const void *)k_measure, measure_lds_bytes(n));
-  if (e == hipSuccess)
-    e = ensure_lds_attr((const void *)k_program_t<kSmallRegs, kSmallThreads>, kProgramLdsBudget);
-  if (e == hipSuccess)
-    e = ensure_lds_attr((const void *)k_program_t<kLargeRegs, kLargeThreads>, kProgramLdsBudget);
+  for (uint32_t nn : {1u, kSmallQubits + 1})
+    for (bool sliced : {false, true})
+      if (e == hipSuccess) e = ensure_lds_attr(program_kernel(nn, sliced), kProgramLdsBudget);
   return e;
 }
 

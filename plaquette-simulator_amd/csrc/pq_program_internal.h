@@ -51,6 +51,12 @@ struct pq_program_s {
   // stream waits for it, since the noise planes and side buffer are shared
   hipEvent_t done_ev = nullptr;
   hipStream_t last_stream = nullptr;
+  // workgroups per word group in persistent runs: requested (0: chosen from
+  // the state size and the device), last used, and the slices' barrier
+  // counters ([W] words) and give-up flag (one word after them)
+  uint32_t slices_req = 0, slices_used = 1;
+  unsigned int *d_sync = nullptr;
+  uint64_t sync_words = 0;
   void drop_graph();
   ~pq_program_s();
 };

@@ -13,6 +13,25 @@
   python profiling.py traffic <fetch csv dir> <write csv dir> <out.json> <out.md>
       Per-dispatch HBM traffic of k_program, each counter scaled by what the
       same counter reads for the calibration copy of known size.
+
+  python profiling.py sizes <read-size csv dir> <write-size csv dir> <out.json> <out.md>
+      HBM traffic of k_program from the request counters by size, no
+      calibration: reads 32 x RDREQ_32B + 64 x RDREQ_64B + 128 x RDREQ_128B,
+      writes 64 x WRREQ_64B + 32 x (WRREQ - WRREQ_64B) (TCC_EA0_*, summed
+      over channels), beside the same for the calibration copy.
+
+  python profiling.py probe-n1024 [slices ...]
+      The n=1024 random circuit (4096 shots, 100000 ops): a device-to-device
+      copy of the whole tableau (calibration), then one persistent run per
+      workgroups-per-word-group count (default 1 2).
+
+  python profiling.py dispatches <out.md> <csv dir> [<csv dir> ...]
+      Every counter of every `--pmc` pass for each k_program dispatch and the
+      calibration copy, side by side.
+
+  python profiling.py pmc <out.md> <csv dir> [<csv dir> ...]
+      Every counter of every `--pmc` pass, for the last k_program dispatch and
+      the calibration copy, side by side.
 """
 import csv
 import glob
@@ -48,6 +67,61 @@ def probe(shots=16384):
     words = (w.batch_size + 63) // 64
     copy_bytes = words * (2 * n + 1) ** 2 * 8
     print(json.dumps({"workload": w.name, "copy_bytes": copy_bytes}))
+
+
+def probe_n1024(slices=(1, 2)):
+    """The n=1024 random circuit (4096 shots, 100000 ops), one persistent run
+    per workgroups-per-word-group count in `slices`, after a device-to-device
+    copy of the whole 2.15 GB tableau (the calibration dispatch)."""
+    sys.path.insert(0, os.path.dirname(HERE))
+    import plaquette_simulator as ps
+    from plaquette_simulator import workloads as W
+    w = W.config_random_n1024()
+    n, B = w.num_qubits, w.batch_size
+    st = ps.CliffordState(n, B, w.seed, True)
+    other = ps.CliffordState(n, B, w.seed, False)
+    other.synchronize()
+    other.device_to_device(st)              # calibration copy
+    other.synchronize()
+    del other
+    prog = ps.Program(st, w.ops)
+    for s in slices:
+        prog.set_slices(s)
+        st.reset_clifford_state()
+        prog.run(st, "persistent")
+        st.synchronize()
+        used, failed = prog.slice_status(st)
+        print(json.dumps({"workload": w.name, "slices": used, "barrier_failures": failed}),
+              flush=True)
+    words = (B + 63) // 64
+    print(json.dumps({"probe": "done", "copy_bytes": words * (2 * n + 1) ** 2 * 8}), flush=True)
+
+
+def dispatches(out, *dirs):
+    """Every counter of every `--pmc` pass for each k_program dispatch (kernel
+    build and grid size name it) and the calibration copy."""
+    table = {}
+    order = []
+    for d in dirs:
+        rs = _counter_rows(d)
+        for r in rs:
+            k = r["Kernel_Name"]
+            if not ("k_program" in k and "noise" not in k) and "copyBuffer" not in k:
+                continue
+            who = ("calibration copy" if "copyBuffer" in k else
+                   "k_program %s grid %s" % ("sliced" if "true>" in k else "one group per workgroup",
+                                            r["Grid_Size"]))
+            if who not in table:
+                table[who] = {}
+                order.append(who)
+            table[who][r["Counter_Name"]] = table[who].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    names = sorted({c for v in table.values() for c in v})
+    md = ["| counter | " + " | ".join(order) + " |", "|---" * (len(order) + 1) + "|"]
+    for c in names:
+        md.append("| %s | " % c + " | ".join("%.6g" % table[w].get(c, float("nan")) for w in order) + " |")
+    with open(out, "w") as f:
+        f.write("\n".join(md) + "\n")
+    print("\n".join(md))
 
 
 def _find(d, pattern):
@@ -136,6 +210,52 @@ def traffic(fetch_dir, write_dir, out_json, out_md, copy_bytes=None):
     print("\n".join(md))
 
 
+def sizes(rd_dir, wr_dir, out_json, out_md):
+    def last(d, name):
+        disp = _per_dispatch(_counter_rows(d), name)
+        prog = [v for (i, k), v in sorted(disp.items()) if "k_program" in k and "noise" not in k]
+        copy = [v for (i, k), v in sorted(disp.items()) if "copyBuffer" in k]
+        return (prog[-1] if prog else 0.0), (copy[-1] if copy else 0.0)
+    res = {}
+    for who, idx in (("k_program", 0), ("calibration_copy", 1)):
+        rd = {b: last(rd_dir, "TCC_EA0_RDREQ_%sB_sum" % b)[idx] for b in (32, 64, 128)}
+        wr = last(wr_dir, "TCC_EA0_WRREQ_sum")[idx]
+        wr64 = last(wr_dir, "TCC_EA0_WRREQ_64B_sum")[idx]
+        read = 32 * rd[32] + 64 * rd[64] + 128 * rd[128]
+        write = 64 * wr64 + 32 * (wr - wr64)
+        res[who] = {"rdreq_32b": rd[32], "rdreq_64b": rd[64], "rdreq_128b": rd[128],
+                    "wrreq": wr, "wrreq_64b": wr64, "read_bytes": read, "write_bytes": write,
+                    "bytes": read + write}
+    with open(out_json, "w") as f:
+        json.dump(res, f, indent=1)
+    md = ["| dispatch | RDREQ 32B | RDREQ 64B | RDREQ 128B | WRREQ | WRREQ 64B | read bytes | write bytes | total |",
+          "|---|---|---|---|---|---|---|---|---|"]
+    for who, r in res.items():
+        md.append("| %s | %.6g | %.6g | %.6g | %.6g | %.6g | %.4g | %.4g | %.4g |" % (
+            who, r["rdreq_32b"], r["rdreq_64b"], r["rdreq_128b"], r["wrreq"], r["wrreq_64b"],
+            r["read_bytes"], r["write_bytes"], r["bytes"]))
+    with open(out_md, "w") as f:
+        f.write("\n".join(md) + "\n")
+    print("\n".join(md))
+
+
+def pmc(out, *dirs):
+    rows = []
+    for d in dirs:
+        rs = _counter_rows(d)
+        names = sorted({r["Counter_Name"] for r in rs})
+        for name in names:
+            disp = _per_dispatch(rs, name)
+            prog = [v for (i, k), v in sorted(disp.items()) if "k_program" in k and "noise" not in k]
+            copy = [v for (i, k), v in sorted(disp.items()) if "copyBuffer" in k]
+            rows.append((name, prog[-1] if prog else float("nan"), copy[-1] if copy else float("nan")))
+    md = ["| counter | k_program (last dispatch) | calibration copy |", "|---|---|---|"]
+    md += ["| %s | %.6g | %.6g |" % r for r in rows]
+    with open(out, "w") as f:
+        f.write("\n".join(md) + "\n")
+    print("\n".join(md))
+
+
 if __name__ == "__main__":
     cmd = sys.argv[1] if len(sys.argv) > 1 else "probe"
     if cmd == "probe":
@@ -144,5 +264,13 @@ if __name__ == "__main__":
         kernels(sys.argv[2], sys.argv[3])
     elif cmd == "traffic":
         traffic(*sys.argv[2:6])
+    elif cmd == "sizes":
+        sizes(*sys.argv[2:6])
+    elif cmd == "pmc":
+        pmc(sys.argv[2], *sys.argv[3:])
+    elif cmd == "probe-n1024":
+        probe_n1024(tuple(int(a) for a in sys.argv[2:]) or (1, 2))
+    elif cmd == "dispatches":
+        dispatches(sys.argv[2], *sys.argv[3:])
     else:
         raise SystemExit(__doc__)
