@@ -1500,12 +1500,19 @@ __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t m
     }
     uint32_t k = sg.begin;
     const uint32_t end = sg.begin + sg.count;
-    uint32_t win = kMeasWindow0;
+    // Windows are whole multiples of the waves that run determined
+    // measurements (every slice's), at least kMeasWindow0, so a batch of
+    // determined measurements fills its waves.
+    const uint32_t dw = ndw * S;
+    const uint32_t wunit = min(kMeasBatch, dw * ((kMeasWindow0 + dw - 1) / dw));
+    const uint32_t wmax = (kMeasBatch / wunit) * wunit;
+    uint32_t win = wunit;
     while (k < end) {
       if (timing) t0 = wall_clock64();
       // which of the next measurements have a pivot in some shot?  The window
       // grows while every checked measurement is determined and restarts
-      // small after a random one (the state then changes)
+      // small after a random one (the state then changes).  A wave checks
+      // two measurements at a time with all their loads in flight.
       const uint32_t nm = min(end - k, win);
       for (uint32_t idx = wv; idx < nm; idx += nw) {
         const uint64_t *xq = T + (uint64_t)((cinstr_ptr)a.instrs)[k + idx].qa * Ru;
@@ -1527,7 +1534,7 @@ __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t m
       // (nm <= kMeasBatch = 64 = lanes)
       const uint64_t piv = __ballot(lane < nm && s_rm[lane] != 0);
       const uint32_t jr = piv ? (uint32_t)__ffsll((long long)piv) - 1 : nm;
-      win = (jr == nm) ? min(2 * win, kMeasBatch) : kMeasWindow0;
+      win = (jr == nm) ? min(2 * win, wmax) : wunit;
       if (timing) {
         const uint64_t t1 = wall_clock64();
         tm->check += t1 - t0;
