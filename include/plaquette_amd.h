@@ -273,6 +273,12 @@ int pq_program_set_launch(pq_program p, uint32_t threads, uint32_t blocks);
  * if they do not fit the device at once).  Graph-form runs and server runs
  * use one.  No reference counterpart (launch tuning, tests). */
 int pq_program_set_slices(pq_program p, uint32_t slices);
+/* How many times a barrier between slices polls before it gives up (0: the
+ * default, 2^22).  A small value makes a slice that waits for a slower
+ * partner give up: the run still finishes, every later barrier of it passes
+ * at once and pq_program_slice_status reports the failure.  For tests of
+ * that path; no reference counterpart. */
+int pq_program_set_slice_wait(pq_program p, uint32_t polls);
 /* Slices the last persistent run used, and whether any of the slices'
  * barriers gave up waiting (nonzero: the results of that run are not to be
  * trusted; it stays set).  Waits for the state's stream.  No reference

@@ -454,6 +454,10 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
            [](Program &p, uint32_t slices) {
              check(pq_program_set_slices(p.p, slices), "set_slices");
            })
+      .def("set_slice_wait",
+           [](Program &p, uint32_t polls) {
+             check(pq_program_set_slice_wait(p.p, polls), "set_slice_wait");
+           })
       .def(
           "slice_status",
           [](Program &p, State &s) {

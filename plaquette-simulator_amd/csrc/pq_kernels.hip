@@ -279,7 +279,8 @@ __device__ uint32_t block_excl_sum(uint32_t v, ScanShared &sh, uint32_t *total) 
 // barriers on a device counter of that group: each arrival adds 1, and the
 // k-th barrier waits until the counter reaches k * slices.  The launch is
 // cooperative, so all of them are resident; a wait still gives up after
-// kSliceSpinLimit polls, sets *err and lets every later barrier pass at once,
+// kSliceSpinLimit polls (or the launch's own limit), sets *err and lets
+// every later barrier pass at once,
 // so no kernel can hang on it (the host reports the error).  Thread 0 releases
 // the workgroup's writes at device scope before arriving and acquires after
 // the wait; the workgroup barriers around it order the other threads.
@@ -289,6 +290,7 @@ struct SliceSync {
   unsigned int *ctr = nullptr;
   unsigned int *err = nullptr;
   uint32_t epoch = 0;
+  uint32_t spin_limit = kSliceSpinLimit;
 };
 __device__ __forceinline__ void slice_barrier(SliceSync &ss) {
   if (ss.slices <= 1) return;
@@ -302,7 +304,7 @@ __device__ __forceinline__ void slice_barrier(SliceSync &ss) {
     while (__hip_atomic_load(ss.ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       if (__hip_atomic_load(ss.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
       __builtin_amdgcn_s_sleep(2);
-      if (++spins >= kSliceSpinLimit) {
+      if (++spins >= ss.spin_limit) {
         atomicOr(ss.err, 1u);
         break;
       }
@@ -1615,6 +1617,7 @@ __global__ __launch_bounds__(MAXT) void k_program_t(pq::ProgramArgs a,
       ss.slices = a.slices;
       ss.ctr = a.sync + wi;
       ss.err = a.sync_err;
+      if (a.sync_spin) ss.spin_limit = a.sync_spin;
       program_word<NREG>(la, seed, mseq0, a.word0 + wi, dyn, sh, tm, &ss);
     }
   } else {

@@ -519,6 +519,7 @@ static int enqueue_persistent(pq_state s, pq_program p, bool timed, bool allow_s
     a.slices = S;
     a.sync = p->d_sync;
     a.sync_err = p->d_sync + a.nwords;
+    a.sync_spin = p->slice_wait;
     // barrier counters start at zero; the give-up flag keeps its value
     // until read
     HIPCHK(launch_zero_u32(s->stream, a.sync, (uint32_t)a.nwords));
@@ -629,6 +630,12 @@ extern "C" int pq_program_set_slices(pq_program p, uint32_t slices) {
   if (!p) return pq_set_error(PQ_ERR_ARG, "null program");
   if (slices > 1024) return pq_set_error(PQ_ERR_ARG, "slices must be at most 1024");
   p->slices_req = slices;
+  return PQ_OK;
+}
+
+extern "C" int pq_program_set_slice_wait(pq_program p, uint32_t polls) {
+  if (!p) return pq_set_error(PQ_ERR_ARG, "null program");
+  p->slice_wait = polls;
   return PQ_OK;
 }
 

@@ -55,6 +55,7 @@ struct pq_program_s {
   // the state size and the device), last used, and the slices' barrier
   // counters ([W] words) and give-up flag (one word after them)
   uint32_t slices_req = 0, slices_used = 1;
+  uint32_t slice_wait = 0;  // polls before a barrier gives up (0: default)
   unsigned int *d_sync = nullptr;
   uint64_t sync_words = 0;
   void drop_graph();

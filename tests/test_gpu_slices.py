@@ -159,3 +159,23 @@ def test_slices_that_do_not_fit_are_refused(ps):
     prog.set_slices(1000)  # 64 groups x 1000 workgroups cannot be resident
     with pytest.raises(RuntimeError):
         prog.run(st, "persistent")
+
+
+def test_barrier_give_up_is_reported_and_the_run_ends(ps):
+    # a one-poll wait makes a slice that arrives first give up at once: the
+    # run must still end, and the status must say its results are not sound
+    n, B, seed = 520, 128, 5
+    ops = random_ops(ps, n, 2000, 13, 0.1)
+    say(f"[case] random n520 B{B}: slices 2, barriers poll once")
+    st = ps.CliffordState(n, B, seed, True)
+    prog = ps.Program(st, ops)
+    prog.set_slices(2)
+    prog.set_slice_wait(1)
+    prog.run(st, "persistent")
+    st.synchronize()
+    used, failed = prog.slice_status(st)
+    assert used == 2 and failed == 1
+    # a fresh program with the default wait on a fresh state is sound again
+    a, used, failed = run_sliced(ps, n, B, seed, ops, 2)
+    assert (used, failed) == (2, 0)
+    check_against_oracle(a, n, B, seed, ops)
