@@ -336,13 +336,19 @@ int pq_batch_wait(pq_batch b);
  * (40 KB..158 KB; it sets workgroups per CU and parallel determined waves). */
 int pq_batch_set_launch(pq_batch b, uint32_t threads, uint32_t blocks,
                         uint64_t lds_budget);
-/* Times candidate launch shapes on the batch and keeps the fastest (cached
- * per batch shape).  Members are reset to |0...0> afterwards with their
+/* Times candidate launch shapes on the batch and keeps the fastest, then the
+ * word tile at that shape (cached per batch shape).  Members are reset to |0...0> afterwards with their
  * sequence numbers restored. */
 int pq_batch_autotune(pq_batch b, int reps, uint32_t *threads, uint64_t *lds_budget,
                       float *best_ms);
 int pq_batch_info(pq_batch b, uint64_t *members, uint64_t *items, uint32_t *threads,
                   uint32_t *blocks, uint64_t *lds_budget);
+/* Word groups per work item of persistent/graph batch runs (1..64; an item
+ * then covers that many consecutive 64-shot groups of one member, fewer at
+ * its end), and its current value.  The autotuner also picks it.  Launch
+ * tuning; no reference counterpart. */
+int pq_batch_set_word_tile(pq_batch b, uint32_t words);
+int pq_batch_word_tile(pq_batch b, uint32_t *words);
 /* HIP-event time of the last run (enable first). */
 int pq_batch_set_timing(pq_batch b, int enable);
 int pq_batch_last_ms(pq_batch b, float *ms);

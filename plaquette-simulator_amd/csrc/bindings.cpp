@@ -523,9 +523,12 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
             uint64_t l = 0;
             float ms = 0;
             check(pq_batch_autotune(b.b, reps, &t, &l, &ms), "Batch.autotune");
+            uint32_t wt = 1;
+            check(pq_batch_word_tile(b.b, &wt), "Batch.autotune");
             py::dict d;
             d["threads"] = t;
             d["lds_budget"] = l;
+            d["word_tile"] = wt;
             d["best_ms"] = ms;
             return d;
           },
@@ -535,13 +538,20 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
              uint64_t mem = 0, items = 0, lds = 0;
              uint32_t t = 0, bl = 0;
              check(pq_batch_info(b.b, &mem, &items, &t, &bl, &lds), "Batch.info");
+             uint32_t wt = 1;
+             check(pq_batch_word_tile(b.b, &wt), "Batch.info");
              py::dict d;
              d["members"] = mem;
              d["items"] = items;
              d["threads"] = t;
              d["blocks"] = bl;
              d["lds_budget"] = lds;
+             d["word_tile"] = wt;
              return d;
+           })
+      .def("set_word_tile",
+           [](Batch &b, uint32_t words) {
+             check(pq_batch_set_word_tile(b.b, words), "Batch.set_word_tile");
            })
       .def("set_timing",
            [](Batch &b, bool on) { check(pq_batch_set_timing(b.b, on ? 1 : 0), "Batch.set_timing"); })

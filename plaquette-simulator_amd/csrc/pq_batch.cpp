@@ -46,6 +46,11 @@ struct pq_batch_s {
   unsigned long long *d_counter = nullptr;
   uint64_t nitems = 0;
   std::vector<uint64_t> h_items;   // the work list, host copy
+  // persistent form: word groups per work item; with more than one, the
+  // items are every wtile-th entry of h_items (d_titems, ntitems)
+  uint32_t wtile = 1, titems_tile = 0;
+  uint64_t *d_titems = nullptr;
+  uint64_t ntitems = 0;
   // frame form: the work list split by LDS class (members whose plans let 4
   // groups share a CU, then the rest), one launch each
   uint64_t *d_fitems = nullptr;
@@ -80,6 +85,7 @@ pq_batch_s::~pq_batch_s() {
   if (d_fargs) (void)hipFree(d_fargs);
   if (h_fargs) (void)hipHostFree(h_fargs);
   if (d_items) (void)hipFree(d_items);
+  if (d_titems) (void)hipFree(d_titems);
   if (d_fitems) (void)hipFree(d_fitems);
   for (hipEvent_t e : member_ev) (void)hipEventDestroy(e);
   if (d_counter) (void)hipFree(d_counter);
@@ -185,6 +191,9 @@ static int build(pq_batch b) {
   }
   if (b->d_items) HIPCHK(hipFree(b->d_items));
   b->d_items = nullptr;
+  if (b->d_titems) HIPCHK(hipFree(b->d_titems));
+  b->d_titems = nullptr;
+  b->titems_tile = 0;
   if (b->d_circ) HIPCHK(hipFree(b->d_circ));
   b->d_circ = nullptr;
   if (b->h_circ) HIPCHK(hipHostFree(b->h_circ));
@@ -228,6 +237,25 @@ static size_t launch_lds(pq_batch b) {
   return lds;
 }
 
+// The persistent form's work list for word tiles of b->wtile groups: the
+// items of h_items whose word group is a multiple of the tile (each
+// circuit's groups are consecutive there, from 0).
+static int tiled_items(pq_batch b) {
+  if (b->wtile <= 1 || b->titems_tile == b->wtile) return PQ_OK;
+  std::vector<uint64_t> list;
+  for (uint64_t it : b->h_items)
+    if ((it & ((1ull << 40) - 1)) % b->wtile == 0) list.push_back(it);
+  if (b->d_titems) HIPCHK(hipFree(b->d_titems));
+  b->d_titems = nullptr;
+  if (!list.empty()) {
+    HIPCHK(hipMalloc(&b->d_titems, list.size() * 8));
+    HIPCHK(hipMemcpy(b->d_titems, list.data(), list.size() * 8, hipMemcpyHostToDevice));
+  }
+  b->ntitems = list.size();
+  b->titems_tile = b->wtile;
+  return PQ_OK;
+}
+
 static int enqueue(pq_batch b) {
   const uint32_t nc = (uint32_t)b->members.size();
   uint32_t noise_blocks = 0;
@@ -235,9 +263,11 @@ static int enqueue(pq_batch b) {
     if (b->h_circ[c].nnoise) noise_blocks = b->noise_blocks;
   uint32_t max_n = 0;
   for (uint32_t c = 0; c < nc; ++c) max_n = std::max<uint32_t>(max_n, b->h_circ[c].a.n);
-  return launch_batch(b->stream, b->d_circ, nc, b->d_items, b->nitems, b->d_counter,
-                      b->threads, launch_blocks(b), launch_lds(b), noise_blocks,
-                      max_n) == hipSuccess
+  const bool tiled = b->wtile > 1;
+  return launch_batch(b->stream, b->d_circ, nc, tiled ? b->d_titems : b->d_items,
+                      tiled ? b->ntitems : b->nitems, b->d_counter, b->threads,
+                      launch_blocks(b), launch_lds(b), noise_blocks, max_n,
+                      b->wtile) == hipSuccess
              ? PQ_OK
              : pq_set_error(PQ_ERR_HIP, "batch launch failed");
 }
@@ -411,6 +441,10 @@ extern "C" int pq_batch_run(pq_batch b, int form) {
     s->noise_seq += p->nnoise;
     gens[c] = s->arena_gen;
   }
+  if (form != PQ_RUN_FRAME) {
+    int rc = tiled_items(b);  // allocates: before any capture
+    if (rc) return rc;
+  }
   const uint64_t lds_words = launch_lds(b) / 8;
   for (uint32_t c = 0; c < nc; ++c) b->h_circ[c].a.lds_words = lds_words;
   HIPCHK(hipMemcpyAsync(b->d_circ, b->h_circ, nc * sizeof(BatchCircuit),
@@ -461,6 +495,20 @@ extern "C" int pq_batch_last_ms(pq_batch b, float *ms) {
   return PQ_OK;
 }
 
+extern "C" int pq_batch_set_word_tile(pq_batch b, uint32_t words) {
+  if (!b) return pq_set_error(PQ_ERR_ARG, "null batch");
+  if (words < 1 || words > 64) return pq_set_error(PQ_ERR_ARG, "word tile must be 1..64 groups");
+  if (words != b->wtile) b->drop_graph();
+  b->wtile = words;
+  return PQ_OK;
+}
+
+extern "C" int pq_batch_word_tile(pq_batch b, uint32_t *words) {
+  if (!b || !words) return pq_set_error(PQ_ERR_ARG, "null argument");
+  *words = b->wtile;
+  return PQ_OK;
+}
+
 extern "C" int pq_batch_info(pq_batch b, uint64_t *members, uint64_t *items,
                              uint32_t *threads, uint32_t *blocks, uint64_t *lds_budget) {
   if (!b) return pq_set_error(PQ_ERR_ARG, "null batch");
@@ -479,7 +527,8 @@ extern "C" int pq_batch_info(pq_batch b, uint64_t *members, uint64_t *items,
 // Autotuner: times each launch shape (block size x LDS budget, the latter
 // setting how many workgroups share a CU and how many waves run fully
 // determined measurements in parallel) on the batch itself and keeps the
-// fastest.  Trial runs are real runs: afterwards every member is reset to
+// fastest, then the word tile (1, 2 or 4 word groups per work item) at that
+// shape.  Trial runs are real runs: afterwards every member is reset to
 // |0...0> and its coin/noise sequence numbers are restored.  Choices are
 // cached per batch shape (member qubit counts, word-group counts and program
 // sizes) for the life of the process.
@@ -488,7 +537,12 @@ extern "C" int pq_batch_autotune(pq_batch b, int reps, uint32_t *threads_out,
   if (!b) return pq_set_error(PQ_ERR_ARG, "null batch");
   if (b->members.empty()) return PQ_OK;
   static std::mutex mu;
-  static std::map<std::vector<uint64_t>, std::pair<uint32_t, uint64_t>> cache;
+  struct Choice {
+    uint32_t threads;
+    uint64_t lds;
+    uint32_t wtile;
+  };
+  static std::map<std::vector<uint64_t>, Choice> cache;
   std::vector<uint64_t> key;
   for (auto &m : b->members) {
     key.push_back(m.first->n);
@@ -499,9 +553,10 @@ extern "C" int pq_batch_autotune(pq_batch b, int reps, uint32_t *threads_out,
     std::lock_guard<std::mutex> g(mu);
     auto it = cache.find(key);
     if (it != cache.end()) {
-      int rc = pq_batch_set_launch(b, it->second.first, 0, it->second.second);
-      if (threads_out) *threads_out = it->second.first;
-      if (lds_out) *lds_out = it->second.second;
+      int rc = pq_batch_set_launch(b, it->second.threads, 0, it->second.lds);
+      if (rc == PQ_OK) rc = pq_batch_set_word_tile(b, it->second.wtile);
+      if (threads_out) *threads_out = it->second.threads;
+      if (lds_out) *lds_out = it->second.lds;
       if (ms_out) *ms_out = -1.0f;
       return rc;
     }
@@ -511,10 +566,26 @@ extern "C" int pq_batch_autotune(pq_batch b, int reps, uint32_t *threads_out,
   const uint32_t cand_threads[] = {256, 512, 768};
   const uint64_t cand_lds[] = {kProgramLdsBytes, 72 * 1024, 48 * 1024};
   float best = 1e30f;
-  uint32_t bt = b->threads;
+  uint32_t bt = b->threads, bw = 1;
   uint64_t bl = b->lds_budget;
   const bool was_timed = b->timed;
   b->timed = true;
+  // average HIP-event time of `reps` runs after one warm-up run
+  auto time_runs = [&](float *avg) -> int {
+    float tot = 0;
+    for (int r = 0; r < std::max(1, reps) + 1; ++r) {
+      int rc = pq_batch_run(b, PQ_RUN_PERSISTENT);
+      if (rc) return rc;
+      float ms = 0;
+      rc = pq_batch_last_ms(b, &ms);
+      if (rc) return rc;
+      if (r > 0) tot += ms;  // first run warms up
+    }
+    *avg = tot / std::max(1, reps);
+    return PQ_OK;
+  };
+  int rc = pq_batch_set_word_tile(b, 1);
+  if (rc) return rc;
   for (uint32_t t : cand_threads) {
     for (uint64_t l : cand_lds) {
       // every member must fit: base area + one determined-wave slab
@@ -522,23 +593,26 @@ extern "C" int pq_batch_autotune(pq_batch b, int reps, uint32_t *threads_out,
       for (auto &m : b->members)
         if (program_lds_bytes((uint32_t)m.first->n, 1) > l) fits = false;
       if (!fits) continue;
-      int rc = pq_batch_set_launch(b, t, 0, l);
+      rc = pq_batch_set_launch(b, t, 0, l);
       if (rc) return rc;
-      float tot = 0;
-      for (int r = 0; r < std::max(1, reps) + 1; ++r) {
-        rc = pq_batch_run(b, PQ_RUN_PERSISTENT);
-        if (rc) return rc;
-        float ms = 0;
-        rc = pq_batch_last_ms(b, &ms);
-        if (rc) return rc;
-        if (r > 0) tot += ms;  // first run warms up
-      }
-      const float avg = tot / std::max(1, reps);
+      float avg = 0;
+      if ((rc = time_runs(&avg))) return rc;
       if (avg < best) {
         best = avg;
         bt = t;
         bl = l;
       }
+    }
+  }
+  // then the word tile (word groups per work item) at the best shape
+  if ((rc = pq_batch_set_launch(b, bt, 0, bl))) return rc;
+  for (uint32_t wt : {2u, 4u}) {
+    if ((rc = pq_batch_set_word_tile(b, wt))) return rc;
+    float avg = 0;
+    if ((rc = time_runs(&avg))) return rc;
+    if (avg < best) {
+      best = avg;
+      bw = wt;
     }
   }
   b->timed = was_timed;
@@ -550,11 +624,12 @@ extern "C" int pq_batch_autotune(pq_batch b, int reps, uint32_t *threads_out,
     s->meas_seq = saved[c].first;
     s->noise_seq = saved[c].second;
   }
-  int rc = pq_batch_set_launch(b, bt, 0, bl);
+  rc = pq_batch_set_launch(b, bt, 0, bl);
+  if (rc == PQ_OK) rc = pq_batch_set_word_tile(b, bw);
   if (rc) return rc;
   {
     std::lock_guard<std::mutex> g(mu);
-    cache[key] = {bt, bl};
+    cache[key] = {bt, bl, bw};
   }
   if (threads_out) *threads_out = bt;
   if (lds_out) *lds_out = bl;

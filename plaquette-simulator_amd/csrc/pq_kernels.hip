@@ -1643,7 +1643,7 @@ __global__ __launch_bounds__(MAXT) void k_program_t(pq::ProgramArgs a,
 template <int NREG, int MAXT>
 __global__ __launch_bounds__(MAXT) void k_batch_t(
     const pq::BatchCircuit *__restrict__ circ, const uint64_t *__restrict__ items,
-    uint64_t nitems, unsigned long long *counter) {
+    uint64_t nitems, unsigned long long *counter, uint32_t wtile) {
   extern __shared__ uint64_t dyn[];
   __shared__ MeasShared sh;
   __shared__ uint64_t s_item;
@@ -1657,7 +1657,11 @@ __global__ __launch_bounds__(MAXT) void k_batch_t(
     const uint32_t c = (uint32_t)(code >> 40);
     const uint64_t w = code & ((1ull << 40) - 1);
     const pq::BatchCircuit &bc = circ[c];
-    program_word<NREG>(bc.a, bc.seed, bc.mseq0, w, dyn, sh, nullptr);
+    // an item is `wtile` consecutive word groups of one circuit (fewer at
+    // the circuit's end)
+    const uint64_t wend = min(w + wtile, bc.a.nwords);
+    for (uint64_t wi = w; wi < wend; ++wi)
+      program_word<NREG>(bc.a, bc.seed, bc.mseq0, wi, dyn, sh, nullptr);
   }
 }
 
@@ -2388,7 +2392,7 @@ hipError_t launch_batch(hipStream_t s, const BatchCircuit *circ, uint32_t ncirc,
                         const uint64_t *items, uint64_t nitems,
                         unsigned long long *counter, uint32_t threads,
                         uint32_t blocks, size_t lds_bytes, uint32_t noise_blocks,
-                        uint32_t max_qubits) {
+                        uint32_t max_qubits, uint32_t wtile) {
   if (!nitems) return hipSuccess;
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, s, counter);
   hipError_t e = hipGetLastError();
@@ -2401,11 +2405,11 @@ hipError_t launch_batch(hipStream_t s, const BatchCircuit *circ, uint32_t ncirc,
   if (max_qubits <= kSmallQubits)
     hipLaunchKernelGGL((k_batch_t<kSmallRegs, kSmallThreads>), dim3(blocks),
                        dim3(std::min<uint32_t>(threads, kSmallThreads)), lds_bytes, s, circ,
-                       items, nitems, counter);
+                       items, nitems, counter, std::max<uint32_t>(wtile, 1));
   else
     hipLaunchKernelGGL((k_batch_t<kLargeRegs, kLargeThreads>), dim3(blocks),
                        dim3(std::min<uint32_t>(threads, kLargeThreads)), lds_bytes, s, circ,
-                       items, nitems, counter);
+                       items, nitems, counter, std::max<uint32_t>(wtile, 1));
   return hipGetLastError();
 }
 

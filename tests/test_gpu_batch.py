@@ -55,8 +55,9 @@ def standalone(ps, spec, runs=1):
     return snapshot(st)
 
 
-def batched(ps, specs, form, runs=1, autotune=False):
+def batched(ps, specs, form, runs=1, autotune=False, word_tile=1):
     batch = ps.Batch()
+    batch.set_word_tile(word_tile)
     states = []
     for n, B, seed, ops in specs:
         st = ps.CliffordState(n, B, seed, True)
@@ -91,6 +92,18 @@ def test_batch_equals_standalone(ps, form):
     info = batch.info()
     assert info["members"] == len(specs)
     assert info["items"] == sum((B + 63) // 64 for _, B, _, _ in specs)
+    for spec, g in zip(specs, got):
+        same(standalone(ps, spec), g)
+
+
+@pytest.mark.parametrize("form,tile", [("persistent", 2), ("persistent", 3), ("graph", 4)])
+def test_batch_word_tiles_equal_standalone(ps, form, tile):
+    # several word groups per work item (members of 1..6 groups, so tiles
+    # end short at a member's last group)
+    specs = members()
+    say(f"[case] batch of {len(specs)} form={form} word tile {tile}")
+    got, batch = batched(ps, specs, form, word_tile=tile)
+    assert batch.info()["word_tile"] == tile
     for spec, g in zip(specs, got):
         same(standalone(ps, spec), g)
 
