@@ -26,6 +26,50 @@ sys.path.insert(0, os.path.join(ROOT, "plaquette-simulator_amd"))
 HBM_PEAK_GBS = 8000.0
 
 
+def live_traffic(timeout_s=150):
+    """HBM bytes of one k_program launch of the d=9 workload, measured now:
+    two rocprofv3 --pmc runs of `profiling.py probe` (FETCH_SIZE, then
+    WRITE_SIZE), each counter scaled by what it reads for the probe's
+    device-to-device copy of the whole tableau (a known byte count).  Child
+    processes, each under a time limit; returns (result dict, None) or
+    (None, reason)."""
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None, "rocprofv3 not found"
+    probe = os.path.join(ROOT, "plaquette-simulator_amd", "plaquette_simulator", "profiling.py")
+    work = tempfile.mkdtemp(prefix="pq_traffic_", dir="/tmp")
+    env = dict(os.environ, TMPDIR="/tmp")
+    try:
+        dirs = []
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            out = os.path.join(work, ctr.lower())
+            progress(f"traffic: rocprofv3 --pmc {ctr}")
+            try:
+                r = subprocess.run([prof, "--pmc", ctr, "--output-format", "csv", "-d", out,
+                                    "-o", "run", "--", sys.executable, probe, "probe"],
+                                   cwd="/tmp", env=env, stdout=subprocess.DEVNULL,
+                                   stderr=subprocess.DEVNULL, timeout=timeout_s)
+            except subprocess.TimeoutExpired:
+                return None, f"rocprofv3 {ctr} pass over {timeout_s} s"
+            if r.returncode != 0:
+                return None, f"rocprofv3 {ctr} pass exit {r.returncode}"
+            dirs.append(out)
+        import contextlib
+        sys.path.insert(0, os.path.dirname(probe))
+        import profiling
+        with contextlib.redirect_stdout(sys.stderr):  # stdout carries the JSON line only
+            res = profiling.traffic(dirs[0], dirs[1], os.path.join(work, "t.json"),
+                                    os.path.join(work, "t.md"))
+        return res, None
+    except Exception as e:  # a missing counter file or the like: report, do not fail the line
+        return None, f"{type(e).__name__}: {e}"
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def progress(msg):
     """One progress line on stderr (long modes print one per phase)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -662,6 +706,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 passes that measure the headline kernel's HBM bytes")
     ap.add_argument("--shots", type=int, default=16384)
     ap.add_argument("--threads", type=int, default=0, help="persistent kernel block size")
     ap.add_argument("--slices", type=int, default=0,
@@ -864,20 +910,30 @@ def main():
         "algorithmic_bytes": abytes,
         "census": census,
     }
-    # measured HBM bytes of one k_program launch of this workload: rocprofv3's
-    # L2 memory-side request counters by size, committed beside the kernel
-    # trace (counters cannot be read from inside the timed run)
-    traffic_file = os.path.join(ROOT, "profiles", "k_program_surface_d9_traffic.json")
-    if os.path.exists(traffic_file):
-        tr = json.load(open(traffic_file))
-        out["roofline"]["traffic"] = tr.get("bytes_per_launch")
-        out["roofline"]["traffic_source"] = ("profiles/k_program_surface_d9_traffic.json: " +
-                                             tr.get("method", "rocprofv3 counters"))
-        # measured HBM bytes (rocprof, same kernel and workload) per launch time
-        if tr.get("bytes_per_launch"):
-            gbs = tr["bytes_per_launch"] / (kernel_ms / 1e3) / 1e9
-            out["roofline"]["traffic_gbs"] = gbs
-            out["roofline"]["traffic_frac"] = gbs / HBM_PEAK_GBS
+    # measured HBM bytes of one k_program launch of this workload: counted
+    # now by rocprofv3 in child processes (counters cannot be read inside the
+    # timed run), else the committed request-size count of the same kernel
+    live, why = (None, "--no-traffic") if args.no_traffic else live_traffic()
+    rf = out["roofline"]
+    if live:
+        rf["traffic"] = live["bytes_per_launch"]
+        rf["traffic_source"] = ("measured in this run: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
+                                "of profiling.py probe, each scaled by the same counter for a "
+                                "device-to-device copy of the tableau")
+        rf["traffic_detail"] = live
+    else:
+        traffic_file = os.path.join(ROOT, "profiles", "k_program_surface_d9_traffic.json")
+        if os.path.exists(traffic_file):
+            tr = json.load(open(traffic_file))
+            rf["traffic"] = tr.get("bytes_per_launch")
+            rf["traffic_source"] = ("profiles/k_program_surface_d9_traffic.json: " +
+                                    tr.get("method", "rocprofv3 counters") +
+                                    f" (not measured in this run: {why})")
+    if rf.get("traffic"):
+        # measured HBM bytes (same kernel and workload) per launch time
+        gbs = rf["traffic"] / (kernel_ms / 1e3) / 1e9
+        rf["traffic_gbs"] = gbs
+        rf["traffic_frac"] = gbs / HBM_PEAK_GBS
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
     print(json.dumps(out))

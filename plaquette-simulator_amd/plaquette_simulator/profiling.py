@@ -208,6 +208,7 @@ def traffic(fetch_dir, write_dir, out_json, out_md, copy_bytes=None):
     with open(out_md, "w") as f:
         f.write("\n".join(md) + "\n")
     print("\n".join(md))
+    return res
 
 
 def sizes(rd_dir, wr_dir, out_json, out_md):
