@@ -62,7 +62,7 @@ struct pq_batch_s {
   uint32_t threads = kProgramThreadsMax;
   uint32_t blocks = 0;             // 0: CUs x blocks per CU
   uint64_t lds_budget = kProgramLdsBytes;
-  uint32_t noise_blocks = 64;
+  uint64_t noise_tasks = 0;  // the run's noise tasks (word group x 32 ops), all members
   hipEvent_t ev_upload = nullptr, ev_a = nullptr, ev_b = nullptr;
   bool timed = false;
   hipGraph_t graph = nullptr;
@@ -258,15 +258,12 @@ static int tiled_items(pq_batch b) {
 
 static int enqueue(pq_batch b) {
   const uint32_t nc = (uint32_t)b->members.size();
-  uint32_t noise_blocks = 0;
-  for (uint32_t c = 0; c < nc; ++c)
-    if (b->h_circ[c].nnoise) noise_blocks = b->noise_blocks;
   uint32_t max_n = 0;
   for (uint32_t c = 0; c < nc; ++c) max_n = std::max<uint32_t>(max_n, b->h_circ[c].a.n);
   const bool tiled = b->wtile > 1;
   return launch_batch(b->stream, b->d_circ, nc, tiled ? b->d_titems : b->d_items,
                       tiled ? b->ntitems : b->nitems, b->d_counter, b->threads,
-                      launch_blocks(b), launch_lds(b), noise_blocks, max_n,
+                      launch_blocks(b), launch_lds(b), b->noise_tasks, max_n,
                       b->wtile) == hipSuccess
              ? PQ_OK
              : pq_set_error(PQ_ERR_HIP, "batch launch failed");
@@ -446,7 +443,14 @@ extern "C" int pq_batch_run(pq_batch b, int form) {
     if (rc) return rc;
   }
   const uint64_t lds_words = launch_lds(b) / 8;
-  for (uint32_t c = 0; c < nc; ++c) b->h_circ[c].a.lds_words = lds_words;
+  // the noise tasks (word group x 32 ops) of all members in one numbering
+  b->noise_tasks = 0;
+  for (uint32_t c = 0; c < nc; ++c) {
+    BatchCircuit &bc = b->h_circ[c];
+    bc.a.lds_words = lds_words;
+    bc.task0 = b->noise_tasks;
+    b->noise_tasks += bc.a.W * ((bc.nnoise + 31) / 32);
+  }
   HIPCHK(hipMemcpyAsync(b->d_circ, b->h_circ, nc * sizeof(BatchCircuit),
                         hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipEventRecord(b->ev_upload, b->stream));

@@ -745,21 +745,22 @@ __device__ __forceinline__ void stage_chunk(const pq::ProgramArgs &a, uint64_t w
                                             uint64_t *s_msk) {
   const uint64_t W = a.W;
   cinstr_ptr ins = (cinstr_ptr)a.instrs;
-  cword_ptr planes = (cword_ptr)a.planes;
+  // this word group's planes, group-major: plane k's X and Z words at 2k, 2k+1
+  cword_ptr gp = (cword_ptr)a.planes + 2ull * a.nk * w;
   const uint64_t *__restrict__ recs = a.recs;
   for (uint32_t t = threadIdx.x; t < cn; t += blockDim.x) {
     const DInstr I = ld_instr(ins, cb + t);
     uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
     if (I.na != kNone) {
-      m0 = planes[(2ull * I.na) * W + w];
-      m1 = planes[(2ull * I.na + 1) * W + w];
+      m0 = gp[2ull * I.na];
+      m1 = gp[2ull * I.na + 1];
     }
     if (I.nb != kNone) {
-      m2 = planes[(2ull * I.nb) * W + w];
-      m3 = planes[(2ull * I.nb + 1) * W + w];
+      m2 = gp[2ull * I.nb];
+      m3 = gp[2ull * I.nb + 1];
     } else if (I.code >= PQ_OP_H_FLAG && I.code != PQ_OP_NOISE) {
       // flag of a flagged op: a record's results or a random-flag plane
-      m2 = (I.slot & kSlotPlane) ? planes[(2ull * (I.slot & ~kSlotPlane)) * W + w]
+      m2 = (I.slot & kSlotPlane) ? gp[2ull * (I.slot & ~kSlotPlane)]
                                  : recs[(2ull * I.slot) * W + w];
     }
     // for the row-tiled path: whether the op's noise masks are non-zero on
@@ -1763,67 +1764,10 @@ __global__ __launch_bounds__(256) void k_batch_frame_noise(const pq::BatchCircui
   }
 }
 
-__global__ __launch_bounds__(256) void k_batch_noise(const pq::BatchCircuit *__restrict__ circ) {
-  const pq::BatchCircuit &bc = circ[blockIdx.y];
-  const uint64_t W = bc.a.W;
-  const uint64_t nwv = (uint64_t)gridDim.x * (blockDim.x / 64);
-  const uint64_t wave = blockIdx.x * (uint64_t)(blockDim.x / 64) + (threadIdx.x >> 6);
-
-  // waves along word groups (about 16 groups each, so one stream key serves
-  // many draws) and along noise ops
-  const uint64_t ww = W / 16 < 1 ? 1 : (W / 16 < nwv ? W / 16 : nwv);
-  const uint64_t kstride = nwv / ww;
-  if (!W || wave >= kstride * ww) return;
-  const uint32_t lane = lane_id();
-  const uint64_t lane_g = lane_golden();
-  uint64_t *planes = const_cast<uint64_t *>(bc.a.planes);
-  for (uint64_t k = wave / ww; k < bc.nnoise; k += kstride) {
-    const NoiseDesc d = bc.noise[k];
-    const uint64_t key = stream_key(bc.seed, 2 * (bc.nseq0 + k) + 1);
-    for (uint64_t w = wave % ww; w < W; w += ww) {
-      bool bx, bz;
-      noise_bits(d, draw_word_lane(key, w, lane_g), bx, bz);
-      const uint64_t mx = __ballot(bx);
-      const uint64_t mz = __ballot(bz);
-      if (lane == 0) {
-        planes[(2 * k) * W + w] = mx;
-        planes[(2 * k + 1) * W + w] = mz;
-      }
-    }
-  }
-}
-
 __global__ void k_zero_counter(unsigned long long *counter) {
   if (threadIdx.x == 0) *counter = 0;
 }
 
-// Noise planes of a program: plane k uses stream 2*(nseq0 + k) + 1.
-__global__ __launch_bounds__(256) void k_program_noise(
-    uint64_t *__restrict__ planes, const NoiseDesc *__restrict__ desc,
-    uint32_t nk, uint64_t W, const ProgParams *__restrict__ pp) {
-  // blockIdx.y strides over noise ops (one stream key per op), the waves of
-  // blockIdx.x over word groups: one draw per lane per (op, word group)
-  const uint32_t lane = lane_id();
-  const uint64_t lane_g = lane_golden();
-  const uint64_t seed = pp->seed, nseq0 = pp->nseq0;
-  const uint64_t wv = blockIdx.x * (uint64_t)(blockDim.x / 64) + (threadIdx.x >> 6);
-  const uint64_t nwv = (uint64_t)gridDim.x * (blockDim.x / 64);
-  for (uint32_t k = blockIdx.y; k < nk; k += gridDim.y) {
-    const NoiseDesc d = desc[k];
-    const uint64_t key = stream_key(seed, 2 * (nseq0 + k) + 1);
-    uint64_t *px = planes + (2ull * k) * W, *pz = px + W;
-    for (uint64_t w = wv; w < W; w += nwv) {
-      bool bx, bz;
-      noise_bits(d, draw_word_lane(key, w, lane_g), bx, bz);
-      const uint64_t mx = __ballot(bx);
-      const uint64_t mz = __ballot(bz);
-      if (lane == 0) {
-        px[w] = mx;
-        pz[w] = mz;
-      }
-    }
-  }
-}
 
 // RowMultiply / RowProductSign (MeasurementFunctors.hpp:98-166) on every
 // shot: one thread per word group; sign_out (if set) receives the sign plane
@@ -2295,15 +2239,8 @@ size_t program_lds_bytes(uint32_t n, uint32_t det_waves) {
 hipError_t launch_program_noise(hipStream_t s, uint64_t *planes,
                                 const NoiseDesc *desc, uint32_t nk, uint64_t W,
                                 const void *params) {
-  const uint64_t items = (uint64_t)nk * W;
-  if (!items) return hipSuccess;
-  // about 64 word groups per wave, so the stream key (a full SplitMix64
-  // round of 64-bit multiplies) is amortized over many draws
-  const uint32_t gx = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(W / 256, 64));
-  const uint32_t gy = std::min<uint32_t>(nk, 65535);
-  hipLaunchKernelGGL(k_program_noise, dim3(gx, gy), dim3(256), 0, s, planes, desc, nk, W,
-                     (const ProgParams *)params);
-  return hipGetLastError();
+  // plane k uses stream 2 (nseq0 + k) + 1, as the frame form's planes
+  return launch_frame_noise(s, planes, desc, nk, 0, W, params, 0, nk);
 }
 
 hipError_t launch_frame(hipStream_t s, const FrameArgs &a) {
@@ -2345,12 +2282,6 @@ hipError_t launch_batch_frame_noise(hipStream_t s, const BatchCircuit *circ, uin
   return hipGetLastError();
 }
 
-hipError_t launch_batch_noise(hipStream_t s, const BatchCircuit *circ, uint32_t ncirc,
-                              uint32_t noise_blocks) {
-  if (!ncirc || !noise_blocks) return hipSuccess;
-  hipLaunchKernelGGL(k_batch_noise, dim3(noise_blocks, ncirc), dim3(256), 0, s, circ);
-  return hipGetLastError();
-}
 
 hipError_t launch_frame_xz(hipStream_t s, uint64_t *tab, const Geom &g, const uint64_t *X,
                            const uint64_t *Z, uint32_t nw) {
@@ -2391,15 +2322,14 @@ hipError_t prepare_kernels(uint32_t n) {
 hipError_t launch_batch(hipStream_t s, const BatchCircuit *circ, uint32_t ncirc,
                         const uint64_t *items, uint64_t nitems,
                         unsigned long long *counter, uint32_t threads,
-                        uint32_t blocks, size_t lds_bytes, uint32_t noise_blocks,
+                        uint32_t blocks, size_t lds_bytes, uint64_t noise_tasks,
                         uint32_t max_qubits, uint32_t wtile) {
   if (!nitems) return hipSuccess;
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, s, counter);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if (noise_blocks) {
-    hipLaunchKernelGGL(k_batch_noise, dim3(noise_blocks, ncirc), dim3(256), 0, s, circ);
-    e = hipGetLastError();
+  if (noise_tasks) {
+    e = launch_batch_frame_noise(s, circ, ncirc, noise_tasks);
     if (e != hipSuccess) return e;
   }
   if (max_qubits <= kSmallQubits)

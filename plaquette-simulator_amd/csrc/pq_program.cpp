@@ -435,6 +435,7 @@ ProgramArgs program_args(pq_state s, pq_program p, uint64_t lds_budget) {
   a.tab = s->tab;
   a.recs = s->recs;
   a.planes = p->d_planes;
+  a.nk = (uint32_t)p->nnoise;
   a.segs = p->d_segs;
   a.instrs = p->d_instrs;
   a.coin_planes = nullptr;
