@@ -906,17 +906,17 @@ constexpr uint32_t kTileMaxRows = 64;
 constexpr uint32_t kDmaWords = 32;  // words one wave-wide 4-byte LDS DMA moves
 constexpr int kTileOps = PQ_TILE_OPS;
 
-__device__ __forceinline__ uint64_t tile_words(uint32_t nq, uint32_t col_words) {
+__host__ __device__ __forceinline__ uint64_t tile_words(uint32_t nq, uint32_t col_words) {
   const uint64_t w = 2ull * nq * col_words;
   return (w + kDmaWords - 1) / kDmaWords * kDmaWords;
 }
-__device__ __forceinline__ uint32_t tile_rows(uint64_t lds_words, uint32_t nq) {
+__host__ __device__ __forceinline__ uint32_t tile_rows(uint64_t lds_words, uint32_t nq) {
   if (!nq) return 0;
   const uint64_t fixed = kStageWords + (nq + 1) / 2;
   if (lds_words <= fixed + kDmaWords + 2ull * nq) return 0;
   // TR r words + 2 nq (TR + 1) tile words + up to a chunk of rounding
-  const uint64_t tr = min<uint64_t>((lds_words - fixed - kDmaWords - 2ull * nq) / (2ull * nq + 1),
-                                    kTileMaxRows);
+  const uint64_t fit = (lds_words - fixed - kDmaWords - 2ull * nq) / (2ull * nq + 1);
+  const uint64_t tr = fit < kTileMaxRows ? fit : kTileMaxRows;
   return tr >= kTileMinRows ? (uint32_t)tr : 0;
 }
 
@@ -1122,7 +1122,7 @@ __device__ __forceinline__ uint64_t tile_run_ops(lds_u64 *tl, uint32_t TRp, uint
 __device__ void run_gate_segment_tiled(uint64_t *__restrict__ T, uint64_t w,
                                        const pq::ProgramArgs &a, const DSegment &sg,
                                        uint64_t *dyn, uint32_t TR, uint32_t slice = 0,
-                                       uint32_t slices = 1) {
+                                       uint32_t slices = 1, bool fresh = false) {
   const uint64_t Ru = a.R;
   const uint32_t n = a.n;
   const uint32_t nq = sg.nq;
@@ -1156,7 +1156,20 @@ __device__ void run_gate_segment_tiled(uint64_t *__restrict__ T, uint64_t w,
     // l/2), with no register round trip, so a wave keeps dozens of chunk
     // loads in flight.  Spare words and rows past the tableau's last row
     // load that last row (they are never stored back).
-    {
+    if (fresh) {
+      // X/Z are still the identity (word (c, row) all ones iff row == c) and
+      // not in memory: the tile is written here, not loaded
+      // (a wave per column; the rounding words past the last column are
+      // never read)
+      const uint32_t last = a.R - 1 - r0;
+      for (uint32_t c = wv; c < ncol; c += nwv) {
+        const uint32_t q = s_q[c >> 1];
+        const uint32_t col = (c & 1) ? n + q : q;
+        for (uint32_t j = lane; j < TRp; j += 64)
+          tile[c * TRp + j] = (r0 + min(j, last) == col) ? ~0ull : 0ull;
+      }
+      if (slot == 0 && active) s_r[ti] = T[(uint64_t)(2 * n) * Ru + r0 + ti];
+    } else {
       const uint32_t nchunk = (uint32_t)(tile_words(nq, TRp) / kDmaWords);
       const uint32_t last = a.R - 1 - r0;
       const uint32_t step = kDmaWords * nwv;  // tile words between a lane's chunks
@@ -1492,7 +1505,7 @@ __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t m
       if (timing) t0 = wall_clock64();
       const uint32_t TR = tile_rows(a.lds_words, sg.nq);
       if (TR)
-        run_gate_segment_tiled(T, w, a, sg, dyn, TR, slice, S);
+        run_gate_segment_tiled(T, w, a, sg, dyn, TR, slice, S, a.fresh && si == 0);
       else
         run_gate_segment(T, w, a, sg, reinterpret_cast<DInstr *>(dyn),
                          dyn + kStage * (sizeof(DInstr) / 8), slice, S);
@@ -2226,6 +2239,8 @@ bool program_slices_fit(uint32_t n, uint64_t nwords, uint32_t slices, uint32_t t
     return false;
   return (uint64_t)per_cu * (uint64_t)cus >= nwords * slices;
 }
+
+uint32_t program_tile_rows(uint64_t lds_words, uint32_t nq) { return tile_rows(lds_words, nq); }
 
 uint32_t program_max_threads(uint32_t n) {
   return n <= kSmallQubits ? kSmallThreads : kLargeThreads;

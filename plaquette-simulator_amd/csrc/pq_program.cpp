@@ -510,8 +510,10 @@ static int choose_slices(pq_state s, pq_program p, const ProgramArgs &a, bool al
   return PQ_OK;
 }
 
-static int enqueue_persistent(pq_state s, pq_program p, bool timed, bool allow_slices) {
+static int enqueue_persistent(pq_state s, pq_program p, bool timed, bool allow_slices,
+                              bool fresh = false) {
   ProgramArgs a = program_args(s, p, kProgramLdsBytes);
+  a.fresh = fresh ? 1u : 0u;
   uint32_t S = 1;
   int rc = choose_slices(s, p, a, allow_slices, &S);
   if (rc) return rc;
@@ -591,7 +593,17 @@ static int program_run(pq_state s, pq_program p, int form) {
   if (form == PQ_RUN_FRAME && s->n > measure_max_qubits())
     return pq_set_error(PQ_ERR_UNSUPPORTED, "frame form supports at most 4095 qubits");
   if (form == PQ_RUN_FRAME) return run_frame(s, p);
-  if ((rc = materialize_tableau(s))) return rc;
+  // X/Z still the identity after a reset, and the program opens with a
+  // row-tiled gate segment over every qubit: the persistent kernel writes
+  // that segment's tiles itself, so the identity fill is not needed
+  const bool fresh = form == PQ_RUN_PERSISTENT && s->xz_pending && !s->xz_plan &&
+                     !p->segs.empty() && p->segs[0].kind == 0 && p->segs[0].nq == s->n &&
+                     program_tile_rows(program_args(s, p, kProgramLdsBytes).lds_words,
+                                       (uint32_t)s->n) > 0;
+  if (fresh)
+    s->xz_pending = false;  // written by the run
+  else if ((rc = materialize_tableau(s)))
+    return rc;
   s->xz_identity = false;
   if (form == PQ_RUN_PLAIN) return run_plain(s, p);
   if (s->n > measure_max_qubits())
@@ -600,7 +612,7 @@ static int program_run(pq_state s, pq_program p, int form) {
                            s->noise_seq));
   s->meas_seq += p->nmeas;
   s->noise_seq += p->nnoise;
-  if (form == PQ_RUN_PERSISTENT) return enqueue_persistent(s, p, p->timing_on, true);
+  if (form == PQ_RUN_PERSISTENT) return enqueue_persistent(s, p, p->timing_on, true, fresh);
   if (form != PQ_RUN_GRAPH) return pq_set_error(PQ_ERR_ARG, "unknown run form");
   if (p->gexec && (p->graph_state != s || p->graph_arena_gen != s->arena_gen)) {
     (void)hipGraphExecDestroy(p->gexec);

@@ -54,24 +54,43 @@ def test_surface_d9_slice_digests(ps, form):
     assert wire.digest(recs[:, :, :2]) == DIG["surface_d9_slice"]["records_sha256"]
 
 
+_DLPACK_CHECK = r"""
+import sys
+import numpy as np
+import torch  # first: its HIP runtime must be the one the process loads
+assert torch.cuda.is_available()
+sys.path[:0] = sys.argv[1:]
+import plaquette_simulator as ps
+from plaquette_simulator import workloads as W
+ops, n = W.surface_code_memory_z(3, 3, 0.01)
+st = ps.CliffordState(n, 300, 5, True)
+st.run_program(ops, "persistent")
+t = torch.from_dlpack(st.planes_dlpack())
+assert t.is_cuda and tuple(t.shape) == (5, 2 * n + 1, 2 * n + 1)
+assert np.array_equal(t.cpu().numpy().astype(np.uint64), st.read_planes())
+rt = torch.from_dlpack(st.records_dlpack())
+assert np.array_equal(rt.cpu().numpy().astype(np.uint64), st.read_records())
+# zero copy: the engine's own later writes show through the tensor
+st.apply_hadamard_gate(0)
+st.synchronize()
+assert np.array_equal(t.cpu().numpy().astype(np.uint64), st.read_planes())
+del t, rt, st
+print("dlpack ok")
+"""
+
+
 def test_dlpack_zero_copy(ps):
-    torch = pytest.importorskip("torch")
-    from plaquette_simulator import workloads as W
-    ops, n = W.surface_code_memory_z(3, 3, 0.01)
-    st = ps.CliffordState(n, 300, 5, True)
-    st.run_program(ops, "persistent")
-    say("[case] dlpack export")
-    t = torch.from_dlpack(st.planes_dlpack())
-    assert t.is_cuda and tuple(t.shape) == (5, 2 * n + 1, 2 * n + 1)
-    host = st.read_planes()
-    assert np.array_equal(t.cpu().numpy().astype(np.uint64), host)
-    rt = torch.from_dlpack(st.records_dlpack())
-    assert np.array_equal(rt.cpu().numpy().astype(np.uint64), st.read_records())
-    # zero copy: the engine's own later writes show through the tensor
-    st.apply_hadamard_gate(0)
-    st.synchronize()
-    assert np.array_equal(t.cpu().numpy().astype(np.uint64), st.read_planes())
-    del t, rt, st
+    # torch carries its own HIP runtime; in a process where the engine's
+    # runtime came first torch may see no device, so the check runs in a
+    # child process that imports torch before the engine
+    pytest.importorskip("torch")
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    say("[case] dlpack export (child process)")
+    r = subprocess.run([sys.executable, "-c", _DLPACK_CHECK,
+                        os.path.join(root, "plaquette-simulator_amd")],
+                       capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0 and "dlpack ok" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
 
 
 def test_state_files(ps, tmp_path):
