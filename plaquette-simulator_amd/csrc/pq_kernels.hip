@@ -1527,8 +1527,8 @@ __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t m
       if (timing) t0 = wall_clock64();
       // which of the next measurements have a pivot in some shot?  The window
       // grows while every checked measurement is determined and restarts
-      // small after a random one (the state then changes).  A wave checks
-      // two measurements at a time with all their loads in flight.
+      // small after a random one (the state then changes).  One wave per
+      // measurement.
       const uint32_t nm = min(end - k, win);
       for (uint32_t idx = wv; idx < nm; idx += nw) {
         const uint64_t *xq = T + (uint64_t)((cinstr_ptr)a.instrs)[k + idx].qa * Ru;
