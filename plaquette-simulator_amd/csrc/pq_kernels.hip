@@ -1069,9 +1069,10 @@ typedef __attribute__((address_space(3))) uint64_t lds_u64;
 
 // A wave's share of one run, one op at a time on every lane's tile row (the
 // spare word for lanes without a row).  Op o of the share has code and flags
-// in lane o of my_code and local columns in lane o of my_t0; its masks are
-// at s_msk[4 * (i0 + o * S)].  CODE != 0 when every op of the share has that
-// code: the dispatch is then resolved at compile time.
+// in lane o of my_code and the tile word offsets of its qa and qb columns
+// (x column; z follows TRp words on) in the low and high 16 bits of lane o of
+// my_t0; its masks are at s_msk[4 * (i0 + o * S)].  CODE != 0 when every op
+// of the share has that code: the dispatch is then resolved at compile time.
 template <uint32_t CODE>
 __device__ __forceinline__ uint64_t tile_run_ops(lds_u64 *tl, uint32_t TRp, uint32_t lr,
                                                  uint32_t my_code, uint32_t my_t0, uint32_t cnt,
@@ -1080,11 +1081,13 @@ __device__ __forceinline__ uint64_t tile_run_ops(lds_u64 *tl, uint32_t TRp, uint
   uint64_t rp = 0;
   for (uint32_t o = 0; o < cnt; ++o) {
     const uint32_t cw = __builtin_amdgcn_readlane(my_code, o);
-    if (cw & (kOpIdentity << 16)) continue;  // nothing to do here
+    // only noise-only and flagged ops can be the identity on a word group
+    constexpr bool kMayBeIdentity = CODE == 0 || CODE >= PQ_OP_H_FLAG;
+    if (kMayBeIdentity && (cw & (kOpIdentity << 16))) continue;  // nothing to do here
     const uint32_t code = CODE ? CODE : (cw & 0xFFFFu);
     const uint32_t t0 = __builtin_amdgcn_readlane(my_t0, o);
-    lds_u64 *pa = tl + (2 * (t0 & 0xFFFFu) * TRp + lr);
-    lds_u64 *pb = tl + (2 * (t0 >> 16) * TRp + lr);
+    lds_u64 *pa = tl + ((t0 & 0xFFFFu) + lr);
+    lds_u64 *pb = tl + ((t0 >> 16) + lr);
     const uint64_t vx = pa[0], vz = pa[TRp];
     uint64_t vxt = 0, vzt = 0;
     if (code == PQ_OP_CX || code == PQ_OP_CZ || code == PQ_OP_CX_FLAG ||
@@ -1218,7 +1221,10 @@ __device__ void run_gate_segment_tiled(uint64_t *__restrict__ T, uint64_t w,
             if (base + lane < nmine) {
               const uint32_t idx = k + wv + (base + lane) * S;
               my_code = s_ins[idx].code | (s_ins[idx].slot << 16);
-              my_t0 = s_ins[idx].t0;
+              // local columns -> tile word offsets (a tile holds fewer
+              // than 2^16 words: LDS is 160 KB)
+              const uint32_t t = s_ins[idx].t0;
+              my_t0 = (2u * (t & 0xFFFFu) * TRp) | ((2u * (t >> 16) * TRp) << 16);
             }
             const uint32_t cnt = min(64u, nmine - base);
             const uint32_t i0 = k + wv + base * S;
@@ -1272,7 +1278,9 @@ __device__ void run_gate_segment_tiled(uint64_t *__restrict__ T, uint64_t w,
             }
           }
         }
-        __syncthreads();
+        // runs of one layer act on distinct qubits: a barrier only before
+        // a new layer and at the end of the staged chunk
+        if (e == cn || (s_ins[e].pad & kLayerStart)) __syncthreads();
         k = e;
       }
     }

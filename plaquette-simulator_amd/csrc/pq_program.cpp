@@ -72,15 +72,18 @@ int validate(uint64_t n, const pq_op *ops, uint64_t nops) {
 int validate_ops(uint64_t n, const pq_op *ops, uint64_t nops) { return validate(n, ops, nops); }
 
 // Compile `ops` into segments/instructions (host side only).
-// Run lengths of a gate segment: a run starts at every layer start and at
-// every staging-chunk start and ends before the next one; its length goes
-// into bits 8-19 of its first instruction's pad.
+// Run lengths of a gate segment: a run starts at every layer start, at
+// every staging-chunk start and where the op code changes (a layer's ops are
+// sorted by code), and ends before the next one; its length goes into bits
+// 8-19 of its first instruction's pad.  Every run then has one op code, so
+// the row-tiled kernel dispatches on it once per wave share.
 static void mark_runs(pq_program_s *p, const DSegment &sg) {
   const uint32_t end = (uint32_t)p->instrs.size();
   uint32_t start = sg.begin;
   for (uint32_t k = sg.begin + 1; k <= end; ++k) {
     const bool brk = k == end || (p->instrs[k].pad & kLayerStart) ||
-                     (k - sg.begin) % kProgramStage == 0;
+                     (k - sg.begin) % kProgramStage == 0 ||
+                     p->instrs[k].code != p->instrs[k - 1].code;
     if (brk) {
       p->instrs[start].pad |= (k - start) << kRunShift;
       start = k;
@@ -209,8 +212,11 @@ int compile_program(pq_program_s *p) {
       }
       std::vector<uint32_t> order(seg.size());
       for (uint32_t k = 0; k < order.size(); ++k) order[k] = k;
-      std::stable_sort(order.begin(), order.end(),
-                       [&](uint32_t a, uint32_t b) { return layer[a] < layer[b]; });
+      // by layer, and inside a layer by op code (the ops of a layer act on
+      // distinct qubits, so their order does not change the result)
+      std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return layer[a] != layer[b] ? layer[a] < layer[b] : seg[a].code < seg[b].code;
+      });
       uint32_t k = 0;
       while (k < order.size()) {
         uint32_t e = k;
