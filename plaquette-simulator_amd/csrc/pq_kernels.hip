@@ -740,6 +740,8 @@ constexpr uint32_t kStageWords = kStage * (sizeof(DInstr) / 8 + 4);
 // in LDS; a staged instruction's `slot` (read only here) is replaced by the
 // kOp* flags below.  No barrier.
 constexpr uint32_t kOpNoisy = 1, kOpIdentity = 2;
+// the same flags in a row-tiled op's broadcast word of tile offsets
+constexpr uint32_t kTileNoisyBit = 1u << 15, kTileIdentityBit = 1u << 31;
 __device__ __forceinline__ void stage_chunk(const pq::ProgramArgs &a, uint64_t w,
                                             uint32_t cb, uint32_t cn, DInstr *s_ins,
                                             uint64_t *s_msk) {
@@ -1068,11 +1070,12 @@ __device__ __forceinline__ uint64_t tile_op_uniform(uint32_t code, Ptr pxa, Ptr 
 typedef __attribute__((address_space(3))) uint64_t lds_u64;
 
 // A wave's share of one run, one op at a time on every lane's tile row (the
-// spare word for lanes without a row).  Op o of the share has code and flags
-// in lane o of my_code and the tile word offsets of its qa and qb columns
-// (x column; z follows TRp words on) in the low and high 16 bits of lane o of
-// my_t0; its masks are at s_msk[4 * (i0 + o * S)].  CODE != 0 when every op
-// of the share has that code: the dispatch is then resolved at compile time.
+// spare word for lanes without a row).  Op o of the share has its code in
+// lane o of my_code and, in lane o of my_t0, the tile word offsets of its qa
+// and qb columns (bits 0-14 and 16-30; x column, z follows TRp words on) and
+// its flags (kTileNoisyBit, kTileIdentityBit); its masks are at
+// s_msk[4 * (i0 + o * S)].  CODE != 0 when every op of the share has that
+// code: the dispatch is then resolved at compile time, and my_code is not read.
 template <uint32_t CODE>
 __device__ __forceinline__ uint64_t tile_run_ops(lds_u64 *tl, uint32_t TRp, uint32_t lr,
                                                  uint32_t my_code, uint32_t my_t0, uint32_t cnt,
@@ -1080,14 +1083,13 @@ __device__ __forceinline__ uint64_t tile_run_ops(lds_u64 *tl, uint32_t TRp, uint
                                                  uint32_t S) {
   uint64_t rp = 0;
   for (uint32_t o = 0; o < cnt; ++o) {
-    const uint32_t cw = __builtin_amdgcn_readlane(my_code, o);
+    const uint32_t t0 = __builtin_amdgcn_readlane(my_t0, o);
     // only noise-only and flagged ops can be the identity on a word group
     constexpr bool kMayBeIdentity = CODE == 0 || CODE >= PQ_OP_H_FLAG;
-    if (kMayBeIdentity && (cw & (kOpIdentity << 16))) continue;  // nothing to do here
-    const uint32_t code = CODE ? CODE : (cw & 0xFFFFu);
-    const uint32_t t0 = __builtin_amdgcn_readlane(my_t0, o);
-    lds_u64 *pa = tl + ((t0 & 0xFFFFu) + lr);
-    lds_u64 *pb = tl + ((t0 >> 16) + lr);
+    if (kMayBeIdentity && (t0 & kTileIdentityBit)) continue;  // nothing to do here
+    const uint32_t code = CODE ? CODE : __builtin_amdgcn_readlane(my_code, o);
+    lds_u64 *pa = tl + ((t0 & 0x7FFFu) + lr);
+    lds_u64 *pb = tl + (((t0 >> 16) & 0x7FFFu) + lr);
     const uint64_t vx = pa[0], vz = pa[TRp];
     uint64_t vxt = 0, vzt = 0;
     if (code == PQ_OP_CX || code == PQ_OP_CZ || code == PQ_OP_CX_FLAG ||
@@ -1096,7 +1098,7 @@ __device__ __forceinline__ uint64_t tile_run_ops(lds_u64 *tl, uint32_t TRp, uint
       vzt = pb[TRp];
     }
     const uint64_t *M = s_msk + 4 * (i0 + o * S);
-    if (cw & (kOpNoisy << 16)) {
+    if (t0 & kTileNoisyBit) {
       rp ^= tile_op_uniform<true>(code, pa, pa + TRp, pb, pb + TRp, vx, vz, vxt, vzt, M[0], M[1],
                                   M[2], M[3]);
     } else {
@@ -1220,18 +1222,20 @@ __device__ void run_gate_segment_tiled(uint64_t *__restrict__ T, uint64_t w,
             uint32_t my_code = 0, my_t0 = 0;
             if (base + lane < nmine) {
               const uint32_t idx = k + wv + (base + lane) * S;
-              my_code = s_ins[idx].code | (s_ins[idx].slot << 16);
+              my_code = s_ins[idx].code;
               // local columns -> tile word offsets (a tile holds fewer
-              // than 2^16 words: LDS is 160 KB)
-              const uint32_t t = s_ins[idx].t0;
-              my_t0 = (2u * (t & 0xFFFFu) * TRp) | ((2u * (t >> 16) * TRp) << 16);
+              // than 2^15 words: LDS is 160 KB), flags in bits 15 and 31
+              const uint32_t t = s_ins[idx].t0, f = s_ins[idx].slot;
+              my_t0 = (2u * (t & 0xFFFFu) * TRp) | ((2u * (t >> 16) * TRp) << 16) |
+                      ((f & kOpNoisy) ? kTileNoisyBit : 0u) |
+                      ((f & kOpIdentity) ? kTileIdentityBit : 0u);
             }
             const uint32_t cnt = min(64u, nmine - base);
             const uint32_t i0 = k + wv + base * S;
             // one code for the whole share (the usual case: a layer of one
             // gate kind): dispatch once, outside the op loop
-            const uint32_t c0 = __builtin_amdgcn_readlane(my_code, 0) & 0xFFFFu;
-            const bool uni = __ballot(lane < cnt && (my_code & 0xFFFFu) != c0) == 0;
+            const uint32_t c0 = __builtin_amdgcn_readlane(my_code, 0);
+            const bool uni = __ballot(lane < cnt && my_code != c0) == 0;
             if (uni && c0 == PQ_OP_CX)
               rp ^= tile_run_ops<PQ_OP_CX>(tl, TRp, lr, my_code, my_t0, cnt, s_msk, i0, S);
             else if (uni && c0 == PQ_OP_H)
