@@ -699,6 +699,8 @@ PYBIND11_MODULE(plaquette_simulator_bindings, m) {
       py::return_value_policy::take_ownership,
       "A CliffordState that keeps only its phase column: frame-form programs from |0...0>, "
       "records and reset only (W*(2n+1) words on the device instead of W*(2n+1)^2)");
+  m.def("set_window_qubits",
+        [](uint32_t q) { check(pq_set_window_qubits(q), "set_window_qubits"); });
   m.def("coin_threshold", &pq_coin_threshold);
   m.def("draw", &pq_draw);
   m.def("backend_name", &pq_backend_name);

@@ -15,7 +15,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <memory>
+#include <queue>
 #include <string>
 #include <vector>
 
@@ -134,6 +136,166 @@ static void plan_tiling(pq_program_s *p, DSegment &sg) {
   }
 }
 
+// One gate segment from ops in an order that respects every qubit's program
+// order: ASAP layers on qubit dependencies (ops of a layer act on distinct
+// qubits, so they commute bit-exactly: each rewrites only its own columns and
+// XORs into r a function of them), each layer's ops ordered by code and cut
+// into batches, runs marked, row tiling planned.
+static void emit_gate_segment(pq_program_s *p, const std::vector<DInstr> &seg) {
+  if (seg.empty()) return;
+  DSegment sg{0u, (uint32_t)p->instrs.size(), 0, 0, 0, 0, 0, 0};
+  std::vector<int32_t> qlayer(p->n, -1);
+  std::vector<uint32_t> layer(seg.size());
+  for (size_t k = 0; k < seg.size(); ++k) {
+    const DInstr &I = seg[k];
+    int32_t L = qlayer[I.qa] + 1;
+    if (is_two(I.code)) L = std::max(L, qlayer[I.qb] + 1);
+    qlayer[I.qa] = L;
+    if (is_two(I.code)) qlayer[I.qb] = L;
+    layer[k] = (uint32_t)L;
+  }
+  std::vector<uint32_t> order(seg.size());
+  for (uint32_t k = 0; k < order.size(); ++k) order[k] = k;
+  // by layer, and inside a layer by op code
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    return layer[a] != layer[b] ? layer[a] < layer[b] : seg[a].code < seg[b].code;
+  });
+  uint32_t k = 0;
+  while (k < order.size()) {
+    uint32_t e = k;
+    while (e < order.size() && layer[order[e]] == layer[order[k]]) ++e;
+    for (uint32_t b = k; b < e;) {
+      // batches never straddle a staging chunk of the kernel
+      const uint32_t pos = (uint32_t)(p->instrs.size() - sg.begin) % kProgramStage;
+      const uint32_t len = std::min<uint32_t>({kProgramBatch, e - b, kProgramStage - pos});
+      for (uint32_t u = 0; u < len; ++u) {
+        DInstr I = seg[order[b + u]];
+        I.pad = (u == 0) ? (len | (b == k ? kLayerStart : 0u)) : 0u;
+        p->instrs.push_back(I);
+      }
+      b += len;
+    }
+    k = e;
+  }
+  mark_runs(p, sg);
+  plan_tiling(p, sg);
+  sg.count = (uint32_t)(p->instrs.size() - sg.begin);
+  p->segs.push_back(sg);
+}
+
+static uint32_t count_qubits(const std::vector<DInstr> &seg, uint64_t n) {
+  std::vector<char> seen(n, 0);
+  uint32_t c = 0;
+  for (const DInstr &I : seg) {
+    if (!seen[I.qa]) { seen[I.qa] = 1; ++c; }
+    if (is_two(I.code) && !seen[I.qb]) { seen[I.qb] = 1; ++c; }
+  }
+  return c;
+}
+
+// Most qubits whose row tile keeps kWindowRows rows in the default LDS budget
+// (0: windows off).  Wider gate segments are cut into qubit windows.
+static std::atomic<uint32_t> g_window_qubits{0};
+
+static uint32_t window_qubits(const pq_program_s *p) {
+  const uint32_t set = g_window_qubits.load();
+  if (set) return set == ~0u ? 0u : set;
+  const uint64_t words = kProgramLdsBytes / 8;
+  uint32_t lo = 0, hi = (uint32_t)std::min<uint64_t>(p->n, 0xFFFFu);
+  while (lo < hi) {  // largest nq with program_tile_rows(words, nq) >= kWindowRows
+    const uint32_t mid = (lo + hi + 1) / 2;
+    if (pq::program_tile_rows(words, mid) >= kWindowRows) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Cuts a gate segment (ops in program order) into windows of at most `cap`
+// distinct qubits, each a list of op indices whose concatenation keeps every
+// qubit's ops in program order (an op is taken only when it is the next one
+// on each of its qubits).  A window first takes every ready op on qubits it
+// already holds, then the earliest ready op that fits, so windows grow over
+// neighbouring qubits and run several layers on them before moving on.
+static std::vector<std::vector<uint32_t>> window_schedule(const std::vector<DInstr> &seg,
+                                                          uint64_t n, uint32_t cap) {
+  const uint32_t nops = (uint32_t)seg.size();
+  auto nq_of = [&](uint32_t i) { return is_two(seg[i].code) ? 2u : 1u; };
+  auto q_of = [&](uint32_t i, uint32_t t) { return t ? seg[i].qb : seg[i].qa; };
+  // per-qubit op lists (CSR)
+  std::vector<uint32_t> off(n + 1, 0), pos(n, 0);
+  for (uint32_t i = 0; i < nops; ++i)
+    for (uint32_t t = 0; t < nq_of(i); ++t) ++off[q_of(i, t) + 1];
+  for (uint64_t q = 0; q < n; ++q) off[q + 1] += off[q];
+  std::vector<uint32_t> list(off[n]);
+  {
+    std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+    for (uint32_t i = 0; i < nops; ++i)
+      for (uint32_t t = 0; t < nq_of(i); ++t) list[fill[q_of(i, t)]++] = i;
+  }
+  auto next_on = [&](uint64_t q) -> int64_t {
+    return off[q] + pos[q] < off[q + 1] ? (int64_t)list[off[q] + pos[q]] : -1;
+  };
+  auto ready = [&](uint32_t i) {
+    for (uint32_t t = 0; t < nq_of(i); ++t)
+      if (next_on(q_of(i, t)) != (int64_t)i) return false;
+    return true;
+  };
+  typedef std::priority_queue<uint32_t, std::vector<uint32_t>, std::greater<uint32_t>> MinHeap;
+  MinHeap others, inside;
+  std::vector<char> done(nops, 0), in_win(n, 0);
+  std::vector<uint32_t> win_q;
+  for (uint32_t i = 0; i < nops; ++i)
+    if (ready(i)) others.push(i);
+  auto all_in = [&](uint32_t i) {
+    for (uint32_t t = 0; t < nq_of(i); ++t)
+      if (!in_win[q_of(i, t)]) return false;
+    return true;
+  };
+  std::vector<std::vector<uint32_t>> parts;
+  uint32_t left = nops;
+  while (left) {
+    for (uint32_t q : win_q) in_win[q] = 0;
+    win_q.clear();
+    std::vector<uint32_t> part;
+    for (;;) {
+      uint32_t i;
+      if (!inside.empty()) {
+        i = inside.top();
+        inside.pop();
+      } else {
+        while (!others.empty() && done[others.top()]) others.pop();
+        if (others.empty()) break;
+        i = others.top();
+        uint32_t fresh = 0;
+        for (uint32_t t = 0; t < nq_of(i); ++t) fresh += in_win[q_of(i, t)] ? 0u : 1u;
+        if (!part.empty() && win_q.size() + fresh > cap) break;
+        others.pop();
+      }
+      if (done[i]) continue;
+      done[i] = 1;
+      --left;
+      part.push_back(i);
+      for (uint32_t t = 0; t < nq_of(i); ++t) {
+        const uint32_t q = q_of(i, t);
+        if (!in_win[q]) {
+          in_win[q] = 1;
+          win_q.push_back(q);
+        }
+        ++pos[q];
+      }
+      // ops this one made ready, and ready ops now wholly inside the window
+      for (uint32_t t = 0; t < nq_of(i); ++t) {
+        const int64_t jn = next_on(q_of(i, t));
+        if (jn < 0) continue;
+        const uint32_t j = (uint32_t)jn;
+        if (done[j] || !ready(j)) continue;
+        if (all_in(j)) inside.push(j); else others.push(j);
+      }
+    }
+    parts.push_back(std::move(part));
+  }
+  return parts;
+}
+
 int compile_program(pq_program_s *p) {
   const pq_op *ops = p->ops.data();
   const uint64_t nops = p->ops.size();
@@ -161,11 +323,9 @@ int compile_program(pq_program_s *p) {
         p->instrs.push_back(I);
       }
     } else {
-      // ASAP layering with noise fusion
+      // noise fusion: each NOISE op joins the op that last touched its qubit
       std::vector<DInstr> seg;
-      std::vector<uint32_t> layer;
       std::vector<int64_t> last(p->n, -1);
-      std::vector<int32_t> qlayer(p->n, -1);
       std::vector<char> noise_only;
       for (uint64_t k = i; k < j; ++k) {
         const pq_op &o = ops[k];
@@ -183,22 +343,13 @@ int compile_program(pq_program_s *p) {
               continue;
             }
           }
-          const int32_t L = qlayer[o.q0] + 1;
-          qlayer[o.q0] = L;
           last[o.q0] = (int64_t)seg.size();
           seg.push_back(DInstr{PQ_OP_NOISE, o.q0, 0, 0, kn, kNone, 0, 0});
-          layer.push_back((uint32_t)L);
           noise_only.push_back(1);
           continue;
         }
-        int32_t L = qlayer[o.q0] + 1;
-        if (is_two(o.code)) L = std::max(L, qlayer[o.q1] + 1);
-        qlayer[o.q0] = L;
         last[o.q0] = (int64_t)seg.size();
-        if (is_two(o.code)) {
-          qlayer[o.q1] = L;
-          last[o.q1] = (int64_t)seg.size();
-        }
+        if (is_two(o.code)) last[o.q1] = (int64_t)seg.size();
         uint32_t slot = o.slot;
         if (is_random_flag(o)) {  // the flag is the X part of a noise plane
           const uint32_t kn = nnoise++;
@@ -206,40 +357,27 @@ int compile_program(pq_program_s *p) {
           slot = kSlotPlane | kn;
         }
         seg.push_back(DInstr{o.code, o.q0, is_two(o.code) ? o.q1 : 0, slot, kNone, kNone, 0, 0});
-        layer.push_back((uint32_t)L);
         noise_only.push_back(0);
         if (o.code >= PQ_OP_H && o.code <= PQ_OP_CZ) ++p->ngates;
       }
-      std::vector<uint32_t> order(seg.size());
-      for (uint32_t k = 0; k < order.size(); ++k) order[k] = k;
-      // by layer, and inside a layer by op code (the ops of a layer act on
-      // distinct qubits, so their order does not change the result)
-      std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-        return layer[a] != layer[b] ? layer[a] < layer[b] : seg[a].code < seg[b].code;
-      });
-      uint32_t k = 0;
-      while (k < order.size()) {
-        uint32_t e = k;
-        while (e < order.size() && layer[order[e]] == layer[order[k]]) ++e;
-        for (uint32_t b = k; b < e;) {
-          // batches never straddle a staging chunk of the kernel
-          const uint32_t pos = (uint32_t)(p->instrs.size() - sg.begin) % kProgramStage;
-          const uint32_t len =
-              std::min<uint32_t>({kProgramBatch, e - b, kProgramStage - pos});
-          for (uint32_t u = 0; u < len; ++u) {
-            DInstr I = seg[order[b + u]];
-            I.pad = (u == 0) ? (len | (b == k ? kLayerStart : 0u)) : 0u;
-            p->instrs.push_back(I);
-          }
-          b += len;
+      // more qubits than a row tile of kWindowRows rows holds: windows of
+      // fewer qubits, each its own row-tiled segment
+      const uint32_t cap = window_qubits(p);
+      if (cap && count_qubits(seg, p->n) > cap) {
+        for (const std::vector<uint32_t> &part : window_schedule(seg, p->n, cap)) {
+          std::vector<DInstr> sub;
+          sub.reserve(part.size());
+          for (uint32_t k : part) sub.push_back(seg[k]);
+          emit_gate_segment(p, sub);
         }
-        k = e;
+      } else {
+        emit_gate_segment(p, seg);
       }
-      mark_runs(p, sg);
-      plan_tiling(p, sg);
     }
-    sg.count = (uint32_t)(p->instrs.size() - sg.begin);
-    if (sg.count) p->segs.push_back(sg);
+    if (meas) {
+      sg.count = (uint32_t)(p->instrs.size() - sg.begin);
+      if (sg.count) p->segs.push_back(sg);
+    }
     i = j;
   }
   p->nmeas = nmeas;
@@ -355,6 +493,11 @@ extern "C" int pq_program_layout(pq_program p, uint64_t *segments, uint64_t *til
     for (const DSegment &sg : p->segs) t += (sg.kind == 0 && sg.nq > 0);
     *tiled_segments = t;
   }
+  return PQ_OK;
+}
+
+extern "C" int pq_set_window_qubits(uint32_t qubits) {
+  g_window_qubits.store(qubits);
   return PQ_OK;
 }
 
