@@ -247,9 +247,10 @@ int pq_program_info(pq_program p, uint64_t *gates, uint64_t *measurements,
 int pq_program_layout(pq_program p, uint64_t *segments, uint64_t *tiled_segments);
 /* Most distinct qubits per gate segment for programs created after the call
  * (process-wide): a segment over more qubits is cut into windows of at most
- * this many, each run row-tiled out of LDS.  0 (the default): as many as a
- * 32-row tile holds in the default LDS budget; UINT32_MAX: never cut.  No
- * reference counterpart: the compiler's window size, for tuning and tests. */
+ * this many, each run row-tiled out of LDS.  0 (the default): segments wider
+ * than a 32-row tile holds in the default LDS budget are cut into windows a
+ * 48-row tile holds; UINT32_MAX: never cut.  No reference counterpart: the
+ * compiler's window size, for tuning and tests. */
 int pq_set_window_qubits(uint32_t qubits);
 /* Frame form plan of the op list `ops` on n qubits, as PQ_RUN_FRAME compiles
  * it (host only; PQ_ERR_UNSUPPORTED if the ops hold flagged H/S/CX/CZ).

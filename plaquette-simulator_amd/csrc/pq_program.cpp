@@ -193,28 +193,39 @@ static uint32_t count_qubits(const std::vector<DInstr> &seg, uint64_t n) {
   return c;
 }
 
-// Most qubits whose row tile keeps kWindowRows rows in the default LDS budget
-// (0: windows off).  Wider gate segments are cut into qubit windows.
-static std::atomic<uint32_t> g_window_qubits{0};
-
-static uint32_t window_qubits(const pq_program_s *p) {
-  const uint32_t set = g_window_qubits.load();
-  if (set) return set == ~0u ? 0u : set;
+// Most qubits whose row tile keeps `rows` rows in the default LDS budget.
+static uint32_t tile_qubits(uint64_t n, uint32_t rows) {
   const uint64_t words = kProgramLdsBytes / 8;
-  uint32_t lo = 0, hi = (uint32_t)std::min<uint64_t>(p->n, 0xFFFFu);
-  while (lo < hi) {  // largest nq with program_tile_rows(words, nq) >= kWindowRows
+  uint32_t lo = 0, hi = (uint32_t)std::min<uint64_t>(n, 0xFFFFu);
+  while (lo < hi) {  // largest nq with program_tile_rows(words, nq) >= rows
     const uint32_t mid = (lo + hi + 1) / 2;
-    if (pq::program_tile_rows(words, mid) >= kWindowRows) lo = mid; else hi = mid - 1;
+    if (pq::program_tile_rows(words, mid) >= rows) lo = mid; else hi = mid - 1;
   }
   return lo;
+}
+
+static std::atomic<uint32_t> g_window_qubits{0};
+
+// Window size for gate segments wider than *cut qubits (0: windows off).
+static uint32_t window_qubits(const pq_program_s *p, uint32_t *cut) {
+  const uint32_t set = g_window_qubits.load();
+  if (set) {
+    *cut = set == ~0u ? 0u : set;
+    return *cut;
+  }
+  *cut = tile_qubits(p->n, kWindowCutRows);
+  return tile_qubits(p->n, kWindowRows);
 }
 
 // Cuts a gate segment (ops in program order) into windows of at most `cap`
 // distinct qubits, each a list of op indices whose concatenation keeps every
 // qubit's ops in program order (an op is taken only when it is the next one
-// on each of its qubits).  A window first takes every ready op on qubits it
-// already holds, then the earliest ready op that fits, so windows grow over
-// neighbouring qubits and run several layers on them before moving on.
+// on each of its qubits).  A window takes, among the ready ops, one that needs
+// the fewest qubits it does not hold yet (the earliest of those), while it
+// fits: it grows over neighbouring qubits and runs the layers on them as far
+// as the ops outside let it before it closes.  A ready op's qubits can join
+// the window only through the op itself, so its count of new qubits holds
+// until the window closes.
 static std::vector<std::vector<uint32_t>> window_schedule(const std::vector<DInstr> &seg,
                                                           uint64_t n, uint32_t cap) {
   const uint32_t nops = (uint32_t)seg.size();
@@ -239,38 +250,42 @@ static std::vector<std::vector<uint32_t>> window_schedule(const std::vector<DIns
       if (next_on(q_of(i, t)) != (int64_t)i) return false;
     return true;
   };
-  typedef std::priority_queue<uint32_t, std::vector<uint32_t>, std::greater<uint32_t>> MinHeap;
-  MinHeap others, inside;
   std::vector<char> done(nops, 0), in_win(n, 0);
   std::vector<uint32_t> win_q;
-  for (uint32_t i = 0; i < nops; ++i)
-    if (ready(i)) others.push(i);
-  auto all_in = [&](uint32_t i) {
-    for (uint32_t t = 0; t < nq_of(i); ++t)
-      if (!in_win[q_of(i, t)]) return false;
-    return true;
+  auto fresh = [&](uint32_t i) {
+    uint32_t f = 0;
+    for (uint32_t t = 0; t < nq_of(i); ++t) f += in_win[q_of(i, t)] ? 0u : 1u;
+    return f;
   };
+  // ready ops by the qubits they need beyond the window: 0, 1, 2
+  typedef std::priority_queue<uint32_t, std::vector<uint32_t>, std::greater<uint32_t>> MinHeap;
+  MinHeap heap[3];
+  for (uint32_t i = 0; i < nops; ++i)
+    if (ready(i)) heap[nq_of(i)].push(i);
   std::vector<std::vector<uint32_t>> parts;
   uint32_t left = nops;
   while (left) {
     for (uint32_t q : win_q) in_win[q] = 0;
     win_q.clear();
+    // an empty window: every ready op needs all its qubits
+    std::vector<uint32_t> keep;
+    for (int h = 0; h < 2; ++h)
+      for (; !heap[h].empty(); heap[h].pop())
+        if (!done[heap[h].top()]) keep.push_back(heap[h].top());
+    for (uint32_t i : keep) heap[nq_of(i)].push(i);
     std::vector<uint32_t> part;
     for (;;) {
-      uint32_t i;
-      if (!inside.empty()) {
-        i = inside.top();
-        inside.pop();
-      } else {
-        while (!others.empty() && done[others.top()]) others.pop();
-        if (others.empty()) break;
-        i = others.top();
-        uint32_t fresh = 0;
-        for (uint32_t t = 0; t < nq_of(i); ++t) fresh += in_win[q_of(i, t)] ? 0u : 1u;
-        if (!part.empty() && win_q.size() + fresh > cap) break;
-        others.pop();
+      int64_t pick = -1;
+      for (int h = 0; h < 3 && pick < 0; ++h) {
+        while (!heap[h].empty() && done[heap[h].top()]) heap[h].pop();
+        if (heap[h].empty()) continue;
+        const uint32_t i = heap[h].top();
+        if (!part.empty() && win_q.size() + fresh(i) > cap) continue;
+        heap[h].pop();
+        pick = i;
       }
-      if (done[i]) continue;
+      if (pick < 0) break;
+      const uint32_t i = (uint32_t)pick;
       done[i] = 1;
       --left;
       part.push_back(i);
@@ -282,13 +297,11 @@ static std::vector<std::vector<uint32_t>> window_schedule(const std::vector<DIns
         }
         ++pos[q];
       }
-      // ops this one made ready, and ready ops now wholly inside the window
       for (uint32_t t = 0; t < nq_of(i); ++t) {
         const int64_t jn = next_on(q_of(i, t));
         if (jn < 0) continue;
         const uint32_t j = (uint32_t)jn;
-        if (done[j] || !ready(j)) continue;
-        if (all_in(j)) inside.push(j); else others.push(j);
+        if (!done[j] && ready(j)) heap[fresh(j)].push(j);
       }
     }
     parts.push_back(std::move(part));
@@ -360,10 +373,12 @@ int compile_program(pq_program_s *p) {
         noise_only.push_back(0);
         if (o.code >= PQ_OP_H && o.code <= PQ_OP_CZ) ++p->ngates;
       }
-      // more qubits than a row tile of kWindowRows rows holds: windows of
-      // fewer qubits, each its own row-tiled segment
-      const uint32_t cap = window_qubits(p);
-      if (cap && count_qubits(seg, p->n) > cap) {
+      // more qubits than a row tile of kWindowCutRows rows holds: windows of
+      // at most as many as a kWindowRows-row tile holds, each its own
+      // row-tiled segment
+      uint32_t cut = 0;
+      const uint32_t cap = window_qubits(p, &cut);
+      if (cap && cut && count_qubits(seg, p->n) > cut) {
         for (const std::vector<uint32_t> &part : window_schedule(seg, p->n, cap)) {
           std::vector<DInstr> sub;
           sub.reserve(part.size());

@@ -14,9 +14,11 @@
 constexpr uint32_t kProgramBatch = 8;  // must equal kBatch in pq_kernels.hip
 constexpr uint32_t kProgramThreadsMax = 768;  // largest block of the persistent kernel builds
 constexpr uint32_t kProgramStage = 512;       // must equal kStage there
-// Gate segments with more qubits than a row tile of this many rows can hold
-// (in the default LDS budget) are cut into qubit windows.
-constexpr uint32_t kWindowRows = 32;
+// Gate segments with more qubits than a row tile of kWindowCutRows rows can
+// hold (in the default LDS budget) are cut into qubit windows, each small
+// enough for a tile of kWindowRows rows.
+constexpr uint32_t kWindowCutRows = 32;
+constexpr uint32_t kWindowRows = 48;
 
 struct pq_program_s {
   uint64_t n = 0, W = 0, R = 0;
