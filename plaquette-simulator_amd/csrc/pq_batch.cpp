@@ -154,8 +154,8 @@ extern "C" int pq_batch_set_launch(pq_batch b, uint32_t threads, uint32_t blocks
   if (threads) {
     if (threads % 64 || threads > kProgramThreadsMax)
       return pq_set_error(PQ_ERR_ARG,
-                          "threads must be a multiple of 64, at most 768 (512 when a member "
-                          "has more than 192 qubits: larger blocks are clamped)");
+                          "threads must be a multiple of 64, at most 768 (a block of more "
+                          "than 512 keeps the chains of members of 193..512 qubits in LDS)");
     b->threads = threads;
   }
   b->blocks = blocks;
@@ -258,12 +258,15 @@ static int tiled_items(pq_batch b) {
 
 static int enqueue(pq_batch b) {
   const uint32_t nc = (uint32_t)b->members.size();
-  uint32_t max_n = 0;
-  for (uint32_t c = 0; c < nc; ++c) max_n = std::max<uint32_t>(max_n, b->h_circ[c].a.n);
+  // members whose determined chains the 512-thread build keeps in registers
+  uint32_t chain_n = 0;
+  for (uint32_t c = 0; c < nc; ++c)
+    if (program_max_threads(b->h_circ[c].a.n, b->h_circ[c].a.wide) < kProgramThreadsMax)
+      chain_n = std::max<uint32_t>(chain_n, b->h_circ[c].a.n);
   const bool tiled = b->wtile > 1;
   return launch_batch(b->stream, b->d_circ, nc, tiled ? b->d_titems : b->d_items,
                       tiled ? b->ntitems : b->nitems, b->d_counter, b->threads,
-                      launch_blocks(b), launch_lds(b), b->noise_tasks, max_n,
+                      launch_blocks(b), launch_lds(b), b->noise_tasks, chain_n,
                       b->wtile) == hipSuccess
              ? PQ_OK
              : pq_set_error(PQ_ERR_HIP, "batch launch failed");

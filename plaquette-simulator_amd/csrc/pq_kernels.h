@@ -95,6 +95,9 @@ struct ProgramArgs {
   // 1: X/Z are the identity and not in memory; segment 0 (a row-tiled gate
   // segment over all n qubits) writes its tiles instead of loading them
   uint32_t fresh;
+  // 1: the program has row-tiled gate segments; above 512 qubits it runs in
+  // the 768-thread build (program_max_threads)
+  uint32_t wide;
 };
 // One state of a mixed batch: its program arguments, run parameters and
 // noise descriptors (planes are generated by the batch prologue).
@@ -109,11 +112,14 @@ struct BatchCircuit {
 // wtile > 1 an item stands for wtile consecutive word groups of its circuit
 // from the one it names.  noise_tasks > 0: the noise planes of every circuit
 // first (launch_batch_frame_noise over circ[c].task0 numbering).
+// chain_qubits: the largest member of 193..512 qubits (0: none); such members
+// keep their determined chains in registers when the block has at most 512
+// threads (the large build), in LDS otherwise.
 hipError_t launch_batch(hipStream_t s, const BatchCircuit *circ, uint32_t ncirc,
                         const uint64_t *items, uint64_t nitems,
                         unsigned long long *counter, uint32_t threads,
                         uint32_t blocks, size_t lds_bytes, uint64_t noise_tasks,
-                        uint32_t max_qubits, uint32_t wtile = 1);
+                        uint32_t chain_qubits, uint32_t wtile = 1);
 hipError_t prepare_batch_kernel();
 // Dynamic LDS bytes k_program needs for n qubits and det_waves slabs.
 size_t program_lds_bytes(uint32_t n, uint32_t det_waves);
@@ -135,10 +141,11 @@ hipError_t launch_zero_u32(hipStream_t s, unsigned int *p, uint32_t n);
 // Whether `slices` workgroups per word group for `nwords` groups of n qubits
 // fit the device at once (the cooperative launch's condition).
 bool program_slices_fit(uint32_t n, uint64_t nwords, uint32_t slices, uint32_t threads,
-                        uint64_t lds_bytes);
-// Largest block of the persistent kernel build used for n qubits (768 up to
-// 192 qubits, else 512); launches clamp their block size to it.
-uint32_t program_max_threads(uint32_t n);
+                        uint64_t lds_bytes, bool wide = false);
+// Largest block of the persistent kernel build used for n qubits: 768 up to
+// 192 qubits, 512 for 193..512, and above 512 768 for programs with row-tiled
+// gate segments (`wide`), else 512.  Launches clamp their block size to it.
+uint32_t program_max_threads(uint32_t n, bool wide = false);
 // Rows per tile of a row-tiled gate segment over nq qubits with `lds_words`
 // of dynamic LDS (0: the segment runs in the direct form).
 uint32_t program_tile_rows(uint64_t lds_words, uint32_t nq);

@@ -1613,8 +1613,9 @@ __device__ void program_word(const pq::ProgramArgs &a, uint64_t seed, uint64_t m
 
 // Two builds of the persistent kernels: up to 192 qubits, determined chains
 // keep 3 columns per lane in registers, which leaves room for 768-thread
-// workgroups (12 waves); larger states keep 8 per lane (n <= 512) in
-// 512-thread workgroups.
+// workgroups (12 waves); 193..512 qubits keep 8 per lane in 512-thread
+// workgroups; above 512 the chains' scratch row is in LDS and the 768-thread
+// build runs.
 template <int NREG, int MAXT, bool SLICED>
 __global__ __launch_bounds__(MAXT) void k_program_t(pq::ProgramArgs a,
                                                    const ProgParams *__restrict__ pp) {
@@ -2199,10 +2200,21 @@ hipError_t launch_record_to_int(hipStream_t s, const uint64_t *rec, uint64_t B,
   return hipGetLastError();
 }
 
+// The large build (8 scratch-row columns per lane, 512 threads) serves
+// states of 193..512 qubits, whose determined chains it keeps in registers;
+// the small one (768 threads) serves states up to 192 qubits (3 columns per
+// lane).  Above 512 the chains keep their scratch row in LDS in either build:
+// programs with row-tiled gate segments (`wide`) take the wider block for
+// them, the others (bound by random measurements' row updates, which a wider
+// block slows down) the 512-thread one.
+static bool large_build(uint32_t n, bool wide) {
+  return n > kSmallQubits && (n <= 64u * kLargeRegs || !wide);
+}
+
 // The persistent kernel build for n qubits, one workgroup per word group or
 // sliced word groups.
-static const void *program_kernel(uint32_t n, bool sliced) {
-  if (n <= kSmallQubits)
+static const void *program_kernel(uint32_t n, bool sliced, bool wide) {
+  if (!large_build(n, wide))
     return sliced ? (const void *)k_program_t<kSmallRegs, kSmallThreads, true>
                   : (const void *)k_program_t<kSmallRegs, kSmallThreads, false>;
   return sliced ? (const void *)k_program_t<kLargeRegs, kLargeThreads, true>
@@ -2216,15 +2228,15 @@ hipError_t launch_program(hipStream_t s, const ProgramArgs &a, uint32_t threads,
   if (a.slices > 1) {
     // one workgroup per (word group, slice), all resident at once: the
     // slices wait for one another
-    const void *fn = program_kernel(a.n, true);
-    const uint32_t tb = std::min<uint32_t>(threads, program_max_threads(a.n));
+    const void *fn = program_kernel(a.n, true, a.wide);
+    const uint32_t tb = std::min<uint32_t>(threads, program_max_threads(a.n, a.wide));
     ProgramArgs la = a;
     const ProgParams *pp = (const ProgParams *)params;
     void *args[] = {&la, &pp};
     return hipLaunchCooperativeKernel(fn, dim3((uint32_t)(a.nwords * a.slices)), dim3(tb), args,
                                       (unsigned int)lds, s);
   }
-  if (a.n <= kSmallQubits)
+  if (!large_build(a.n, a.wide))
     hipLaunchKernelGGL((k_program_t<kSmallRegs, kSmallThreads, false>), dim3(blocks),
                        dim3(std::min<uint32_t>(threads, kSmallThreads)), lds, s, a,
                        (const ProgParams *)params);
@@ -2236,8 +2248,8 @@ hipError_t launch_program(hipStream_t s, const ProgramArgs &a, uint32_t threads,
 }
 
 bool program_slices_fit(uint32_t n, uint64_t nwords, uint32_t slices, uint32_t threads,
-                        uint64_t lds_bytes) {
-  const void *fn = program_kernel(n, true);
+                        uint64_t lds_bytes, bool wide) {
+  const void *fn = program_kernel(n, true, wide);
   int dev = 0, cus = 0, per_cu = 0, coop = 0;
   if (hipGetDevice(&dev) != hipSuccess) return false;
   if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
@@ -2245,7 +2257,7 @@ bool program_slices_fit(uint32_t n, uint64_t nwords, uint32_t slices, uint32_t t
     return false;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return false;
-  const uint32_t tb = std::min<uint32_t>(threads, program_max_threads(n));
+  const uint32_t tb = std::min<uint32_t>(threads, program_max_threads(n, wide));
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)tb, (size_t)lds_bytes) !=
       hipSuccess)
     return false;
@@ -2254,8 +2266,8 @@ bool program_slices_fit(uint32_t n, uint64_t nwords, uint32_t slices, uint32_t t
 
 uint32_t program_tile_rows(uint64_t lds_words, uint32_t nq) { return tile_rows(lds_words, nq); }
 
-uint32_t program_max_threads(uint32_t n) {
-  return n <= kSmallQubits ? kSmallThreads : kLargeThreads;
+uint32_t program_max_threads(uint32_t n, bool wide) {
+  return large_build(n, wide) ? kLargeThreads : kSmallThreads;
 }
 
 size_t program_lds_bytes(uint32_t n, uint32_t det_waves) {
@@ -2342,7 +2354,7 @@ hipError_t prepare_kernels(uint32_t n) {
   hipError_t e = ensure_lds_attr((const void *)k_measure, measure_lds_bytes(n));
   for (uint32_t nn : {1u, kSmallQubits + 1})
     for (bool sliced : {false, true})
-      if (e == hipSuccess) e = ensure_lds_attr(program_kernel(nn, sliced), kProgramLdsBudget);
+      if (e == hipSuccess) e = ensure_lds_attr(program_kernel(nn, sliced, false), kProgramLdsBudget);
   return e;
 }
 
@@ -2350,7 +2362,7 @@ hipError_t launch_batch(hipStream_t s, const BatchCircuit *circ, uint32_t ncirc,
                         const uint64_t *items, uint64_t nitems,
                         unsigned long long *counter, uint32_t threads,
                         uint32_t blocks, size_t lds_bytes, uint64_t noise_tasks,
-                        uint32_t max_qubits, uint32_t wtile) {
+                        uint32_t chain_qubits, uint32_t wtile) {
   if (!nitems) return hipSuccess;
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, s, counter);
   hipError_t e = hipGetLastError();
@@ -2359,7 +2371,8 @@ hipError_t launch_batch(hipStream_t s, const BatchCircuit *circ, uint32_t ncirc,
     e = launch_batch_frame_noise(s, circ, ncirc, noise_tasks);
     if (e != hipSuccess) return e;
   }
-  if (max_qubits <= kSmallQubits)
+  // the large build only where a member's chains use it and the block fits it
+  if (!large_build(chain_qubits, false) || threads > kLargeThreads)
     hipLaunchKernelGGL((k_batch_t<kSmallRegs, kSmallThreads>), dim3(blocks),
                        dim3(std::min<uint32_t>(threads, kSmallThreads)), lds_bytes, s, circ,
                        items, nitems, counter, std::max<uint32_t>(wtile, 1));

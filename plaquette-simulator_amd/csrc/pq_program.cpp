@@ -469,7 +469,7 @@ static int create_program(pq_state s, const pq_op *ops, uint64_t nops, hipStream
     }
   }
   // the measurement phases use every wave, so the default is the largest block
-  p->threads = program_max_threads((uint32_t)s->n);
+  p->threads = program_max_threads((uint32_t)s->n, p->tiled);
   p->blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(s->W, 1u << 20));
   if (!as) HIPCHK(prepare_kernels((uint32_t)s->n));
   *out = p.release();
@@ -519,9 +519,9 @@ extern "C" int pq_set_window_qubits(uint32_t qubits) {
 extern "C" int pq_program_set_launch(pq_program p, uint32_t threads, uint32_t blocks) {
   if (!p) return pq_set_error(PQ_ERR_ARG, "null program");
   if (threads) {
-    if (threads % 64 || threads > program_max_threads((uint32_t)p->n))
+    if (threads % 64 || threads > program_max_threads((uint32_t)p->n, p->tiled))
       return pq_set_error(PQ_ERR_ARG, ("threads must be a multiple of 64, at most " +
-                                       std::to_string(program_max_threads((uint32_t)p->n)) +
+                                       std::to_string(program_max_threads((uint32_t)p->n, p->tiled)) +
                                        " for this state size").c_str());
     p->threads = threads;
   }
@@ -599,6 +599,7 @@ ProgramArgs program_args(pq_state s, pq_program p, uint64_t lds_budget) {
   a.tab = s->tab;
   a.recs = s->recs;
   a.planes = p->d_planes;
+  a.wide = p->tiled ? 1u : 0u;
   a.nk = (uint32_t)p->nnoise;
   a.segs = p->d_segs;
   a.instrs = p->d_instrs;
@@ -649,10 +650,11 @@ static int choose_slices(pq_state s, pq_program p, const ProgramArgs &a, bool al
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     S = a.nwords * kAutoSlices <= (uint64_t)cus ? kAutoSlices : 1;
-    while (S > 1 && !program_slices_fit(a.n, a.nwords, S, p->threads, a.lds_words * 8)) S /= 2;
+    while (S > 1 && !program_slices_fit(a.n, a.nwords, S, p->threads, a.lds_words * 8, a.wide))
+      S /= 2;
     (void)hipGetLastError();
     if (S <= 1) return PQ_OK;
-  } else if (!program_slices_fit(a.n, a.nwords, S, p->threads, a.lds_words * 8)) {
+  } else if (!program_slices_fit(a.n, a.nwords, S, p->threads, a.lds_words * 8, a.wide)) {
     (void)hipGetLastError();
     return pq_set_error(PQ_ERR_ARG, ("slices " + std::to_string(S) + " x " +
                                      std::to_string(a.nwords) +

@@ -127,7 +127,7 @@ extern "C" int pq_run_program_static(pq_state s, const pq_op *ops, uint64_t nops
     if (e == hipSuccess) e = put(base, off, p.segq, s->stream, &p.d_segq);
     if (e == hipSuccess) e = put(base, off, p.noise, s->stream, &p.d_noise);
     p.d_planes = p.nnoise ? reinterpret_cast<uint64_t *>(base + off) : nullptr;
-    p.threads = program_max_threads((uint32_t)s->n);
+    p.threads = program_max_threads((uint32_t)s->n, p.tiled);
     p.blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(s->W, 1u << 20));
     if (e == hipSuccess) e = prepare_kernels((uint32_t)s->n);
     if (e == hipSuccess)
