@@ -455,15 +455,17 @@ def bench_serving(args):
 
     # The first rate's requests once more, one at a time on one stream
     # (state allocation + persistent run + sync per request), and their
-    # algorithmic bytes from the engine's work census.
+    # fused-minimum and per-op bytes from the engine's work census.
     first = streams[50.0]
-    abytes = 0
+    abytes = fbytes = 0
     for t_arr, n, shots, seed, ops, gates in first:
         st = ps.CliffordState(n, shots, seed, True)
         pr = ps.Program(st, ops)
         pr.set_census(True)
         pr.run(st, "persistent")
-        abytes += algorithmic_bytes(ops, n, (shots + 63) // 64, pr.read_census())["total"]
+        census = pr.read_census()
+        abytes += algorithmic_bytes(ops, n, (shots + 63) // 64, census)["total"]
+        fbytes += fused_minimum_bytes(ops, n, (shots + 63) // 64, census)["total"]
         del pr, st
     t = time.perf_counter()
     for t_arr, n, shots, seed, ops, gates in first:
@@ -474,10 +476,13 @@ def bench_serving(args):
     alone_s = time.perf_counter() - t
     gs0 = lines[0]["gate_shots"]
     one_at_a_time = {"gate_shots_per_s": gs0 / alone_s, "seconds": alone_s,
-                     "algorithmic_bytes": abytes}
+                     "fused_minimum_bytes": fbytes, "per_op_bytes": abytes}
     for line in lines:
         if line["lambda_per_s"] == 50.0 and line["form"] == "persistent":
-            line["hbm_frac_served"] = abytes / line["span_s"] / (HBM_PEAK_GBS * 1e9)
+            # fused minimum (each gate segment's columns once) over the span,
+            # and the per-op count beside it
+            line["hbm_frac_served"] = fbytes / line["span_s"] / (HBM_PEAK_GBS * 1e9)
+            line["per_op_frac_served"] = abytes / line["span_s"] / (HBM_PEAK_GBS * 1e9)
     out = {
         "metric": "serving: p50/p99 latency and aggregate gate-shots/s under Poisson load",
         "value": max((l["gate_shots_per_s"] for l in lines
