@@ -270,7 +270,9 @@ int pq_set_window_qubits(uint32_t qubits);
 int pq_frame_plan(uint64_t n, const pq_op *ops, uint64_t nops, uint64_t info[16],
                   uint32_t *stream, uint32_t *chunks, uint32_t *preload, uint64_t *x,
                   uint64_t *z);
-/* Launch shape of the persistent kernel (0 keeps the current value). */
+/* Launch shape of the persistent kernel (0 keeps the current value; threads
+ * a multiple of 64, at most 768, or 512 for states of 193..512 qubits and for
+ * states above 512 qubits whose program has no row-tiled gate segment). */
 int pq_program_set_launch(pq_program p, uint32_t threads, uint32_t blocks);
 /* Workgroups per 64-shot word group in persistent runs: 0 chooses (states of
  * 512+ qubits with at most half as many word groups as the device has
@@ -338,9 +340,11 @@ int pq_batch_destroy(pq_batch b);
 int pq_batch_add(pq_batch b, pq_state s, pq_program p);
 int pq_batch_run(pq_batch b, int form);
 int pq_batch_wait(pq_batch b);
-/* Launch shape: threads per workgroup (multiple of 64, <= 512), workgroups
- * (0 = CUs x workgroups per CU) and the dynamic-LDS budget per workgroup
- * (40 KB..158 KB; it sets workgroups per CU and parallel determined waves). */
+/* Launch shape: threads per workgroup (multiple of 64, <= 768; above 512 the
+ * determined chains of members of 193..512 qubits keep their scratch row in
+ * LDS instead of registers), workgroups (0 = CUs x workgroups per CU) and the
+ * dynamic-LDS budget per workgroup (40 KB..158 KB; it sets workgroups per CU
+ * and parallel determined waves). */
 int pq_batch_set_launch(pq_batch b, uint32_t threads, uint32_t blocks,
                         uint64_t lds_budget);
 /* Times candidate launch shapes on the batch and keeps the fastest, then the
