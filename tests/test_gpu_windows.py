@@ -93,6 +93,52 @@ def test_windows_in_a_batch_and_sliced(ps):
     check(c, orc)
 
 
+def every_op_program(n, seed):
+    """Layers of every op kind: gates, Paulis, record-flagged and randomly
+    flagged gates (one and two qubits), noise fused and alone, measurements."""
+    from plaquette_simulator import workloads as W
+    rng = np.random.default_rng(seed)
+    b = W.OpList()
+    for q in range(n):
+        b.h(q)
+    for _ in range(4):
+        for _ in range(3 * n):
+            k = int(rng.integers(0, 7))
+            q, t = (int(x) for x in rng.choice(n, 2, replace=False))
+            if k == 0:
+                b.cx(q, t)
+            elif k == 1:
+                b.cz(q, t)
+            elif k == 2:
+                b.gate(int(rng.choice([W.OP_S, W.OP_X, W.OP_Y, W.OP_Z, W.OP_H])), q)
+            elif k == 3 and b.nrec:
+                b.flagged(int(rng.choice([W.OP_H_FLAG, W.OP_S_FLAG, W.OP_X_FLAG, W.OP_Y_FLAG,
+                                          W.OP_Z_FLAG])), q, int(rng.integers(0, b.nrec)))
+            elif k == 4 and b.nrec:
+                b.flagged(int(rng.choice([W.OP_CX_FLAG, W.OP_CZ_FLAG])), q,
+                          int(rng.integers(0, b.nrec)), t)
+            elif k == 5:
+                b.with_prob(int(rng.choice([W.OP_H, W.OP_S, W.OP_CX])), q, 0.3, t)
+            else:
+                b.noise(q, 0.05, 0.02, 0.05)
+        for q in rng.choice(n, n // 4, replace=False):
+            b.measure(int(q))
+    return b.array()
+
+
+@pytest.mark.parametrize("cap", [5, 23])
+def test_windows_every_op_kind(ps, cap):
+    n, B, seed = 40, 192, 13
+    ops = every_op_program(n, 2)
+    say(f"[case] every op kind n{n}, windows of {cap} qubits, persistent and graph")
+    for form in ("persistent", "graph"):
+        st, prog = segments(ps, n, B, seed, ops, cap)
+        prog.run(st, form)
+        orc = O.OracleState(n, B, seed)
+        orc.run_program(ops.astype(np.int32))
+        check(st, orc)
+
+
 def test_surface_d13_cut_at_default_size(ps):
     from plaquette_simulator import workloads as W
     ops, n = W.surface_code_memory_z(13, 2, 0.005)
