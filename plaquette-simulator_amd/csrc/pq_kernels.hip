@@ -732,8 +732,10 @@ constexpr uint32_t kMeasWindow0 = 8;  // first window of a run
 // planes, flags) are staged in LDS by the whole workgroup in one round trip;
 // after that every batch costs exactly one global round trip (its column
 // loads), and the stores are fire-and-forget.
-constexpr uint32_t kStage = 512;
+constexpr uint32_t kStage = 448;
 constexpr uint32_t kStageWords = kStage * (sizeof(DInstr) / 8 + 4);
+// Row-tiled segments stage each op in one word (TileOp) beside its masks.
+constexpr uint32_t kTileStageWords = kStage * 5;
 
 // Stages instructions [cb, cb+cn) of a gate segment and the per-word masks
 // they need (fused noise planes of qa and qb, or the flag of a flagged op)
@@ -742,9 +744,18 @@ constexpr uint32_t kStageWords = kStage * (sizeof(DInstr) / 8 + 4);
 constexpr uint32_t kOpNoisy = 1, kOpIdentity = 2;
 // the same flags in a row-tiled op's broadcast word of tile offsets
 constexpr uint32_t kTileNoisyBit = 1u << 15, kTileIdentityBit = 1u << 31;
-__device__ __forceinline__ void stage_chunk(const pq::ProgramArgs &a, uint64_t w,
-                                            uint32_t cb, uint32_t cn, DInstr *s_ins,
-                                            uint64_t *s_msk) {
+// A staged op of a row-tiled segment: `code` is the op code (bits 0-7), its
+// kOp* flags (bits 8-9), the length of the run it starts (bits 10-21) and
+// kLayerStart (bit 31, the op opens a layer); `t0` its local columns.
+struct TileOp {
+  uint32_t code, t0;
+};
+constexpr uint32_t kTileRunShift = 10;
+
+template <bool kCompact>
+__device__ __forceinline__ void stage_chunk_t(const pq::ProgramArgs &a, uint64_t w,
+                                              uint32_t cb, uint32_t cn, void *s_ops,
+                                              uint64_t *s_msk) {
   const uint64_t W = a.W;
   cinstr_ptr ins = (cinstr_ptr)a.instrs;
   // this word group's planes, group-major: plane k's X and Z words at 2k, 2k+1
@@ -773,14 +784,29 @@ __device__ __forceinline__ void stage_chunk(const pq::ProgramArgs &a, uint64_t w
     const bool flagged = I.code >= PQ_OP_H_FLAG && I.code != PQ_OP_NOISE;
     const bool noisy = (m0 | m1) != 0 || (two && (m2 | m3) != 0);
     const bool identity = !noisy && (I.code == PQ_OP_NOISE || (flagged && m2 == 0));
-    DInstr J = I;
-    J.slot = (noisy ? kOpNoisy : 0u) | (identity ? kOpIdentity : 0u);
-    s_ins[t] = J;
+    const uint32_t flags = (noisy ? kOpNoisy : 0u) | (identity ? kOpIdentity : 0u);
+    if constexpr (kCompact) {
+      TileOp J;
+      J.code = I.code | (flags << 8) | (((I.pad >> kRunShift) & kRunMask) << kTileRunShift) |
+               (I.pad & kLayerStart);
+      J.t0 = I.t0;
+      reinterpret_cast<TileOp *>(s_ops)[t] = J;
+    } else {
+      DInstr J = I;
+      J.slot = flags;
+      reinterpret_cast<DInstr *>(s_ops)[t] = J;
+    }
     s_msk[4 * t + 0] = m0;
     s_msk[4 * t + 1] = m1;
     s_msk[4 * t + 2] = m2;
     s_msk[4 * t + 3] = m3;
   }
+}
+
+__device__ __forceinline__ void stage_chunk(const pq::ProgramArgs &a, uint64_t w,
+                                            uint32_t cb, uint32_t cn, DInstr *s_ins,
+                                            uint64_t *s_msk) {
+  stage_chunk_t<false>(a, w, cb, cn, s_ins, s_msk);
 }
 
 __device__ void run_gate_segment(uint64_t *__restrict__ T, uint64_t w,
@@ -914,7 +940,7 @@ __host__ __device__ __forceinline__ uint64_t tile_words(uint32_t nq, uint32_t co
 }
 __host__ __device__ __forceinline__ uint32_t tile_rows(uint64_t lds_words, uint32_t nq) {
   if (!nq) return 0;
-  const uint64_t fixed = kStageWords + (nq + 1) / 2;
+  const uint64_t fixed = kTileStageWords + (nq + 1) / 2;
   if (lds_words <= fixed + kDmaWords + 2ull * nq) return 0;
   // TR r words + 2 nq (TR + 1) tile words + up to a chunk of rounding
   const uint64_t fit = (lds_words - fixed - kDmaWords - 2ull * nq) / (2ull * nq + 1);
@@ -1132,16 +1158,16 @@ __device__ void run_gate_segment_tiled(uint64_t *__restrict__ T, uint64_t w,
   const uint32_t n = a.n;
   const uint32_t nq = sg.nq;
   const uint32_t TRp = TR + 1;  // words per tile column
-  DInstr *s_ins = reinterpret_cast<DInstr *>(dyn);
-  uint64_t *s_msk = dyn + kStage * (sizeof(DInstr) / 8);
-  uint32_t *s_q = reinterpret_cast<uint32_t *>(dyn + kStageWords);
-  uint64_t *s_r = dyn + kStageWords + (nq + 1) / 2;
+  TileOp *s_ins = reinterpret_cast<TileOp *>(dyn);
+  uint64_t *s_msk = dyn + kStage;
+  uint32_t *s_q = reinterpret_cast<uint32_t *>(dyn + kTileStageWords);
+  uint64_t *s_r = dyn + kTileStageWords + (nq + 1) / 2;
   uint64_t *tile = s_r + TR;
   lds_u64 *tl = (lds_u64 *)tile;
   const uint32_t seg_end = sg.begin + sg.count;
   const bool one_chunk = sg.count <= kStage;
   for (uint32_t j = threadIdx.x; j < nq; j += blockDim.x) s_q[j] = a.segq[sg.qoff + j];
-  if (one_chunk) stage_chunk(a, w, sg.begin, sg.count, s_ins, s_msk);
+  if (one_chunk) stage_chunk_t<true>(a, w, sg.begin, sg.count, s_ins, s_msk);
   const uint32_t lane = lane_id();
   // tiles of 32+ rows give each wave one op (wave-uniform control flow)
   const uint32_t P = TR >= 32 ? 1 : 64 / TR;
@@ -1204,12 +1230,12 @@ __device__ void run_gate_segment_tiled(uint64_t *__restrict__ T, uint64_t w,
     for (uint32_t cb = sg.begin; cb < seg_end; cb += kStage) {
       const uint32_t cn = min(kStage, seg_end - cb);
       if (!one_chunk) {
-        stage_chunk(a, w, cb, cn, s_ins, s_msk);
+        stage_chunk_t<true>(a, w, cb, cn, s_ins, s_msk);
         __syncthreads();
       }
       uint32_t k = 0;
       while (k < cn) {
-        const uint32_t e = k + ((s_ins[k].pad >> kRunShift) & kRunMask);
+        const uint32_t e = k + ((s_ins[k].code >> kTileRunShift) & kRunMask);
         if (P == 1) {
           // One op per wave at a time, wave-uniform.  The wave's ops of the
           // run (k + wv, k + wv + S, ...) are fetched in one LDS round trip,
@@ -1222,10 +1248,11 @@ __device__ void run_gate_segment_tiled(uint64_t *__restrict__ T, uint64_t w,
             uint32_t my_code = 0, my_t0 = 0;
             if (base + lane < nmine) {
               const uint32_t idx = k + wv + (base + lane) * S;
-              my_code = s_ins[idx].code;
+              const uint32_t cw = s_ins[idx].code;
+              my_code = cw & 0xFFu;
               // local columns -> tile word offsets (a tile holds fewer
               // than 2^15 words: LDS is 160 KB), flags in bits 15 and 31
-              const uint32_t t = s_ins[idx].t0, f = s_ins[idx].slot;
+              const uint32_t t = s_ins[idx].t0, f = (cw >> 8) & 3u;
               my_t0 = (2u * (t & 0xFFFFu) * TRp) | ((2u * (t >> 16) * TRp) << 16) |
                       ((f & kOpNoisy) ? kTileNoisyBit : 0u) |
                       ((f & kOpIdentity) ? kTileIdentityBit : 0u);
@@ -1262,7 +1289,7 @@ __device__ void run_gate_segment_tiled(uint64_t *__restrict__ T, uint64_t w,
               idxs[u] = i0 + u * S;
               code[u] = 0;
               if (idxs[u] < e) {
-                code[u] = s_ins[idxs[u]].code;
+                code[u] = s_ins[idxs[u]].code & 0xFFu;
                 const uint32_t t0 = s_ins[idxs[u]].t0;
                 pa[u] = Tl + 2 * (t0 & 0xFFFFu) * TRp;
                 pb[u] = Tl + 2 * (t0 >> 16) * TRp;
@@ -1284,7 +1311,7 @@ __device__ void run_gate_segment_tiled(uint64_t *__restrict__ T, uint64_t w,
         }
         // runs of one layer act on distinct qubits: a barrier only before
         // a new layer and at the end of the staged chunk
-        if (e == cn || (s_ins[e].pad & kLayerStart)) __syncthreads();
+        if (e == cn || (s_ins[e].code & kLayerStart)) __syncthreads();
         k = e;
       }
     }

@@ -13,7 +13,7 @@
 
 constexpr uint32_t kProgramBatch = 8;  // must equal kBatch in pq_kernels.hip
 constexpr uint32_t kProgramThreadsMax = 768;  // largest block of the persistent kernel builds
-constexpr uint32_t kProgramStage = 512;       // must equal kStage there
+constexpr uint32_t kProgramStage = 448;       // must equal kStage there
 // Gate segments with more qubits than a row tile of kWindowCutRows rows can
 // hold (in the default LDS budget) are cut into qubit windows, each small
 // enough for a tile of kWindowRows rows.
