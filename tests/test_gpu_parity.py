@@ -237,7 +237,7 @@ def test_random_programs(ps, W, form, seed, n, B):
                                                  (16, 300, 70, 2000, 0.002)])
 def test_long_gate_segments(ps, W, form, seed, n, B, nops, pmeas):
     """Long runs of gates (flags, noise, CZ included) between measurements:
-    the compiler marks them row-tiled, segments over 512 instructions are
+    the compiler marks them row-tiled, segments over 448 instructions are
     staged in several chunks, and n=300 and n=600 leave room for so few rows
     per tile that one wave holds several ops (lanes with different ops)."""
     rng = np.random.default_rng(seed)
