@@ -342,10 +342,21 @@ __device__ void measure_word(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
   const uint64_t *xq = T + (uint64_t)q * Ru;
 
   // ---- 1. pivot groups ----
+  // When the block has a thread per row (one workgroup per group, 2n <= NT),
+  // thread t reads x_q of row t once: the pivot scan takes the stabilizer
+  // rows from it and the rowsums their masks.  The random shots' rewrites
+  // change x_q only in the shots of their own group, and every later use
+  // masks it by another group's shots (or by D), so the value stays valid.
+  const bool own_row = S == 1 && 2 * n <= NT;
+  const uint64_t xv = (own_row && tid < 2 * n) ? xq[tid] : 0ull;
   const uint32_t cp = (n + NT - 1) / NT;
   const uint32_t p0 = min(n, tid * cp), p1 = min(n, p0 + cp);
   uint64_t acc = 0;
-  for (uint32_t p = p0; p < p1; ++p) acc |= xq[n + p];
+  if (own_row) {
+    acc = tid >= n ? xv : 0ull;
+  } else {
+    for (uint32_t p = p0; p < p1; ++p) acc |= xq[n + p];
+  }
   if (tid == 0) {
     sh.ngroups = 0;
     sh.krows = 0;
@@ -353,7 +364,14 @@ __device__ void measure_word(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
   uint64_t rm;
   uint64_t before = block_excl_or(acc, sh.scan, &rm);
   if (force_determined) rm = 0;
-  if (S == 1) {
+  if (own_row) {
+    const uint64_t first = acc & ~before;
+    if (first && !force_determined) {
+      const uint32_t g = atomicAdd(&sh.ngroups, 1u);
+      sh.gp[g] = tid - n;
+      sh.gm[g] = first;
+    }
+  } else if (S == 1) {
     for (uint32_t p = p0; p < p1 && !force_determined; ++p) {
       const uint64_t v = xq[n + p];
       const uint64_t first = v & ~before;
@@ -446,9 +464,12 @@ __device__ void measure_word(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
     uint32_t my_rows = 0;
     for (uint32_t i = slice * NT + tid; i < 2 * n; i += NT * S) {
       if (i == p || i == n + p) continue;
-      const uint64_t mi = M & xq[i];
+      const uint64_t mi = M & (own_row ? xv : xq[i]);
       if (!mi) continue;
       ++my_rows;
+      // the phase word is read with the first source columns, not after
+      // the row's stores
+      const uint64_t ri = T[(uint64_t)(2 * n) * Ru + i];
       uint64_t c0 = 0, c1 = 0;
       // eight source columns at a time: all loads issued before any store
       for (uint32_t k0 = 0; k0 < nl; k0 += 8) {
@@ -472,7 +493,7 @@ __device__ void measure_word(uint64_t *__restrict__ T, uint64_t w, uint32_t n,
           }
         }
       }
-      T[(uint64_t)(2 * n) * Ru + i] ^= mi & (rsrc ^ c1);
+      T[(uint64_t)(2 * n) * Ru + i] = ri ^ (mi & (rsrc ^ c1));
     }
     if (census && my_rows) atomicAdd(&sh.krows, my_rows);
     __syncthreads();
