@@ -327,10 +327,12 @@ def test_flag_and_prob_methods(ps, W):
     assert_same(st, orc)
 
 
-def test_divergent_pivots(ps, W):
+@pytest.mark.parametrize("n,B", [(10, 256), (400, 128)])
+def test_divergent_pivots(ps, W, n, B):
     """Per-shot flagged H makes X/Z differ across the 64 lanes of a word, so
-    one measurement sees many pivot groups and mixed random/determined."""
-    n, B = 10, 256
+    one measurement sees many pivot groups and mixed random/determined.  At
+    n=10 a random measurement has a thread per tableau row; at n=400 (800
+    rows, more than a workgroup's threads) each thread walks several rows."""
     b = W.OpList()
     for q in range(n):
         b.h(q)
@@ -340,7 +342,7 @@ def test_divergent_pivots(ps, W):
         b.cx(k + 1, k + 5)
         m = b.measure(k + 2)
         b.flagged(W.OP_S_FLAG, (k + 3) % n, m)
-    for q in range(n):
+    for q in range(min(n, 40)):
         b.measure(q)
     ops = b.array()
     for form in FORMS:
